@@ -1,0 +1,760 @@
+// frep_api.hip -- host driver behind the C ABI of include/frep.h.
+//
+// Owns the HIP stream, device buffers and handle tables.  A voxelization is
+// one init kernel plus one persistent kernel per tree level, enqueued back to
+// back; the host reads the counter block once at the end.  If a level ran
+// out of node or tape-pool capacity the kernels flag it (and later levels
+// return at once), the host grows the buffers to the sizes the counters
+// requested and runs again.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/frep.h"
+#include "frep_internal.hpp"
+
+namespace frep {
+uint32_t point_k_used(uint32_t maxs, uint32_t dim);
+}
+
+using namespace frep;
+
+namespace {
+
+constexpr uint32_t kRootOff = 16;      // words of slack before the root tape
+constexpr uint32_t kPoolPad = 64;      // words of slack after the pool end
+constexpr uint64_t kMaxPoolWords = 0xFFFFFF00ull;
+
+struct TapeObj {
+    std::vector<uint32_t> words;       // [n, instrs]
+    std::vector<float> consts;
+    uint32_t slot_count = 0, maxs = 0, n_mm = 0;
+    uint32_t* d_words = nullptr;
+    float* d_consts = nullptr;
+};
+
+struct TreeObj {
+    frep_grid grid{};
+    bool built = false;
+    uint32_t* d_nodes[kMaxLevels] = {};
+    uint32_t node_cap[kMaxLevels] = {};
+    uint32_t ns[kMaxLevels] = {};
+    float cs[kMaxLevels] = {};
+    uint32_t* d_pool = nullptr;
+    uint64_t pool_cap = 0;             // usable words (allocation has kPoolPad more)
+    float* d_consts = nullptr;
+    uint32_t consts_cap = 0;
+    uint32_t* d_counters = nullptr;
+    uint32_t* h_counters = nullptr;    // pinned mirror
+    uint32_t slot_count = 0, maxs = 0;
+    uint32_t retries = 0;
+};
+
+uint32_t maxs_for(uint32_t slot_count) {
+    uint32_t m = 8;
+    while (m < slot_count) m *= 2;
+    return m;
+}
+
+}  // namespace
+
+struct frep_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cu_count = 0;
+    uint32_t level_grid = 0;
+    std::string err;
+    std::mutex mu;
+    std::vector<std::unique_ptr<TapeObj>> tapes;
+    std::vector<std::unique_ptr<TreeObj>> trees;
+    uint4* d_scratch = nullptr;
+    size_t scratch_bytes = 0;
+    bool timing = false;
+    hipEvent_t ev[kMaxLevels + 4] = {};
+    frep_timing last{};
+    // render output staging
+    void* d_out = nullptr;
+    size_t out_bytes = 0;
+};
+
+namespace {
+
+frep_status fail(frep_ctx* c, frep_status s, const std::string& msg) {
+    if (c) c->err = msg;
+    return s;
+}
+
+#define HIP_TRY(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? FREP_OUT_OF_MEMORY : FREP_DEVICE, \
+                        std::string(#call) + ": " + hipGetErrorString(e_));            \
+    } while (0)
+
+TapeObj* get_tape(frep_ctx* ctx, frep_tape_t h) {
+    if (h == 0 || h > ctx->tapes.size()) return nullptr;
+    return ctx->tapes[h - 1].get();
+}
+
+TreeObj* get_tree(frep_ctx* ctx, frep_tree_t h) {
+    if (h == 0 || h > ctx->trees.size()) return nullptr;
+    return ctx->trees[h - 1].get();
+}
+
+// Returns 0 when the tape is well formed (same checks as the oracle's
+// orc_tape_validate), otherwise a code naming the first problem.
+int validate_tape(const uint32_t* w, uint32_t n_words, uint32_t n_consts, uint32_t slots,
+                  uint32_t format) {
+    if (n_words < 1) return 1;
+    const uint32_t n = w[0];
+    if ((uint64_t)n + 1 > n_words) return 2;
+    if (slots < 4 || slots > 256) return 3;
+    if (format == FREP_TAPE_FORMAT_REF && n_consts > 256) return 9;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t x = w[1 + k];
+        const uint32_t op = x & 0xFFu, o = (x >> 8) & 0xFFu, a = (x >> 16) & 0xFFu, b = x >> 24;
+        if (op > OP_COPY) return 4;
+        if (o >= slots) return 5;
+        if (op == OP_CONST) {
+            if ((a | (b << 8)) >= n_consts) return 6;
+            continue;
+        }
+        if (a >= slots) return 7;
+        const bool binary = op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV ||
+                            op == OP_MIN || op == OP_MAX;
+        if (binary && b >= slots) return 8;
+    }
+    return 0;
+}
+
+frep_status check_grid(frep_ctx* ctx, const frep_grid* g) {
+    if (!g) return fail(ctx, FREP_INVALID_ARG, "grid is null");
+    if (g->n_levels < 1 || g->n_levels > kMaxLevels)
+        return fail(ctx, FREP_INVALID_ARG, "n_levels must be in 1..8");
+    uint64_t fine = 1;
+    for (uint32_t L = 0; L < g->n_levels; ++L) {
+        const uint32_t d = g->dims[L];
+        if (d < 4 || d > 32 || (d * d * d) % 64 != 0)
+            return fail(ctx, FREP_UNSUPPORTED, "dims must satisfy 4 <= d <= 32, d^3 % 64 == 0");
+        fine *= d;
+    }
+    if (fine > (1u << 20))
+        return fail(ctx, FREP_UNSUPPORTED, "finest resolution above 2^20 per axis");
+    if (!(g->world_size > 0.0f) || !std::isfinite(g->world_size))
+        return fail(ctx, FREP_INVALID_ARG, "world_size must be positive and finite");
+    return FREP_OK;
+}
+
+frep_status ensure_scratch(frep_ctx* ctx, size_t bytes) {
+    if (bytes <= ctx->scratch_bytes) return FREP_OK;
+    if (ctx->d_scratch) HIP_TRY(hipFree(ctx->d_scratch));
+    ctx->d_scratch = nullptr;
+    ctx->scratch_bytes = 0;
+    HIP_TRY(hipMalloc(&ctx->d_scratch, bytes));
+    ctx->scratch_bytes = bytes;
+    return FREP_OK;
+}
+
+void free_tree_buffers(TreeObj* t) {
+    for (uint32_t L = 0; L < kMaxLevels; ++L) {
+        if (t->d_nodes[L]) (void)hipFree(t->d_nodes[L]);
+        t->d_nodes[L] = nullptr;
+        t->node_cap[L] = 0;
+    }
+    if (t->d_pool) (void)hipFree(t->d_pool);
+    t->d_pool = nullptr;
+    t->pool_cap = 0;
+    if (t->d_consts) (void)hipFree(t->d_consts);
+    t->d_consts = nullptr;
+    t->consts_cap = 0;
+}
+
+frep_status ensure_level(frep_ctx* ctx, TreeObj* t, uint32_t L, uint32_t cap) {
+    if (t->d_nodes[L] && t->node_cap[L] >= cap) return FREP_OK;
+    if (t->d_nodes[L]) HIP_TRY(hipFree(t->d_nodes[L]));
+    t->d_nodes[L] = nullptr;
+    t->node_cap[L] = 0;
+    HIP_TRY(hipMalloc(&t->d_nodes[L], (size_t)cap * t->ns[L] * sizeof(uint32_t)));
+    t->node_cap[L] = cap;
+    return FREP_OK;
+}
+
+frep_status ensure_pool(frep_ctx* ctx, TreeObj* t, uint64_t words) {
+    if (t->d_pool && t->pool_cap >= words) return FREP_OK;
+    if (t->d_pool) HIP_TRY(hipFree(t->d_pool));
+    t->d_pool = nullptr;
+    t->pool_cap = 0;
+    HIP_TRY(hipMalloc(&t->d_pool, (size_t)(words + kPoolPad) * sizeof(uint32_t)));
+    t->pool_cap = words;
+    return FREP_OK;
+}
+
+uint32_t clamp_cap(uint64_t v, uint64_t bound) {
+    v = std::min(v, bound);
+    v = std::min<uint64_t>(v, 0x7FFFFFFFull);
+    return (uint32_t)std::max<uint64_t>(v, 1);
+}
+
+}  // namespace
+
+extern "C" {
+
+frep_status frep_init(int device, frep_ctx** out) {
+    if (!out) return FREP_INVALID_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return FREP_DEVICE;
+    if (device < 0 || device >= n) return FREP_INVALID_ARG;
+    std::unique_ptr<frep_ctx> c(new frep_ctx());
+    frep_ctx* ctx = c.get();
+    c->device = device;
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    c->cu_count = (uint32_t)prop.multiProcessorCount;
+    c->level_grid = c->cu_count * 4;
+    HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+    *out = c.release();
+    return FREP_OK;
+}
+
+void frep_shutdown(frep_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto& t : ctx->tapes) {
+        if (!t) continue;
+        if (t->d_words) (void)hipFree(t->d_words);
+        if (t->d_consts) (void)hipFree(t->d_consts);
+    }
+    for (auto& t : ctx->trees) {
+        if (!t) continue;
+        free_tree_buffers(t.get());
+        if (t->d_counters) (void)hipFree(t->d_counters);
+        if (t->h_counters) (void)hipHostFree(t->h_counters);
+    }
+    if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
+    if (ctx->d_out) (void)hipFree(ctx->d_out);
+    for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char* frep_last_error(const frep_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+frep_status frep_set_timing(frep_ctx* ctx, int enable) {
+    if (!ctx) return FREP_INVALID_ARG;
+    ctx->timing = enable != 0;
+    return FREP_OK;
+}
+
+frep_status frep_get_timing(frep_ctx* ctx, frep_timing* out) {
+    if (!ctx || !out) return FREP_INVALID_ARG;
+    *out = ctx->last;
+    return FREP_OK;
+}
+
+frep_status frep_device_info(frep_ctx* ctx, uint32_t* cu_count, uint32_t* level_grid) {
+    if (!ctx) return FREP_INVALID_ARG;
+    if (cu_count) *cu_count = ctx->cu_count;
+    if (level_grid) *level_grid = ctx->level_grid;
+    return FREP_OK;
+}
+
+frep_status frep_tape_upload(frep_ctx* ctx, const uint32_t* words, uint32_t n_words,
+                             const float* consts, uint32_t n_consts, uint32_t slot_count,
+                             uint32_t format, frep_tape_t* out) {
+    if (!ctx || !words || !out) return fail(ctx, FREP_INVALID_ARG, "null argument");
+    if (n_consts && !consts) return fail(ctx, FREP_INVALID_ARG, "consts is null");
+    if (format != FREP_TAPE_FORMAT_REF && format != FREP_TAPE_FORMAT_WIDE)
+        return fail(ctx, FREP_INVALID_ARG, "unknown tape format");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const int v = validate_tape(words, n_words, n_consts, slot_count, format);
+    if (v) return fail(ctx, FREP_BAD_TAPE, "malformed tape (code " + std::to_string(v) + ")");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<TapeObj> t(new TapeObj());
+    const uint32_t n = words[0];
+    t->words.assign(words, words + n + 1);
+    t->consts.assign(consts, consts + n_consts);
+    t->slot_count = slot_count;
+    t->maxs = maxs_for(slot_count);
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint32_t op = words[1 + k] & 0xFFu;
+        t->n_mm += (op == OP_MIN || op == OP_MAX);
+    }
+    // padded device copies: scalar loads may read a few words past the end
+    HIP_TRY(hipMalloc(&t->d_words, (size_t)(n + 1 + kPoolPad) * 4));
+    HIP_TRY(hipMemset(t->d_words, 0, (size_t)(n + 1 + kPoolPad) * 4));
+    HIP_TRY(hipMemcpy(t->d_words, words, (size_t)(n + 1) * 4, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&t->d_consts, (size_t)(n_consts + kPoolPad) * 4));
+    if (n_consts)
+        HIP_TRY(hipMemcpy(t->d_consts, consts, (size_t)n_consts * 4, hipMemcpyHostToDevice));
+    ctx->tapes.push_back(std::move(t));
+    *out = (frep_tape_t)ctx->tapes.size();
+    return FREP_OK;
+}
+
+frep_status frep_tape_free(frep_ctx* ctx, frep_tape_t h) {
+    if (!ctx) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TapeObj* t = get_tape(ctx, h);
+    if (!t) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (t->d_words) HIP_TRY(hipFree(t->d_words));
+    if (t->d_consts) HIP_TRY(hipFree(t->d_consts));
+    ctx->tapes[h - 1].reset();
+    return FREP_OK;
+}
+
+frep_status frep_tree_create(frep_ctx* ctx, frep_tree_t* out) {
+    if (!ctx || !out) return fail(ctx, FREP_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<TreeObj> t(new TreeObj());
+    HIP_TRY(hipMalloc(&t->d_counters, CNT_WORDS * 4));
+    HIP_TRY(hipHostMalloc(&t->h_counters, CNT_WORDS * 4, hipHostMallocDefault));
+    std::memset(t->h_counters, 0, CNT_WORDS * 4);
+    ctx->trees.push_back(std::move(t));
+    *out = (frep_tree_t)ctx->trees.size();
+    return FREP_OK;
+}
+
+frep_status frep_tree_free(frep_ctx* ctx, frep_tree_t h) {
+    if (!ctx) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = get_tree(ctx, h);
+    if (!t) return fail(ctx, FREP_INVALID_ARG, "unknown tree handle");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    free_tree_buffers(t);
+    if (t->d_counters) HIP_TRY(hipFree(t->d_counters));
+    if (t->h_counters) HIP_TRY(hipHostFree(t->h_counters));
+    ctx->trees[h - 1].reset();
+    return FREP_OK;
+}
+
+frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, frep_tree_t h) {
+    if (!ctx) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TapeObj* tp = get_tape(ctx, th);
+    TreeObj* t = get_tree(ctx, h);
+    if (!tp) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
+    if (!t) return fail(ctx, FREP_INVALID_ARG, "unknown tree handle");
+    const frep_status gs = check_grid(ctx, grid);
+    if (gs != FREP_OK) return gs;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t nl = grid->n_levels;
+    const bool same_shape = t->built && t->grid.n_levels == nl &&
+                            std::equal(grid->dims, grid->dims + nl, t->grid.dims);
+    if (!same_shape) {
+        // record sizes change with the dims: drop the level buffers
+        for (uint32_t L = 0; L < kMaxLevels; ++L) {
+            if (t->d_nodes[L]) HIP_TRY(hipFree(t->d_nodes[L]));
+            t->d_nodes[L] = nullptr;
+            t->node_cap[L] = 0;
+        }
+    }
+    t->built = false;
+    t->grid = *grid;
+    t->slot_count = tp->slot_count;
+    t->maxs = tp->maxs;
+    // cell sizes by sequential float division (voxelizer.cpp:210-216)
+    float c = grid->world_size;
+    for (uint32_t L = 0; L < nl; ++L) {
+        c /= (float)grid->dims[L];
+        t->cs[L] = c;
+        t->ns[L] = node_words(grid->dims[L], L + 1 == nl);
+    }
+    // capacities: upper bound n_{L+1} <= n_L * dim_L^3, first guess capped
+    uint64_t bound = 1;
+    uint32_t want[kMaxLevels];
+    for (uint32_t L = 0; L < nl; ++L) {
+        const uint64_t guess = L == 0 ? 1 : std::max<uint64_t>(t->node_cap[L], 1u << 16);
+        want[L] = clamp_cap(guess, bound);
+        const uint64_t cells = (uint64_t)grid->dims[L] * grid->dims[L] * grid->dims[L];
+        bound = std::min<uint64_t>(bound * cells, 0x7FFFFFFFull);
+    }
+    uint64_t pool_want = std::max<uint64_t>(t->pool_cap, (uint64_t)kRootOff + (tp->words[0] + 1) * 64ull + (1u << 20));
+    // constants
+    const uint32_t nc = (uint32_t)tp->consts.size();
+    if (!t->d_consts || t->consts_cap < nc + kPoolPad) {
+        if (t->d_consts) HIP_TRY(hipFree(t->d_consts));
+        t->d_consts = nullptr;
+        HIP_TRY(hipMalloc(&t->d_consts, (size_t)(nc + kPoolPad) * 4));
+        t->consts_cap = nc + kPoolPad;
+    }
+    if (nc)
+        HIP_TRY(hipMemcpyAsync(t->d_consts, tp->d_consts, (size_t)nc * 4, hipMemcpyDeviceToDevice,
+                               ctx->stream));
+    const size_t waves = (size_t)ctx->level_grid * (kLevelBlock / 64);
+    const uint32_t mm_stride = std::max<uint32_t>(tp->n_mm, 1);
+    {
+        const frep_status s = ensure_scratch(ctx, waves * mm_stride * sizeof(uint4));
+        if (s != FREP_OK) return s;
+    }
+    t->retries = 0;
+    for (uint32_t attempt = 0;; ++attempt) {
+        for (uint32_t L = 0; L < nl; ++L) {
+            const frep_status s = ensure_level(ctx, t, L, want[L]);
+            if (s != FREP_OK) return s;
+        }
+        {
+            const frep_status s = ensure_pool(ctx, t, pool_want);
+            if (s != FREP_OK) return s;
+        }
+        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+        HIP_TRY(launch_init_tree(ctx->stream, t->d_counters, t->d_nodes[0], t->d_pool, tp->d_words,
+                                 kRootOff, nl));
+        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+        for (uint32_t L = 0; L < nl; ++L) {
+            const bool last = (L + 1 == nl);
+            LevelArgs a{};
+            a.nodes = t->d_nodes[L];
+            a.child_nodes = last ? nullptr : t->d_nodes[L + 1];
+            a.pool = t->d_pool;
+            a.consts = t->d_consts;
+            a.counters = t->d_counters;
+            a.choice_scratch = ctx->d_scratch;
+            a.pool_cap = t->pool_cap;
+            a.level = L;
+            a.dim = grid->dims[L];
+            a.node_cap = t->node_cap[L];
+            a.child_cap = last ? 0 : t->node_cap[L + 1];
+            a.ns = t->ns[L];
+            a.child_ns = last ? 0 : t->ns[L + 1];
+            a.mm_stride = mm_stride;
+            a.cs = t->cs[L];
+            a.low[0] = grid->low[0];
+            a.low[1] = grid->low[1];
+            a.low[2] = grid->low[2];
+            a.tape_time = grid->tape_time;
+            if (last) HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, ctx->level_grid));
+            else HIP_TRY(launch_level_interval(ctx->stream, a, t->maxs, ctx->level_grid));
+            if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
+        }
+        HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4, hipMemcpyDeviceToHost,
+                               ctx->stream));
+        HIP_TRY(hipStreamSynchronize(ctx->stream));
+        const uint32_t ovf = t->h_counters[CNT_OVERFLOW];
+        if (ovf == 0) break;
+        if (attempt > nl + 4) return fail(ctx, FREP_OUT_OF_MEMORY, "tree did not fit after regrowth");
+        ++t->retries;
+        for (uint32_t L = 1; L < nl; ++L) {
+            const uint32_t req = t->h_counters[CNT_NODES + L];
+            if (req > t->node_cap[L]) want[L] = clamp_cap((uint64_t)req + req / 4 + 64, ~0ull);
+        }
+        uint64_t top;
+        std::memcpy(&top, t->h_counters + CNT_POOL_TOP, 8);
+        if (top > t->pool_cap) pool_want = top + top / 4 + 4096;
+        if (pool_want > kMaxPoolWords)
+            return fail(ctx, FREP_OUT_OF_MEMORY, "tape pool exceeds 2^32 words");
+    }
+    if (ctx->timing) {
+        ctx->last = frep_timing{};
+        ctx->last.n_levels = nl;
+        HIP_TRY(hipEventElapsedTime(&ctx->last.init_ms, ctx->ev[0], ctx->ev[1]));
+        for (uint32_t L = 0; L < nl; ++L)
+            HIP_TRY(hipEventElapsedTime(&ctx->last.level_ms[L], ctx->ev[1 + L], ctx->ev[2 + L]));
+    }
+    t->built = true;
+    return FREP_OK;
+}
+
+frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t h, frep_tree_info* out) {
+    if (!ctx || !out) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = get_tree(ctx, h);
+    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
+    *out = frep_tree_info{};
+    out->n_levels = t->grid.n_levels;
+    uint64_t bytes = 0;
+    for (uint32_t L = 0; L < t->grid.n_levels; ++L) {
+        out->dims[L] = t->grid.dims[L];
+        out->node_count[L] = t->h_counters[CNT_NODES + L];
+        out->tape_len_sum[L] = t->h_counters[CNT_LEN_SUM + L];
+        out->tape_len_max[L] = t->h_counters[CNT_LEN_MAX + L];
+        bytes += (uint64_t)t->node_cap[L] * t->ns[L] * 4;
+    }
+    std::memcpy(&out->tape_pool_words, t->h_counters + CNT_POOL_TOP, 8);
+    bytes += (t->pool_cap + kPoolPad) * 4 + (uint64_t)t->consts_cap * 4;
+    out->device_bytes = bytes;
+    out->slot_count = t->slot_count;
+    out->retries = t->retries;
+    return FREP_OK;
+}
+
+frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t h, uint32_t* buf, uint64_t cap,
+                             uint64_t* n_words) {
+    if (!ctx || !n_words) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = get_tree(ctx, h);
+    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t nl = t->grid.n_levels;
+    uint64_t top;
+    std::memcpy(&top, t->h_counters + CNT_POOL_TOP, 8);
+    std::vector<uint32_t> pool(top);
+    HIP_TRY(hipMemcpy(pool.data(), t->d_pool, top * 4, hipMemcpyDeviceToHost));
+    std::vector<std::vector<uint32_t>> recs(nl);
+    std::vector<uint32_t> cnt(nl);
+    for (uint32_t L = 0; L < nl; ++L) {
+        cnt[L] = t->h_counters[CNT_NODES + L];
+        recs[L].resize((size_t)cnt[L] * t->ns[L]);
+        if (cnt[L])
+            HIP_TRY(hipMemcpy(recs[L].data(), t->d_nodes[L], recs[L].size() * 4,
+                              hipMemcpyDeviceToHost));
+    }
+    // canonical order: sort each level by node coords (x, y, z)
+    std::vector<std::vector<uint32_t>> order(nl), rank(nl);
+    for (uint32_t L = 0; L < nl; ++L) {
+        const uint32_t ns = t->ns[L];
+        const uint32_t* r = recs[L].data();
+        order[L].resize(cnt[L]);
+        for (uint32_t j = 0; j < cnt[L]; ++j) order[L][j] = j;
+        std::sort(order[L].begin(), order[L].end(), [&](uint32_t a, uint32_t b) {
+            const uint32_t* ca = r + (size_t)a * ns + 1;
+            const uint32_t* cb = r + (size_t)b * ns + 1;
+            if (ca[0] != cb[0]) return ca[0] < cb[0];
+            if (ca[1] != cb[1]) return ca[1] < cb[1];
+            return ca[2] < cb[2];
+        });
+        rank[L].resize(cnt[L]);
+        for (uint32_t k = 0; k < cnt[L]; ++k) rank[L][order[L][k]] = k;
+    }
+    std::vector<uint32_t> v;
+    v.push_back(0x54505246u);
+    v.push_back(1u);
+    v.push_back(nl);
+    for (uint32_t L = 0; L < nl; ++L) v.push_back(t->grid.dims[L]);
+    for (uint32_t L = 0; L < nl; ++L) v.push_back(cnt[L]);
+    for (uint32_t L = 0; L < nl; ++L) {
+        const uint32_t dim = t->grid.dims[L], cells = dim * dim * dim, W = cells / 32;
+        const bool last = (L + 1 == nl);
+        for (uint32_t k = 0; k < cnt[L]; ++k) {
+            const uint32_t* r = recs[L].data() + (size_t)order[L][k] * t->ns[L];
+            v.insert(v.end(), r + 1, r + 4);
+            const uint32_t toff = r[0];
+            if (toff >= top || (uint64_t)toff + pool[toff] + 1 > top)
+                return fail(ctx, FREP_DEVICE, "corrupt tape offset in tree");
+            v.insert(v.end(), pool.begin() + toff, pool.begin() + toff + pool[toff] + 1);
+            v.insert(v.end(), r + 4, r + 4 + W);
+            if (!last) {
+                v.insert(v.end(), r + 4 + W, r + 4 + 2 * W);
+                for (uint32_t i = 0; i < cells; ++i) {
+                    const uint32_t ch = r[4 + 2 * W + i];
+                    if (ch == kNoChild) { v.push_back(ch); continue; }
+                    if (ch >= cnt[L + 1]) return fail(ctx, FREP_DEVICE, "corrupt child id in tree");
+                    v.push_back(rank[L + 1][ch]);
+                }
+            }
+        }
+    }
+    *n_words = v.size();
+    if (buf && cap >= v.size()) std::memcpy(buf, v.data(), v.size() * 4);
+    return FREP_OK;
+}
+
+frep_status frep_camera_look(frep_camera* cam, const float initial_up[3], float fov_deg) {
+    if (!cam || !initial_up) return FREP_INVALID_ARG;
+    const float* f = cam->forward;
+    const float cr[3] = {f[1] * initial_up[2] - f[2] * initial_up[1],
+                         f[2] * initial_up[0] - f[0] * initial_up[2],
+                         f[0] * initial_up[1] - f[1] * initial_up[0]};
+    const float l = std::sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    const float r[3] = {cr[0] / l, cr[1] / l, cr[2] / l};
+    cam->right[0] = r[0]; cam->right[1] = r[1]; cam->right[2] = r[2];
+    cam->up[0] = r[1] * f[2] - r[2] * f[1];
+    cam->up[1] = r[2] * f[0] - r[0] * f[2];
+    cam->up[2] = r[0] * f[1] - r[1] * f[0];
+    if (!cam->ortho) {
+        const float rad = fov_deg * (3.14159265358979323846f / 180.0f);
+        cam->screen_world[0] = 2.0f * std::tan(rad / 2.0f);
+        cam->screen_world[1] = cam->screen_world[0] * ((float)cam->height / (float)cam->width);
+    }
+    return FREP_OK;
+}
+
+frep_status frep_render(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam, uint8_t* rgba,
+                        uint64_t* hit_ids, float* normals) {
+    if (!ctx || !cam) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = get_tree(ctx, h);
+    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
+    if (cam->width == 0 || cam->height == 0 || cam->width > 16384 || cam->height > 16384)
+        return fail(ctx, FREP_INVALID_ARG, "image size out of range");
+    if (cam->light_count > 4) return fail(ctx, FREP_INVALID_ARG, "at most 4 lights");
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t spp = cam->spp ? cam->spp : 1;
+    const size_t px = (size_t)cam->width * cam->height;
+    const size_t rgba_b = rgba ? px * 4 : 0;
+    const size_t hit_b = hit_ids ? px * spp * 8 : 0;
+    const size_t nrm_b = normals ? px * spp * 12 : 0;
+    const size_t off_hit = (rgba_b + 255) & ~(size_t)255;
+    const size_t off_nrm = (off_hit + hit_b + 255) & ~(size_t)255;
+    const size_t total = off_nrm + nrm_b + 256;
+    if (total > ctx->out_bytes) {
+        if (ctx->d_out) HIP_TRY(hipFree(ctx->d_out));
+        ctx->d_out = nullptr;
+        ctx->out_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->d_out, total));
+        ctx->out_bytes = total;
+    }
+    char* base = (char*)ctx->d_out;
+    RenderArgs a{};
+    const uint32_t nl = t->grid.n_levels;
+    for (uint32_t L = 0; L < nl; ++L) {
+        a.lv[L].nodes = t->d_nodes[L];
+        a.lv[L].ns = t->ns[L];
+        a.lv[L].dim = t->grid.dims[L];
+        a.lv[L].cs = t->cs[L];
+    }
+    a.n_levels = nl;
+    a.pool = t->d_pool;
+    a.consts = t->d_consts;
+    for (int c = 0; c < 3; ++c) {
+        a.low[c] = t->grid.low[c];
+        a.pos[c] = cam->pos[c];
+        a.fwd[c] = cam->forward[c];
+        a.right[c] = cam->right[c];
+        a.up[c] = cam->up[c];
+    }
+    a.world_size = t->grid.world_size;
+    a.tape_time = t->grid.tape_time;
+    a.sw[0] = cam->screen_world[0];
+    a.sw[1] = cam->screen_world[1];
+    a.width = cam->width;
+    a.height = cam->height;
+    a.spp = spp;
+    a.ortho = cam->ortho;
+    a.jitter = cam->jitter;
+    a.seed = cam->seed;
+    a.light_count = cam->light_count;
+    for (int c = 0; c < 4; ++c) {
+        a.background[c] = cam->background[c];
+        a.ambient[c] = cam->ambient[c];
+        for (int l = 0; l < 4; ++l) {
+            a.light_dir[l][c] = cam->light_dir[l][c];
+            a.light_color[l][c] = cam->light_color[l][c];
+        }
+    }
+    a.rgba = rgba ? (uint8_t*)base : nullptr;
+    a.hit_ids = hit_ids ? (unsigned long long*)(base + off_hit) : nullptr;
+    a.normals = normals ? (float*)(base + off_nrm) : nullptr;
+    if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 2], ctx->stream));
+    HIP_TRY(launch_render(ctx->stream, a, t->maxs));
+    if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 3], ctx->stream));
+    if (rgba) HIP_TRY(hipMemcpyAsync(rgba, base, rgba_b, hipMemcpyDeviceToHost, ctx->stream));
+    if (hit_ids)
+        HIP_TRY(hipMemcpyAsync(hit_ids, base + off_hit, hit_b, hipMemcpyDeviceToHost, ctx->stream));
+    if (normals)
+        HIP_TRY(hipMemcpyAsync(normals, base + off_nrm, nrm_b, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->timing)
+        HIP_TRY(hipEventElapsedTime(&ctx->last.render_ms, ctx->ev[kMaxLevels + 2],
+                                    ctx->ev[kMaxLevels + 3]));
+    return FREP_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Runs one batch evaluation: copies `in` arrays to the device, launches, and
+// copies `out` arrays back.  Temporary buffers live for the call only.
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+
+extern "C" {
+
+frep_status frep_eval_points(frep_ctx* ctx, frep_tape_t th, const float* xyz, uint32_t n, float t,
+                             float* out) {
+    if (!ctx || (n && (!xyz || !out))) return fail(ctx, FREP_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TapeObj* tp = get_tape(ctx, th);
+    if (!tp) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
+    if (n == 0) return FREP_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf din, dout;
+    HIP_TRY(hipMalloc(&din.p, (size_t)n * 12));
+    HIP_TRY(hipMalloc(&dout.p, (size_t)n * 4));
+    HIP_TRY(hipMemcpyAsync(din.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_eval_points(ctx->stream, tp->d_words, tp->d_consts, (const float*)din.p, n, t,
+                               (float*)dout.p, tp->maxs));
+    HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FREP_OK;
+}
+
+frep_status frep_eval_intervals(frep_ctx* ctx, frep_tape_t th, const float* lo, const float* hi,
+                                uint32_t n, float t, float* out, uint8_t* choices) {
+    if (!ctx || (n && (!lo || !hi || !out))) return fail(ctx, FREP_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TapeObj* tp = get_tape(ctx, th);
+    if (!tp) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
+    if (n == 0) return FREP_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf dlo, dhi, dout, dch;
+    HIP_TRY(hipMalloc(&dlo.p, (size_t)n * 12));
+    HIP_TRY(hipMalloc(&dhi.p, (size_t)n * 12));
+    HIP_TRY(hipMalloc(&dout.p, (size_t)n * 8));
+    const size_t chb = (size_t)n * tp->n_mm;
+    if (choices && chb) HIP_TRY(hipMalloc(&dch.p, chb));
+    HIP_TRY(hipMemcpyAsync(dlo.p, lo, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(dhi.p, hi, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_eval_intervals(ctx->stream, tp->d_words, tp->d_consts, (const float*)dlo.p,
+                                  (const float*)dhi.p, n, t, (float*)dout.p, (uint8_t*)dch.p,
+                                  tp->n_mm, tp->maxs));
+    HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream));
+    if (choices && chb)
+        HIP_TRY(hipMemcpyAsync(choices, dch.p, chb, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FREP_OK;
+}
+
+frep_status frep_eval_gradients(frep_ctx* ctx, frep_tape_t th, const float* xyz, uint32_t n,
+                                float t, float* out4) {
+    if (!ctx || (n && (!xyz || !out4))) return fail(ctx, FREP_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TapeObj* tp = get_tape(ctx, th);
+    if (!tp) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
+    if (n == 0) return FREP_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf din, dout;
+    HIP_TRY(hipMalloc(&din.p, (size_t)n * 12));
+    HIP_TRY(hipMalloc(&dout.p, (size_t)n * 16));
+    HIP_TRY(hipMemcpyAsync(din.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_eval_gradients(ctx->stream, tp->d_words, tp->d_consts, (const float*)din.p, n,
+                                  t, (float*)dout.p, tp->maxs));
+    HIP_TRY(hipMemcpyAsync(out4, dout.p, (size_t)n * 16, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FREP_OK;
+}
+
+frep_status frep_math(frep_ctx* ctx, uint32_t fn, const float* x, uint32_t n, float* out) {
+    if (!ctx || fn > 2 || (n && (!x || !out))) return fail(ctx, FREP_INVALID_ARG, "bad argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (n == 0) return FREP_OK;
+    HIP_TRY(hipSetDevice(ctx->device));
+    DevBuf din, dout;
+    HIP_TRY(hipMalloc(&din.p, (size_t)n * 4));
+    HIP_TRY(hipMalloc(&dout.p, (size_t)n * 4));
+    HIP_TRY(hipMemcpyAsync(din.p, x, (size_t)n * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(launch_math(ctx->stream, fn, (const float*)din.p, n, (float*)dout.p));
+    HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FREP_OK;
+}
+
+}  // extern "C"

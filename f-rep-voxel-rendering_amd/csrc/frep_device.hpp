@@ -1,0 +1,300 @@
+// frep_device.hpp -- device-side arithmetic and tape interpreters (gfx950).
+//
+// Everything here is compiled with -ffp-contract=off: each +, -, *, / and
+// sqrt rounds once (IEEE binary32, round to nearest), exactly like the
+// reference's CPU evaluator (vulkan/src/csg/tape.cpp:274-302).
+//
+// Tape word layout (vulkan/include/csg/tape.h:33-39): op | out<<8 | A<<16 |
+// B<<24; CONST reads pool[A | B<<8].  Tapes are read through the constant
+// address space so wave-uniform instruction fetches become scalar loads.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "frep_internal.hpp"
+
+namespace frep {
+
+typedef const __attribute__((address_space(4))) uint32_t* ctape_t;
+typedef const __attribute__((address_space(4))) float* cconst_t;
+
+__device__ __forceinline__ ctape_t as_const(const uint32_t* p) { return (ctape_t)p; }
+__device__ __forceinline__ cconst_t as_const(const float* p) { return (cconst_t)p; }
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// GLM scalar min / max (y < x ? y : x, x < y ? y : x): exact behaviour on
+// NaN and signed zero, as used by vulkan/src/csg/tape.cpp:291-292.
+__device__ __forceinline__ float fmin_glm(float x, float y) { return y < x ? y : x; }
+__device__ __forceinline__ float fmax_glm(float x, float y) { return x < y ? y : x; }
+
+__device__ __forceinline__ float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+
+// ---------------------------------------------------------------------------
+// Transcendentals.  Cody & Waite (1980) argument reduction and Taylor
+// polynomials, using only correctly rounded operations and floor(), so the
+// CPU and GPU produce identical bits.
+//   exp:  x = k ln2 + r, |r| <= ln2/2, two-part ln2, degree-7 polynomial,
+//         result scaled by 2^(k/2) and 2^(k - k/2) (two exact products).
+//   sin / cos: x = k pi/2 + r with a three-part pi/2 whose products with k
+//         are exact for |k| < 4096, degree-9 / degree-10 polynomials.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float dev_expf(float x) {
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return __builtin_inff();
+    if (x < -103.972084045410156f) return 0.0f;
+    const float kf = __builtin_floorf(x * 1.44269502162933349609375f + 0.5f);
+    const float r = (x - kf * 0.693145751953125f) - kf * 1.428606765330187e-06f;
+    float p = 1.98412701138295233250e-04f;
+    p = 1.38888892251998186111e-03f + r * p;
+    p = 8.33333376795053482056e-03f + r * p;
+    p = 4.16666679084300994873e-02f + r * p;
+    p = 1.66666671633720397949e-01f + r * p;
+    p = 0.5f + r * p;
+    p = 1.0f + r * p;
+    p = 1.0f + r * p;
+    const int k = (int)kf;
+    const int k1 = k / 2;
+    const int k2 = k - k1;
+    const float s1 = u2f((uint32_t)(k1 + 127) << 23);
+    const float s2 = u2f((uint32_t)(k2 + 127) << 23);
+    return (p * s1) * s2;
+}
+
+__device__ __forceinline__ float sin_poly(float r) {
+    const float z = r * r;
+    float p = 2.75573188446287530661e-06f;
+    p = -1.98412701138295233250e-04f + z * p;
+    p = 8.33333376795053482056e-03f + z * p;
+    p = -1.66666671633720397949e-01f + z * p;
+    return r + (r * z) * p;
+}
+
+__device__ __forceinline__ float cos_poly(float r) {
+    const float z = r * r;
+    float p = -2.75573199814971303567e-07f;
+    p = 2.48015876422869041562e-05f + z * p;
+    p = -1.38888892251998186111e-03f + z * p;
+    p = 4.16666679084300994873e-02f + z * p;
+    p = -0.5f + z * p;
+    return 1.0f + z * p;
+}
+
+__device__ __forceinline__ float reduce_half_pi(float x, int* q) {
+    const float kf = __builtin_floorf(x * 0.636619746685028076171875f + 0.5f);
+    const float r = ((x - kf * 1.5703125f) - kf * 4.837512969970703125e-04f) -
+                    kf * 7.549790126404332e-08f;
+    // k mod 4 computed exactly in float for every finite k.
+    *q = (int)(kf - 4.0f * __builtin_floorf(kf * 0.25f));
+    return r;
+}
+
+__device__ __forceinline__ float dev_sinf(float x) {
+    if (!(__builtin_fabsf(x) <= 3.0e38f)) return x - x;
+    int q;
+    const float r = reduce_half_pi(x, &q);
+    const float s = sin_poly(r), c = cos_poly(r);
+    return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+}
+
+__device__ __forceinline__ float dev_cosf(float x) {
+    if (!(__builtin_fabsf(x) <= 3.0e38f)) return x - x;
+    int q;
+    const float r = reduce_half_pi(x, &q);
+    const float s = sin_poly(r), c = cos_poly(r);
+    return q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+}
+
+// ---------------------------------------------------------------------------
+// Point rules (vulkan/shaders/tape/point.comp:19-59).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float point_op(uint32_t op, float a, float b, float c) {
+    switch (op) {
+        case OP_CONST: return c;
+        case OP_SIN:   return dev_sinf(a);
+        case OP_COS:   return dev_cosf(a);
+        case OP_ADD:   return a + b;
+        case OP_SUB:   return a - b;
+        case OP_MUL:   return a * b;
+        case OP_DIV:   return a / b;
+        case OP_MIN:   return fmin_glm(a, b);
+        case OP_MAX:   return fmax_glm(a, b);
+        case OP_EXP:   return dev_expf(a);
+        case OP_NEG:   return -a;
+        case OP_SQRT:  return __builtin_sqrtf(a);
+        default:       return a;
+    }
+}
+
+// Evaluates a tape at K points per lane.  s[slot][k] lives in VGPRs when
+// MAXS*K is small enough for the compiler to keep the array in registers;
+// instruction fields are wave-uniform (scalar loads, scalar decode).
+template <int MAXS, int K>
+__device__ __forceinline__ void eval_point_k(ctape_t tape, cconst_t consts, float (&s)[MAXS][K]) {
+    const uint32_t n = tape[0];
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t w = tape[1 + i];
+        // slot fields masked to the register file; CONST uses the raw index
+        const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+        const uint32_t a = ((w >> 16) & 0xFFu) & (MAXS - 1), b = (w >> 24) & (MAXS - 1);
+        float r[K];
+        switch (op) {
+            case OP_CONST: {
+                const float c = consts[w >> 16];
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = c;
+            } break;
+            case OP_ADD:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = s[a][k] + s[b][k];
+                break;
+            case OP_SUB:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = s[a][k] - s[b][k];
+                break;
+            case OP_MUL:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = s[a][k] * s[b][k];
+                break;
+            case OP_MIN:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = fmin_glm(s[a][k], s[b][k]);
+                break;
+            case OP_MAX:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = fmax_glm(s[a][k], s[b][k]);
+                break;
+            case OP_SQRT:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = __builtin_sqrtf(s[a][k]);
+                break;
+            case OP_NEG:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = -s[a][k];
+                break;
+            case OP_DIV:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = s[a][k] / s[b][k];
+                break;
+            case OP_SIN:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = dev_sinf(s[a][k]);
+                break;
+            case OP_COS:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = dev_cosf(s[a][k]);
+                break;
+            case OP_EXP:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = dev_expf(s[a][k]);
+                break;
+            default:
+#pragma unroll
+                for (int k = 0; k < K; ++k) r[k] = s[a][k];
+                break;
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) s[o][k] = r[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Interval rules (vulkan/shaders/tape/interval.comp:8-111) and min/max choice
+// recording (interval.comp:140-155).
+// ---------------------------------------------------------------------------
+struct Itv { float lo, hi; };
+
+__device__ __forceinline__ Itv imul(Itv a, Itv b) {
+    const float p0 = a.lo * b.lo, p1 = a.lo * b.hi, p2 = a.hi * b.lo, p3 = a.hi * b.hi;
+    return Itv{fmin_glm(fmin_glm(p0, p1), fmin_glm(p2, p3)),
+               fmax_glm(fmax_glm(p0, p1), fmax_glm(p2, p3))};
+}
+
+__device__ __forceinline__ bool contains_multiple(float lo, float hi, float x) {
+    const float ql = lo / x, qh = hi / x;
+    const float fl = __builtin_floorf(ql), fh = __builtin_floorf(qh);
+    return fl < fh || fl == ql || fh == qh;
+}
+
+__device__ __forceinline__ Itv interval_op(uint32_t op, Itv a, Itv b, float c) {
+    const float kPi = 3.14159265358979323846f;
+    const float two_pi = 2 * kPi, half_pi = kPi / 2;
+    const float inf = __builtin_inff();
+    Itv r;
+    switch (op) {
+        case OP_CONST: r = Itv{c, c}; break;
+        case OP_COS:
+            r.lo = contains_multiple(a.lo - kPi, a.hi - kPi, two_pi)
+                       ? -1.0f : fmin_glm(dev_cosf(a.lo), dev_cosf(a.hi));
+            r.hi = contains_multiple(a.lo, a.hi, two_pi)
+                       ? 1.0f : fmax_glm(dev_cosf(a.lo), dev_cosf(a.hi));
+            break;
+        case OP_SIN:
+            r.lo = contains_multiple(a.lo + half_pi, a.hi + half_pi, two_pi)
+                       ? -1.0f : fmin_glm(dev_sinf(a.lo), dev_sinf(a.hi));
+            r.hi = contains_multiple(a.lo - half_pi, a.hi - half_pi, two_pi)
+                       ? 1.0f : fmax_glm(dev_sinf(a.lo), dev_sinf(a.hi));
+            break;
+        case OP_ADD: r = Itv{a.lo + b.lo, a.hi + b.hi}; break;
+        case OP_SUB: r = Itv{a.lo - b.hi, a.hi - b.lo}; break;
+        case OP_MUL: r = imul(a, b); break;
+        case OP_DIV:
+            if ((b.lo == 0 && b.hi == 0) || (b.lo < 0 && b.hi > 0)) r = Itv{-inf, inf};
+            else if (b.lo == 0) r = imul(a, Itv{1.0f / b.hi, inf});
+            else if (b.hi == 0) r = imul(a, Itv{-inf, 1.0f / b.lo});
+            else r = imul(a, Itv{1.0f / b.hi, 1.0f / b.lo});
+            break;
+        case OP_MIN: r = Itv{fmin_glm(a.lo, b.lo), fmin_glm(a.hi, b.hi)}; break;
+        case OP_MAX: r = Itv{fmax_glm(a.lo, b.lo), fmax_glm(a.hi, b.hi)}; break;
+        case OP_EXP: r = Itv{dev_expf(a.lo), dev_expf(a.hi)}; break;
+        case OP_NEG: r = Itv{-a.hi, -a.lo}; break;
+        case OP_SQRT:
+            r = Itv{__builtin_sqrtf(fmax_glm(0.0f, a.lo)), __builtin_sqrtf(fmax_glm(0.0f, a.hi))};
+            break;
+        default: r = a; break;
+    }
+    return r;
+}
+
+__device__ __forceinline__ uint32_t mm_choice(uint32_t op, Itv a, Itv b) {
+    if (op == OP_MIN) return a.hi < b.lo ? MM_FIRST : (b.hi < a.lo ? MM_SECOND : MM_BOTH);
+    return a.hi < b.lo ? MM_SECOND : (b.hi < a.lo ? MM_FIRST : MM_BOTH);
+}
+
+// ---------------------------------------------------------------------------
+// Gradient rules (vulkan/shaders/tape/gradient.comp:24-84).  Slot = (dx, dy,
+// dz, value).
+// ---------------------------------------------------------------------------
+struct V4 { float x, y, z, w; };
+
+__device__ __forceinline__ V4 gradient_op(uint32_t op, V4 a, V4 b, float c) {
+    V4 r;
+    switch (op) {
+        case OP_CONST: r = V4{0, 0, 0, c}; break;
+        case OP_SIN: { const float cc = dev_cosf(a.w);
+            r = V4{cc * a.x, cc * a.y, cc * a.z, dev_sinf(a.w)}; } break;
+        case OP_COS: { const float ms = -dev_sinf(a.w);
+            r = V4{ms * a.x, ms * a.y, ms * a.z, dev_cosf(a.w)}; } break;
+        case OP_ADD: r = V4{a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w}; break;
+        case OP_SUB: r = V4{a.x - b.x, a.y - b.y, a.z - b.z, a.w - b.w}; break;
+        case OP_MUL:
+            r = V4{a.x * b.w + a.w * b.x, a.y * b.w + a.w * b.y, a.z * b.w + a.w * b.z,
+                   a.w * b.w};
+            break;
+        case OP_DIV: { const float d = b.w * b.w;
+            r = V4{(a.x * b.w - a.w * b.x) / d, (a.y * b.w - a.w * b.y) / d,
+                   (a.z * b.w - a.w * b.z) / d, a.w / b.w}; } break;
+        case OP_MIN: r = (a.w < b.w) ? a : b; break;
+        case OP_MAX: r = (a.w > b.w) ? a : b; break;
+        case OP_EXP: { const float e = dev_expf(a.w);
+            r = V4{e * a.x, e * a.y, e * a.z, e * 1.0f}; } break;
+        case OP_NEG: r = V4{-a.x, -a.y, -a.z, -a.w}; break;
+        case OP_SQRT: { const float sq = __builtin_sqrtf(a.w), d = 2 * sq;
+            r = V4{a.x / d, a.y / d, a.z / d, sq}; } break;
+        default: r = a; break;
+    }
+    return r;
+}
+
+}  // namespace frep

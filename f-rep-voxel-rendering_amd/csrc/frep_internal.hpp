@@ -1,0 +1,113 @@
+// frep_internal.hpp -- structures shared by the kernels and the host driver.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace frep {
+
+constexpr uint32_t kMaxLevels = 8;
+constexpr uint32_t kNoChild = 0xFFFFFFFFu;
+
+// Tape op codes (vulkan/include/csg/tape.h:15-30) and min/max choices
+// (vulkan/shaders/tape/tape.comp:10-13).
+enum : uint32_t {
+    OP_CONST = 0, OP_SIN = 1, OP_COS = 2, OP_ADD = 3, OP_SUB = 4, OP_MUL = 5,
+    OP_DIV = 6, OP_MIN = 7, OP_MAX = 8, OP_EXP = 9, OP_NEG = 10, OP_SQRT = 11,
+    OP_COPY = 12
+};
+enum : uint32_t { MM_NONE = 0, MM_FIRST = 1, MM_SECOND = 2, MM_BOTH = 3 };
+
+// Device counter block of one tree (u32 words unless noted).
+//   [0..7]   node count per level (level 0 = 1)
+//   [8..9]   u64 tape-pool top (words)
+//   [10]     overflow flags: 1 = node capacity, 2 = tape pool
+//   [16..23] sum of child tape lengths per level (stats)
+//   [24..31] max child tape length per level (stats)
+enum : uint32_t {
+    CNT_NODES = 0, CNT_POOL_TOP = 8, CNT_OVERFLOW = 10, CNT_LEN_SUM = 16, CNT_LEN_MAX = 24,
+    CNT_WORDS = 32
+};
+enum : uint32_t { OVF_NODES = 1u, OVF_POOL = 2u };
+
+// Node record (u32 words), per level, in the reference's order
+// (vulkan/shaders/node.comp:4-15, vulkan/include/engine/voxel_storage.h:63-88):
+//   [0] tape offset in the tree's tape pool (words)
+//   [1..3] node coords (x, y, z) in units of this level's nodes
+//   [4 .. 4+W)            leaf mask, W = dim^3/32
+//   [4+W .. 4+2W)         interior mask       (non-last levels)
+//   [4+2W .. 4+2W+dim^3)  child ids, 0xFFFFFFFF = none  (non-last levels)
+__host__ __device__ inline uint32_t node_words(uint32_t dim, bool last) {
+    const uint32_t cells = dim * dim * dim;
+    return last ? 4u + cells / 32u : 4u + cells / 16u + cells;
+}
+
+struct LevelArgs {
+    uint32_t* nodes;          // this level's records
+    uint32_t* child_nodes;    // next level's records (non-last levels)
+    uint32_t* pool;           // tape pool
+    const float* consts;      // constant pool of the shape
+    uint32_t* counters;       // CNT_* block
+    uint4* choice_scratch;    // per-wave min/max choice ballots
+    uint64_t pool_cap;        // words
+    uint32_t level;
+    uint32_t dim;
+    uint32_t node_cap;        // capacity of this level
+    uint32_t child_cap;       // capacity of the next level
+    uint32_t ns;              // record words, this level
+    uint32_t child_ns;        // record words, next level
+    uint32_t mm_stride;       // uint4 entries per wave in choice_scratch
+    float cs;                 // cell size at this level
+    float low[3];
+    float tape_time;
+};
+
+struct RenderLevel {
+    const uint32_t* nodes;
+    uint32_t ns;
+    uint32_t dim;
+    float cs;
+};
+
+struct RenderArgs {
+    RenderLevel lv[kMaxLevels];
+    uint32_t n_levels;
+    const uint32_t* pool;
+    const float* consts;
+    float low[3];
+    float world_size;
+    float tape_time;
+    // camera
+    float pos[3], fwd[3], right[3], up[3];
+    float sw[2];
+    uint32_t width, height, spp, ortho, jitter, seed;
+    float background[4];
+    uint32_t light_count;
+    float light_dir[4][4];
+    float light_color[4][4];
+    float ambient[4];
+    // outputs (any may be null)
+    uint8_t* rgba;
+    unsigned long long* hit_ids;
+    float* normals;
+};
+
+// Launchers (frep_kernels.hip).  Each returns hipSuccess or the launch error.
+hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0, uint32_t* pool,
+                            const uint32_t* root_tape, uint32_t root_off, uint32_t n_levels);
+hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid);
+hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid);
+hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs);
+hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
+                              const float* xyz, uint32_t n, float t, float* out, uint32_t maxs);
+hipError_t launch_eval_intervals(hipStream_t st, const uint32_t* tape, const float* consts,
+                                 const float* lo, const float* hi, uint32_t n, float t,
+                                 float* out, uint8_t* choices, uint32_t n_mm, uint32_t maxs);
+hipError_t launch_eval_gradients(hipStream_t st, const uint32_t* tape, const float* consts,
+                                 const float* xyz, uint32_t n, float t, float* out, uint32_t maxs);
+hipError_t launch_math(hipStream_t st, uint32_t fn, const float* x, uint32_t n, float* out);
+
+// Waves per block of the persistent level kernels.
+constexpr uint32_t kLevelBlock = 256;
+
+}  // namespace frep

@@ -1,0 +1,660 @@
+// frep_kernels.hip -- HIP kernels for gfx950: tree levels, render, batch evals.
+//
+// Level kernels are persistent: a fixed grid of 256-thread blocks loops over
+// "chunks" (64 consecutive mask bits of one node = one wavefront of work) up
+// to the node count that the previous level left in device memory, so the
+// whole voxelization is enqueued without a host round trip per level (the
+// reference waits on a fence and reads the child count back after every
+// level: vulkan/src/engine/voxelizer.cpp:323-328).
+#include "frep_device.hpp"
+#include "frep_internal.hpp"
+
+namespace frep {
+
+namespace {
+
+__device__ __forceinline__ uint32_t lane_id() {
+    return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of m below this lane.
+__device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ uint32_t load_counter(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Write the 64-bit ballot as two mask words (lanes 0 and 1, one word each).
+__device__ __forceinline__ void store_mask64(uint32_t* dst, uint64_t m, uint32_t lane) {
+    if (lane < 2) dst[lane] = lane ? (uint32_t)(m >> 32) : (uint32_t)m;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Tree initialisation: root tape copied into the pool, root record, counters.
+// ---------------------------------------------------------------------------
+__global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool,
+                            const uint32_t* root_tape, uint32_t root_off, uint32_t n_levels) {
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = root_tape[0];
+    for (uint32_t i = tid; i <= n; i += blockDim.x) pool[root_off + i] = root_tape[i];
+    if (tid < CNT_WORDS) {
+        uint32_t v = 0;
+        if (tid == CNT_NODES) v = 1;
+        if (tid == CNT_POOL_TOP) v = root_off + ((n + 1 + 7u) & ~7u);
+        if (tid == CNT_LEN_SUM || tid == CNT_LEN_MAX) v = n;
+        counters[tid] = v;
+    }
+    if (tid == 0) {
+        nodes0[0] = root_off;
+        nodes0[1] = nodes0[2] = nodes0[3] = 0;
+    }
+    (void)n_levels;
+}
+
+// ---------------------------------------------------------------------------
+// Non-last level: interval classification of every cell, min/max choice
+// recording, and tape shortening of ambiguous cells
+// (vulkan/shaders/voxelizer/main.comp:88-149, vulkan/shaders/tape/
+// interval.comp:121-160, vulkan/shaders/tape/shorten.comp:148-223).
+//
+// One wavefront owns 64 consecutive mask bits of one node; lane l evaluates
+// the cell whose bit index is 64*sub + l.  Mask words are built by ballot and
+// written once (no atomics).  Each MIN/MAX records two 64-bit ballots (lanes
+// whose choice includes FIRST / SECOND) in the wave's scratch row; the
+// backward liveness pass of the shortening reads them back.  Ambiguous lanes
+// reserve their child id and an (n+1)-word tape region with one atomic per
+// wave, then write the kept instructions from the end of the region towards
+// its start, so the shortened tape [k, instrs] is contiguous without a
+// second pass.  If nothing was removed the child reuses the parent's tape
+// (shorten.comp:188-191).
+// ---------------------------------------------------------------------------
+template <int MAXS>
+__global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
+    constexpr int AW = (MAXS + 31) / 32;
+    if (load_counter(a.counters + CNT_OVERFLOW) != 0) return;
+    const uint32_t lane = lane_id();
+    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
+    uint32_t n_nodes = uni(load_counter(a.counters + CNT_NODES + a.level));
+    if (n_nodes > a.node_cap) n_nodes = a.node_cap;
+    const uint32_t dim = a.dim, dim2 = dim * dim, cells = dim2 * dim, W = cells / 32;
+    const uint32_t cpn = cells / 64;
+    const uint32_t total = n_nodes * cpn;
+    uint4* scr = a.choice_scratch + (size_t)gwave * a.mm_stride;
+    cconst_t consts = as_const(a.consts);
+
+    for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
+        const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
+        uint32_t* rec = a.nodes + (size_t)node * a.ns;
+        ctape_t hdr = as_const(rec);
+        const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
+        const uint32_t idx = sub * 64 + lane;
+        const uint32_t gx = ncx * dim + idx / dim2;
+        const uint32_t gy = ncy * dim + (idx / dim) % dim;
+        const uint32_t gz = ncz * dim + idx % dim;
+        // main.comp:113-116, 124-126: box [pos, pos + cell_size]
+        const float lx = a.low[0] + (float)gx * a.cs;
+        const float ly = a.low[1] + (float)gy * a.cs;
+        const float lz = a.low[2] + (float)gz * a.cs;
+        ctape_t tape = as_const(a.pool + toff);
+        const uint32_t n = tape[0];
+
+        float slo[MAXS], shi[MAXS];
+        slo[0] = lx; shi[0] = lx + a.cs;
+        slo[1] = ly; shi[1] = ly + a.cs;
+        slo[2] = lz; shi[2] = lz + a.cs;
+        slo[3] = a.tape_time; shi[3] = a.tape_time;
+        uint32_t mm = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t w = tape[1 + i];
+            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+            // operand fields are slot indices except for CONST (pool index);
+            // masking keeps every register-file access in range
+            const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+            const Itv A{slo[ia], shi[ia]}, B{slo[ib], shi[ib]};
+            const float c = (op == OP_CONST) ? consts[w >> 16] : 0.0f;
+            const Itv r = interval_op(op, A, B, c);
+            if (op == OP_MIN || op == OP_MAX) {
+                const uint32_t ch = mm_choice(op, A, B);
+                const uint64_t F = __ballot((ch & MM_FIRST) != 0);
+                const uint64_t S = __ballot((ch & MM_SECOND) != 0);
+                scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
+                                     (uint32_t)(S >> 32));
+                ++mm;
+            }
+            slo[o] = r.lo;
+            shi[o] = r.hi;
+        }
+        // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
+        const float ilo = slo[0], ihi = shi[0];
+        const bool full = ihi < 0.0f;
+        const bool empty = ilo > 0.0f;
+        const bool amb = !full && !empty;
+        const uint64_t Bfull = __ballot(full);
+        const uint64_t Bamb = __ballot(amb);
+        store_mask64(rec + 4 + sub * 2, Bfull, lane);
+        store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
+
+        uint32_t child_id = kNoChild;
+        if (Bamb != 0) {
+            const uint32_t namb = (uint32_t)__popcll(Bamb);
+            const uint32_t rank = lane_rank(Bamb);
+            const uint32_t stride = n + 1;
+            uint32_t cbase = 0;
+            unsigned long long pbase = 0;
+            if (lane == 0) {
+                cbase = atomicAdd(a.counters + CNT_NODES + a.level + 1, namb);
+                pbase = atomicAdd((unsigned long long*)(a.counters + CNT_POOL_TOP),
+                                  (unsigned long long)namb * stride);
+            }
+            cbase = __shfl(cbase, 0);
+            pbase = ((unsigned long long)(uint32_t)__shfl((uint32_t)(pbase >> 32), 0) << 32) |
+                    (uint32_t)__shfl((uint32_t)pbase, 0);
+            const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
+            const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
+            if (!(nodes_ok && pool_ok)) {
+                if (lane == 0)
+                    atomicOr(a.counters + CNT_OVERFLOW,
+                             (nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL));
+            } else if (amb) {
+                const uint64_t mybase = pbase + (uint64_t)rank * stride;
+                uint32_t* out = a.pool + mybase;
+                uint32_t act[AW];
+#pragma unroll
+                for (int k = 0; k < AW; ++k) act[k] = 0;
+                act[0] = 1u;                                   // slot 0 is the result
+                uint32_t kept = 0;
+                int32_t j = (int32_t)mm - 1;
+                for (int32_t i = (int32_t)n - 1; i >= 0; --i) {
+                    const uint32_t w = tape[1 + i];
+                    const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+                    const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1);
+                    const uint32_t ib = (w >> 24) & (MAXS - 1);
+                    const bool is_mm = (op == OP_MIN || op == OP_MAX);
+                    uint32_t ch = 0;
+                    if (is_mm) {
+                        const uint4 v = scr[j];
+                        --j;
+                        const uint64_t F = ((uint64_t)v.y << 32) | v.x;
+                        const uint64_t S = ((uint64_t)v.w << 32) | v.z;
+                        ch = (uint32_t)((F >> lane) & 1u) | ((uint32_t)((S >> lane) & 1u) << 1);
+                    }
+                    if ((act[o >> 5] >> (o & 31u)) & 1u) {
+                        act[o >> 5] &= ~(1u << (o & 31u));     // output first (shorten.comp:179)
+                        uint32_t sel;
+                        if (op == OP_CONST) sel = MM_NONE;
+                        else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
+                            sel = MM_BOTH;
+                        else if (is_mm) sel = ch;
+                        else sel = MM_FIRST;
+                        if (sel & MM_FIRST) act[ia >> 5] |= 1u << (ia & 31u);
+                        if (sel & MM_SECOND) act[ib >> 5] |= 1u << (ib & 31u);
+                        uint32_t ow = w;
+                        if (is_mm && ch == MM_FIRST) ow = (w & ~0xFFu) | OP_COPY;
+                        else if (is_mm && ch == MM_SECOND)
+                            ow = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
+                        out[n - kept] = ow;
+                        ++kept;
+                    }
+                }
+                uint32_t final_off = toff;
+                if (kept < n) {
+                    out[n - kept] = kept;
+                    final_off = (uint32_t)(mybase + n - kept);
+                }
+                child_id = cbase + rank;
+                uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
+                crec[0] = final_off;
+                crec[1] = gx;
+                crec[2] = gy;
+                crec[3] = gz;
+                atomicAdd(a.counters + CNT_LEN_SUM + a.level + 1, kept);
+                atomicMax(a.counters + CNT_LEN_MAX + a.level + 1, kept);
+            }
+        }
+        rec[4 + 2 * W + idx] = child_id;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Last level: point evaluation at each voxel's low corner, leaf iff < 0
+// (vulkan/shaders/voxelizer/main.comp:74-83).  Each lane evaluates K voxels
+// of the same node (same tape) so instruction decode is shared K ways.
+// ---------------------------------------------------------------------------
+template <int MAXS, int K>
+__global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
+    if (load_counter(a.counters + CNT_OVERFLOW) != 0) return;
+    const uint32_t lane = lane_id();
+    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
+    uint32_t n_nodes = uni(load_counter(a.counters + CNT_NODES + a.level));
+    if (n_nodes > a.node_cap) n_nodes = a.node_cap;
+    const uint32_t dim = a.dim, dim2 = dim * dim, cells = dim2 * dim;
+    const uint32_t cpw = cells / (64 * K);
+    const uint32_t total = n_nodes * cpw;
+    cconst_t consts = as_const(a.consts);
+
+    for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
+        const uint32_t node = chunk / cpw, sub = chunk - node * cpw;
+        uint32_t* rec = a.nodes + (size_t)node * a.ns;
+        ctape_t hdr = as_const(rec);
+        const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
+        float s[MAXS][K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t idx = (sub * K + k) * 64 + lane;
+            const uint32_t gx = ncx * dim + idx / dim2;
+            const uint32_t gy = ncy * dim + (idx / dim) % dim;
+            const uint32_t gz = ncz * dim + idx % dim;
+            s[0][k] = a.low[0] + (float)gx * a.cs;
+            s[1][k] = a.low[1] + (float)gy * a.cs;
+            s[2][k] = a.low[2] + (float)gz * a.cs;
+            s[3][k] = a.tape_time;
+        }
+        eval_point_k<MAXS, K>(as_const(a.pool + toff), consts, s);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint64_t B = __ballot(s[0][k] < 0.0f);
+            store_mask64(rec + 4 + (sub * K + k) * 2, B, lane);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Render: ray generation (raytracer/main.comp:245-267), raygrid_intersect +
+// kd-restart dda (main.comp:24-111), gradient shading at the hit
+// (main.comp:115-139).  One wavefront per 8x8 pixel tile.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr float kEps = 0.0001f;            // utils.comp:19
+constexpr uint32_t kMaxSteps = 1u << 14;
+
+__device__ __forceinline__ uint32_t hash32(uint32_t v) {
+    const uint32_t s = v * 747796405u + 2891336453u;
+    const uint32_t w = ((s >> ((s >> 28u) + 4u)) ^ s) * 277803737u;
+    return (w >> 22u) ^ w;
+}
+
+__device__ __forceinline__ float jitter01(uint32_t seed, uint32_t x, uint32_t y, uint32_t sample,
+                                          uint32_t axis) {
+    const uint32_t h = hash32(seed ^ hash32(x ^ hash32(y ^ hash32(sample * 2u + axis))));
+    return (float)(h >> 8) * 5.9604644775390625e-08f;
+}
+
+struct Hit {
+    float t;
+    uint32_t node, level;
+    uint32_t gc[3];
+};
+
+__device__ bool dda(const RenderArgs& a, const float o[3], const float d[3], Hit* hit) {
+    float inv[3], t0[3], t1[3], sgn[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        inv[c] = 1.0f / d[c];
+        t0[c] = ((a.low[c] + kEps) - o[c]) * inv[c];
+        t1[c] = (((a.low[c] + a.world_size) - kEps) - o[c]) * inv[c];
+        if (inv[c] < 0.0f) { const float tmp = t0[c]; t0[c] = t1[c]; t1[c] = tmp; }
+        sgn[c] = inv[c] >= 0.0f ? 1.0f : 0.0f;
+    }
+    const float t_enter = fmax_glm(t0[0], fmax_glm(t0[1], t0[2]));
+    const float t_leave = fmin_glm(t1[0], fmin_glm(t1[1], t1[2]));
+    if (!(t_enter <= t_leave && t_leave >= 0.0f)) return false;
+    float t_curr = fmax_glm(t_enter, 0.0f) + kEps;
+    float t_max = t_curr;
+    uint32_t steps = 0;
+    while (t_max < t_leave - kEps) {
+        uint32_t lvl = 0, node = 0;
+        float np[3] = {a.low[0], a.low[1], a.low[2]};
+        t_curr = t_max + kEps;
+        t_max = t_leave;
+        while (t_curr < t_max - kEps) {
+            if (++steps > kMaxSteps) return false;
+            const RenderLevel& L = a.lv[lvl];
+            const float cs = L.cs;
+            const uint32_t dim = L.dim;
+            const float s = t_curr + kEps;
+            int cell[3];
+            float tn[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float p = (o[c] + d[c] * s) - np[c];
+                const float norm = p / cs;
+                int ci = (int)__builtin_floorf(norm);
+                ci = ci < 0 ? 0 : (ci > (int)dim - 1 ? (int)dim - 1 : ci);
+                cell[c] = ci;
+                tn[c] = s + ((((float)ci + sgn[c]) - norm) * cs) * inv[c];
+            }
+            const uint32_t idx = (uint32_t)cell[2] + dim * (uint32_t)cell[1] +
+                                 dim * dim * (uint32_t)cell[0];
+            const uint32_t* rec = L.nodes + (size_t)node * L.ns;
+            const uint32_t W = dim * dim * dim / 32;
+            const uint32_t bit = 1u << (idx & 31u);
+            const bool last = (lvl + 1 == a.n_levels);
+            if (rec[4 + (idx >> 5)] & bit) {
+                hit->t = t_curr;
+                hit->node = node;
+                hit->level = lvl;
+#pragma unroll
+                for (int c = 0; c < 3; ++c) hit->gc[c] = rec[1 + c] * dim + (uint32_t)cell[c];
+                return true;
+            } else if (!last && (rec[4 + W + (idx >> 5)] & bit)) {
+                node = rec[4 + 2 * W + idx];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) np[c] = np[c] + (float)cell[c] * cs;
+                t_max = fmin_glm(tn[0], fmin_glm(tn[1], tn[2]));
+                ++lvl;
+            } else {
+                t_curr = fmin_glm(tn[0], fmin_glm(tn[1], tn[2]));
+            }
+        }
+    }
+    return false;
+}
+
+template <int MAXS>
+__device__ __forceinline__ V4 eval_gradient(ctape_t tape, cconst_t consts, float x, float y,
+                                            float z, float t) {
+    V4 s[MAXS];
+    s[0] = V4{1, 0, 0, x};
+    s[1] = V4{0, 1, 0, y};
+    s[2] = V4{0, 0, 1, z};
+    s[3] = V4{0, 0, 0, t};
+    const uint32_t n = tape[0];
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t w = tape[1 + i];
+        const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+        const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+        const float c = (op == OP_CONST) ? consts[w >> 16] : 0.0f;
+        s[o] = gradient_op(op, s[ia], s[ib], c);
+    }
+    return s[0];
+}
+
+__device__ __forceinline__ void normalize3(float v[3]) {
+    const float l = __builtin_sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    v[0] = v[0] / l; v[1] = v[1] / l; v[2] = v[2] / l;
+}
+
+}  // namespace
+
+template <int MAXS>
+__global__ __launch_bounds__(64) void k_render(RenderArgs a) {
+    const uint32_t lane = lane_id();
+    const uint32_t tiles_x = (a.width + 7) / 8;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t px = (tile % tiles_x) * 8 + (lane & 7u);
+    const uint32_t py = (tile / tiles_x) * 8 + (lane >> 3);
+    const bool valid = px < a.width && py < a.height;
+    const uint32_t spp = a.spp ? a.spp : 1;
+    float acc[4] = {0, 0, 0, 0};
+    cconst_t consts = as_const(a.consts);
+    for (uint32_t smp = 0; smp < spp; ++smp) {
+        float ox = 0.0f, oy = 0.0f;
+        if (a.jitter) {
+            ox = 2.0f * (jitter01(a.seed, px, py, smp, 0) - 0.5f);
+            oy = 2.0f * (jitter01(a.seed, px, py, smp, 1) - 0.5f);
+        }
+        const float dx = 2.0f * (((float)px + ox) / (float)a.width) - 1.0f;
+        const float dy = 2.0f * (((float)py + oy) / (float)a.height) - 1.0f;
+        float orig[3], dir[3];
+        if (a.ortho) {
+            const float hx = (dx * a.sw[0]) * 0.5f, hy = (dy * a.sw[1]) * 0.5f;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                orig[c] = (a.pos[c] + a.right[c] * hx) + a.up[c] * hy;
+                dir[c] = a.fwd[c];
+            }
+        } else {
+            const float sx = dx * a.sw[0], sy = dy * a.sw[1];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                orig[c] = a.pos[c];
+                dir[c] = (a.fwd[c] + a.right[c] * sx) + a.up[c] * sy;
+            }
+            normalize3(dir);
+        }
+        Hit h{};
+        const bool hit = valid && dda(a, orig, dir, &h);
+        uint32_t toff = 0;
+        if (hit) toff = a.lv[h.level].nodes[(size_t)h.node * a.lv[h.level].ns];
+        // Waterfall over the distinct tapes hit in this wavefront: each round
+        // evaluates one tape, wave-uniform, for the lanes that hit it.
+        V4 g{0, 0, 0, 0};
+        bool need = hit;
+        for (;;) {
+            const uint64_t m = __ballot(need);
+            if (m == 0) break;
+            const uint32_t leader = (uint32_t)__builtin_ctzll(m);
+            const uint32_t tu = uni((uint32_t)__shfl((int)toff, (int)leader));
+            if (need && toff == tu) {
+                float p[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) p[c] = orig[c] + dir[c] * h.t;
+                g = eval_gradient<MAXS>(as_const(a.pool + tu), consts, p[0], p[1], p[2],
+                                        a.tape_time);
+                need = false;
+            }
+        }
+        const size_t sidx = ((size_t)py * a.width + px) * spp + smp;
+        float col[4];
+        if (hit) {
+            float nrm[3] = {g.x, g.y, g.z};
+            normalize3(nrm);
+            if (a.hit_ids)
+                a.hit_ids[sidx] = ((unsigned long long)h.level << 60) |
+                                  ((unsigned long long)h.gc[0] << 40) |
+                                  ((unsigned long long)h.gc[1] << 20) | (unsigned long long)h.gc[2];
+            if (a.normals) {
+                a.normals[sidx * 3 + 0] = nrm[0];
+                a.normals[sidx * 3 + 1] = nrm[1];
+                a.normals[sidx * 3 + 2] = nrm[2];
+            }
+            if (h.t < 0.0f) {
+                col[0] = col[1] = col[2] = col[3] = 1.0f;
+            } else {
+                float diffuse[4] = {0, 0, 0, 0};
+                for (uint32_t l = 0; l < a.light_count; ++l) {
+                    const float it = fmax_glm(-a.light_dir[l][0] * nrm[0] +
+                                                  -a.light_dir[l][1] * nrm[1] +
+                                                  -a.light_dir[l][2] * nrm[2], 0.0f);
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) diffuse[c] = diffuse[c] + it * a.light_color[l][c];
+                }
+#pragma unroll
+                for (int c = 0; c < 4; ++c) col[c] = a.ambient[c] + diffuse[c];
+            }
+        } else {
+            if (valid && a.hit_ids) a.hit_ids[sidx] = ~0ull;
+            if (valid && a.normals) {
+                const float nan = __builtin_nanf("");
+                a.normals[sidx * 3 + 0] = nan;
+                a.normals[sidx * 3 + 1] = nan;
+                a.normals[sidx * 3 + 2] = nan;
+            }
+#pragma unroll
+            for (int c = 0; c < 4; ++c) col[c] = a.background[c];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[c] = acc[c] + col[c];
+    }
+    if (valid && a.rgba) {
+        uchar4 v;
+        float f[4];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            float x = acc[c] / (float)spp;
+            x = x < 0.0f ? 0.0f : (x > 1.0f ? 1.0f : x);
+            f[c] = x * 255.0f + 0.5f;
+        }
+        v.x = (unsigned char)f[0]; v.y = (unsigned char)f[1];
+        v.z = (unsigned char)f[2]; v.w = (unsigned char)f[3];
+        reinterpret_cast<uchar4*>(a.rgba)[(size_t)py * a.width + px] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Batch primitives: one lane per point / box, one tape for the whole launch.
+// ---------------------------------------------------------------------------
+template <int MAXS>
+__global__ __launch_bounds__(256) void k_eval_points(const uint32_t* tape, const float* consts,
+                                                     const float* xyz, uint32_t n, float t,
+                                                     float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = i < n ? i : n - 1;
+    float s[MAXS][1];
+    s[0][0] = xyz[3 * j + 0];
+    s[1][0] = xyz[3 * j + 1];
+    s[2][0] = xyz[3 * j + 2];
+    s[3][0] = t;
+    eval_point_k<MAXS, 1>(as_const(tape), as_const(consts), s);
+    if (i < n) out[i] = s[0][0];
+}
+
+template <int MAXS>
+__global__ __launch_bounds__(256) void k_eval_intervals(const uint32_t* tape_, const float* consts_,
+                                                        const float* lo, const float* hi,
+                                                        uint32_t n, float t, float* out,
+                                                        uint8_t* choices, uint32_t n_mm) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = i < n ? i : n - 1;
+    ctape_t tape = as_const(tape_);
+    cconst_t consts = as_const(consts_);
+    float slo[MAXS], shi[MAXS];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { slo[c] = lo[3 * j + c]; shi[c] = hi[3 * j + c]; }
+    slo[3] = t; shi[3] = t;
+    const uint32_t nt = tape[0];
+    uint32_t mm = 0;
+    for (uint32_t k = 0; k < nt; ++k) {
+        const uint32_t w = tape[1 + k];
+        const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+        const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+        const Itv A{slo[ia], shi[ia]}, B{slo[ib], shi[ib]};
+        const float c = (op == OP_CONST) ? consts[w >> 16] : 0.0f;
+        const Itv r = interval_op(op, A, B, c);
+        if (op == OP_MIN || op == OP_MAX) {
+            if (choices && i < n) choices[(size_t)i * n_mm + mm] = (uint8_t)mm_choice(op, A, B);
+            ++mm;
+        }
+        slo[o] = r.lo;
+        shi[o] = r.hi;
+    }
+    if (i < n) { out[2 * i] = slo[0]; out[2 * i + 1] = shi[0]; }
+}
+
+template <int MAXS>
+__global__ __launch_bounds__(256) void k_eval_gradients(const uint32_t* tape, const float* consts,
+                                                        const float* xyz, uint32_t n, float t,
+                                                        float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t j = i < n ? i : n - 1;
+    const V4 g = eval_gradient<MAXS>(as_const(tape), as_const(consts), xyz[3 * j],
+                                     xyz[3 * j + 1], xyz[3 * j + 2], t);
+    if (i < n) {
+        out[4 * i + 0] = g.x; out[4 * i + 1] = g.y; out[4 * i + 2] = g.z; out[4 * i + 3] = g.w;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_math(uint32_t fn, const float* x, uint32_t n, float* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float v = x[i];
+    out[i] = fn == 0 ? dev_sinf(v) : (fn == 1 ? dev_cosf(v) : dev_expf(v));
+}
+
+// ---------------------------------------------------------------------------
+// Launchers: template dispatch on the slot-file size.
+// ---------------------------------------------------------------------------
+#define FREP_DISPATCH_MAXS(maxs, CALL)                 \
+    switch (maxs) {                                     \
+        case 8: { constexpr int M = 8; CALL; } break;   \
+        case 16: { constexpr int M = 16; CALL; } break; \
+        case 32: { constexpr int M = 32; CALL; } break; \
+        case 64: { constexpr int M = 64; CALL; } break; \
+        case 128: { constexpr int M = 128; CALL; } break; \
+        case 256: { constexpr int M = 256; CALL; } break; \
+        default: return hipErrorInvalidValue;           \
+    }
+
+hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0, uint32_t* pool,
+                            const uint32_t* root_tape, uint32_t root_off, uint32_t n_levels) {
+    k_init_tree<<<1, 256, 0, st>>>(counters, nodes0, pool, root_tape, root_off, n_levels);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid) {
+    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<grid, kLevelBlock, 0, st>>>(a)));
+    return hipGetLastError();
+}
+
+// Voxels per lane of the point kernel for a slot-file size (keeps the slot
+// array within the register budget the compiler promotes).
+__host__ __device__ constexpr int point_k_for(int maxs) { return maxs <= 8 ? 4 : (maxs <= 16 ? 2 : 1); }
+
+hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid) {
+    const uint32_t cpn = a.dim * a.dim * a.dim / 64;
+#define FREP_POINT_CALL                                                              \
+    {                                                                                \
+        constexpr int KM = point_k_for(M);                                           \
+        if (KM >= 4 && cpn % 4 == 0)                                                 \
+            k_level_point<M, (KM >= 4 ? 4 : 1)><<<grid, kLevelBlock, 0, st>>>(a);   \
+        else if (KM >= 2 && cpn % 2 == 0)                                            \
+            k_level_point<M, (KM >= 2 ? 2 : 1)><<<grid, kLevelBlock, 0, st>>>(a);   \
+        else                                                                         \
+            k_level_point<M, 1><<<grid, kLevelBlock, 0, st>>>(a);                    \
+    }
+    FREP_DISPATCH_MAXS(maxs, FREP_POINT_CALL);
+#undef FREP_POINT_CALL
+    return hipGetLastError();
+}
+
+uint32_t point_k_used(uint32_t maxs, uint32_t dim) {
+    const uint32_t cpn = dim * dim * dim / 64;
+    const int km = point_k_for((int)maxs);
+    if (km >= 4 && cpn % 4 == 0) return 4;
+    if (km >= 2 && cpn % 2 == 0) return 2;
+    return 1;
+}
+
+hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs) {
+    const uint32_t tiles = ((a.width + 7) / 8) * ((a.height + 7) / 8);
+    FREP_DISPATCH_MAXS(maxs, (k_render<M><<<tiles, 64, 0, st>>>(a)));
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
+                              const float* xyz, uint32_t n, float t, float* out, uint32_t maxs) {
+    const uint32_t blocks = (n + 255) / 256;
+    FREP_DISPATCH_MAXS(maxs, (k_eval_points<M><<<blocks, 256, 0, st>>>(tape, consts, xyz, n, t, out)));
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_intervals(hipStream_t st, const uint32_t* tape, const float* consts,
+                                 const float* lo, const float* hi, uint32_t n, float t,
+                                 float* out, uint8_t* choices, uint32_t n_mm, uint32_t maxs) {
+    const uint32_t blocks = (n + 255) / 256;
+    FREP_DISPATCH_MAXS(maxs, (k_eval_intervals<M><<<blocks, 256, 0, st>>>(tape, consts, lo, hi, n, t,
+                                                                         out, choices, n_mm)));
+    return hipGetLastError();
+}
+
+hipError_t launch_eval_gradients(hipStream_t st, const uint32_t* tape, const float* consts,
+                                 const float* xyz, uint32_t n, float t, float* out, uint32_t maxs) {
+    const uint32_t blocks = (n + 255) / 256;
+    FREP_DISPATCH_MAXS(maxs, (k_eval_gradients<M><<<blocks, 256, 0, st>>>(tape, consts, xyz, n, t, out)));
+    return hipGetLastError();
+}
+
+hipError_t launch_math(hipStream_t st, uint32_t fn, const float* x, uint32_t n, float* out) {
+    k_math<<<(n + 255) / 256, 256, 0, st>>>(fn, x, n, out);
+    return hipGetLastError();
+}
+
+}  // namespace frep

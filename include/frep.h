@@ -1,0 +1,168 @@
+/*
+ * frep.h -- C ABI of the MI355X F-rep voxelizer / raytracer (libfrep.so).
+ *
+ * Drop-in boundary for the reference's hot path (MathisBD/f-rep-voxel-
+ * rendering, vulkan/): tape upload -> voxelize -> render.  Plain C types
+ * only; the library owns all device memory and hands out opaque handles.
+ * Every entry point returns a frep_status and never aborts; the message of
+ * the last failure is available from frep_last_error().
+ *
+ * Reference interfaces each entry point replaces:
+ *   frep_tape_upload  VoxelStorage::Init + Voxelizer::UploadTape
+ *                     (vulkan/include/engine/voxel_storage.h:35-49,
+ *                      vulkan/src/engine/voxelizer.cpp:385-427); the constant
+ *                     pool that the reference copies into every dispatch's
+ *                     UBO (voxelizer.cpp:222-223) travels with the tape.
+ *   frep_voxelize     Voxelizer::Voxelize (vulkan/include/engine/voxelizer.h:21,
+ *                     vulkan/src/engine/voxelizer.cpp:429-440); the grid is
+ *                     VoxelStorage::gridDims / lowVertex / worldSize
+ *                     (vulkan/include/engine/voxel_storage.h:15-17).
+ *   frep_render       Raytracer::Trace (vulkan/include/engine/raytracer.h:24,
+ *                     vulkan/src/engine/raytracer.cpp:243-272) with the
+ *                     camera basis of camera.cpp:16-27 and the UBO fields of
+ *                     raytracer.cpp:157-212.
+ *   frep_tree_export  canonical dump of VoxelStorage's node/tape buffers
+ *                     (vulkan/include/engine/voxel_storage.h:63-88).
+ *   frep_eval_*       batch forms of the tape evaluators
+ *                     (vulkan/shaders/tape/point.comp:2-63,
+ *                      vulkan/shaders/tape/interval.comp:121-160,
+ *                      vulkan/shaders/tape/gradient.comp:4-88).
+ */
+#ifndef FREP_H
+#define FREP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum frep_status {
+    FREP_OK = 0,
+    FREP_INVALID_ARG = 1,   /* null pointer, bad size, unknown handle */
+    FREP_BAD_TAPE = 2,      /* malformed tape (op, slot or constant out of range) */
+    FREP_OUT_OF_MEMORY = 3, /* device allocation failed or budget exceeded */
+    FREP_DEVICE = 4,        /* HIP runtime error */
+    FREP_UNSUPPORTED = 5    /* valid input outside what this build supports */
+} frep_status;
+
+typedef struct frep_ctx frep_ctx;
+typedef uint32_t frep_tape_t;   /* 0 is never a valid handle */
+typedef uint32_t frep_tree_t;
+
+#define FREP_TAPE_FORMAT_REF 1u   /* reference words: op|out<<8|A<<16|B<<24, <=256 consts */
+#define FREP_TAPE_FORMAT_WIDE 2u  /* same words; CONST index = A | B<<8 (<=65536 consts) */
+
+#define FREP_MAX_LEVELS 8u
+
+/* Grid of the tree (VoxelStorage::gridDims/lowVertex/worldSize + tape time).
+ * dims[L]^3 must be a multiple of 64 (the reference asserts the same,
+ * voxelizer.cpp:270), i.e. dims[L] in {4, 8, 12, 16, ...}, dims[L] <= 32. */
+typedef struct frep_grid {
+    uint32_t n_levels;
+    uint32_t dims[FREP_MAX_LEVELS];
+    float low[3];
+    float world_size;
+    float tape_time;
+} frep_grid;
+
+/* Camera and shading parameters (the reference's raytracer UBO,
+ * raytracer/inputs.comp:46-73).  right/up/screen_world are normally filled
+ * by frep_camera_look(). */
+typedef struct frep_camera {
+    float pos[3];
+    float forward[3];
+    float right[3];
+    float up[3];
+    float screen_world[2];  /* perspective: 2tan(fov/2), *H/W; ortho: view extent */
+    uint32_t width, height;
+    uint32_t spp;           /* samples per pixel, >= 1 */
+    uint32_t ortho;         /* 0 perspective (reference), 1 orthographic */
+    uint32_t jitter;        /* 0: no sub-pixel offset; 1: seeded jitter */
+    uint32_t seed;
+    float background[4];
+    uint32_t light_count;   /* <= 4 */
+    float light_dir[4][4];
+    float light_color[4][4];
+    float ambient[4];
+} frep_camera;
+
+typedef struct frep_tree_info {
+    uint32_t n_levels;
+    uint32_t dims[FREP_MAX_LEVELS];
+    uint32_t node_count[FREP_MAX_LEVELS];
+    uint32_t tape_len_sum[FREP_MAX_LEVELS];  /* sum of node tape lengths per level */
+    uint32_t tape_len_max[FREP_MAX_LEVELS];
+    uint64_t tape_pool_words;                /* words of the tape pool in use */
+    uint64_t device_bytes;                   /* device memory held by the tree */
+    uint32_t slot_count;
+    uint32_t retries;                        /* pool regrowths of the last voxelize */
+} frep_tree_info;
+
+/* Per-launch GPU durations of the last frep_voxelize / frep_render, measured
+ * with HIP events on the library's stream when timing is enabled. */
+typedef struct frep_timing {
+    uint32_t n_levels;
+    float level_ms[FREP_MAX_LEVELS];  /* one kernel per tree level */
+    float init_ms;
+    float render_ms;
+} frep_timing;
+
+/* ---- context ---- */
+frep_status frep_init(int device, frep_ctx** out);
+void        frep_shutdown(frep_ctx* ctx);
+const char* frep_last_error(const frep_ctx* ctx);
+frep_status frep_set_timing(frep_ctx* ctx, int enable);
+frep_status frep_get_timing(frep_ctx* ctx, frep_timing* out);
+/* Number of compute units and the persistent grid size used for level kernels. */
+frep_status frep_device_info(frep_ctx* ctx, uint32_t* cu_count, uint32_t* level_grid);
+
+/* ---- tape upload ---- */
+/* words = [n, instr_0 .. instr_{n-1}], n_words >= n + 1.  Validates every
+ * operand (slot < slot_count, const index < n_consts, op < 13) and copies
+ * the tape and its constant pool to the device. */
+frep_status frep_tape_upload(frep_ctx* ctx, const uint32_t* words, uint32_t n_words,
+                             const float* consts, uint32_t n_consts, uint32_t slot_count,
+                             uint32_t format, frep_tape_t* out);
+frep_status frep_tape_free(frep_ctx* ctx, frep_tape_t tape);
+
+/* ---- voxelize ---- */
+frep_status frep_tree_create(frep_ctx* ctx, frep_tree_t* out);
+/* Builds the sparse tree of `tape` over `grid` into `tree` (blocking, like
+ * Voxelizer::Voxelize).  Buffers of the tree are reused and grown as needed. */
+frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t tape, const frep_grid* grid,
+                          frep_tree_t tree);
+frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t tree, frep_tree_info* out);
+/* Canonical blob of the tree (layout in the INTEGRATION notes); returns the
+ * word count in *n_words and writes the blob when buf has room (cap words). */
+frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t tree, uint32_t* buf, uint64_t cap,
+                             uint64_t* n_words);
+frep_status frep_tree_free(frep_ctx* ctx, frep_tree_t tree);
+
+/* ---- render ---- */
+/* Fills right/up/screen_world from forward, initial up and a horizontal fov
+ * (camera.cpp:16-27, raytracer.cpp:177-181); ortho keeps screen_world. */
+frep_status frep_camera_look(frep_camera* cam, const float initial_up[3], float fov_deg);
+/* rgba: W*H*4 bytes; hit_ids: W*H*spp u64 (level<<60 | x<<40 | y<<20 | z of
+ * the hit cell at its level, UINT64_MAX = miss); normals: W*H*spp*3 floats.
+ * Host pointers; any may be NULL. */
+frep_status frep_render(frep_ctx* ctx, frep_tree_t tree, const frep_camera* cam,
+                        uint8_t* rgba, uint64_t* hit_ids, float* normals);
+
+/* ---- batch tape evaluation (host pointers) ---- */
+frep_status frep_eval_points(frep_ctx* ctx, frep_tape_t tape, const float* xyz, uint32_t n,
+                             float t, float* out);
+/* out: n * {low, high}; choices (optional): n * n_mm bytes (1 FIRST, 2 SECOND,
+ * 3 BOTH) with n_mm = number of MIN/MAX instructions of the tape. */
+frep_status frep_eval_intervals(frep_ctx* ctx, frep_tape_t tape, const float* lo,
+                                const float* hi, uint32_t n, float t, float* out,
+                                uint8_t* choices);
+frep_status frep_eval_gradients(frep_ctx* ctx, frep_tape_t tape, const float* xyz, uint32_t n,
+                                float t, float* out4);
+/* fn: 0 sin, 1 cos, 2 exp -- the device implementations used by the tapes. */
+frep_status frep_math(frep_ctx* ctx, uint32_t fn, const float* x, uint32_t n, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

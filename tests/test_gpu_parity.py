@@ -1,0 +1,213 @@
+"""GPU parity: libfrep.so (HIP, through the C ABI) against the CPU oracle.
+
+Bar: bit-exact for trees (canonical blob: structure, masks, child links and
+every shortened tape), hit ids, densities, interval bounds and min/max
+choices; normals and gradients within 2 ulp (FP32), which the shared
+operation order makes exact in practice.
+"""
+import numpy as np
+import pytest
+
+from frep import lib, shapes
+from frep.tape import Builder, compile_tape
+
+pytestmark = pytest.mark.gpu
+
+MISS = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def same_bits(a, b):
+    """Bitwise float equality, all NaNs equal."""
+    a = np.asarray(a, dtype=np.float32)
+    b = np.asarray(b, dtype=np.float32)
+    nan = np.isnan(a) & np.isnan(b)
+    return bool(np.all(nan | (a.view(np.uint32) == b.view(np.uint32))))
+
+
+def ulp_diff(a, b):
+    a = np.asarray(a, dtype=np.float32).view(np.int32).astype(np.int64)
+    b = np.asarray(b, dtype=np.float32).view(np.int32).astype(np.int64)
+    a = np.where(a < 0, -(a & 0x7FFFFFFF), a)
+    b = np.where(b < 0, -(b & 0x7FFFFFFF), b)
+    return np.abs(a - b)
+
+
+def trig_tape():
+    """A tape touching every op: sin, cos, exp, div, neg, sqrt, min, max, t."""
+    b = Builder()
+    x, y, z, t = b.x(), b.y(), b.z(), b.t()
+    g = b.add(b.mul(b.sin(b.mul(b.const(6.0), x)), b.cos(b.mul(b.const(5.0), y))),
+              b.div(b.exp(b.mul(b.const(-1.5), b.square(z))), b.add(b.const(1.5), t)))
+    s = shapes.sphere_sdf(b, (0.1, -0.2, 0.05), 0.6)
+    f = b.max(b.min(b.sub(g, b.const(0.3)), s), b.neg(b.add(s, b.const(0.4))))
+    f = b.min(f, b.sub(b.sqrt(b.add(b.square(x), b.const(0.01))), b.const(0.05)))
+    return compile_tape(b, f)
+
+
+TAPES = {
+    "spheres8": lambda: shapes.spheres_union(8, 1),
+    "csg24": lambda: shapes.random_csg(24, 2),
+    "trig": trig_tape,
+}
+
+
+@pytest.mark.parametrize("fn", [0, 1, 2])
+def test_math_bit_exact(ctx, oracle, fn):
+    rng = np.random.default_rng(10 + fn)
+    if fn == 2:
+        x = np.concatenate([rng.uniform(-110, 95, 20000), rng.uniform(-2, 2, 20000)])
+    else:
+        x = np.concatenate([rng.uniform(-50, 50, 20000), rng.uniform(-7000, 7000, 5000),
+                            rng.uniform(-1e6, 1e6, 1000)])
+    special = [0.0, -0.0, np.inf, -np.inf, np.nan, 1e-30, -1e-30, 3e38, -3e38, 88.72, -87.3,
+               -103.9, -104.0, 1.5707964, 3.1415927, 6.2831855]
+    x = np.concatenate([x, special]).astype(np.float32)
+    got = ctx.math(fn, x)
+    f = [oracle.sinf, oracle.cosf, oracle.expf][fn]
+    ref = np.array([f(float(v)) for v in x], dtype=np.float32)
+    assert same_bits(got, ref)
+
+
+@pytest.mark.parametrize("name", sorted(TAPES))
+def test_eval_points_bit_exact(ctx, oracle, name):
+    tape = TAPES[name]()
+    th = ctx.upload(tape)
+    rng = np.random.default_rng(1)
+    p = rng.uniform(-1.2, 1.2, (4096, 3)).astype(np.float32)
+    for t in (0.0, 0.75):
+        got = ctx.eval_points(th, p, t)
+        ref = np.array([oracle.eval_point(tape, *q, t) for q in p], dtype=np.float32)
+        assert same_bits(got, ref), name
+
+
+@pytest.mark.parametrize("name", sorted(TAPES))
+def test_eval_intervals_bit_exact(ctx, oracle, name):
+    tape = TAPES[name]()
+    th = ctx.upload(tape)
+    n_mm = sum(1 for op, *_ in tape.instrs() if op in (7, 8))
+    rng = np.random.default_rng(2)
+    lo = rng.uniform(-1.2, 1.0, (2048, 3)).astype(np.float32)
+    size = rng.choice([0.0, 1e-3, 0.05, 0.3, 1.0], size=(2048, 1)).astype(np.float32)
+    hi = (lo + size).astype(np.float32)
+    got, ch = ctx.eval_intervals(th, lo, hi, 0.25, n_mm=n_mm)
+    for k in range(len(lo)):
+        (rlo, rhi), rch = oracle.eval_interval(tape, lo[k], hi[k], 0.25)
+        assert same_bits(got[k], [rlo, rhi]), (name, k)
+        assert np.array_equal(ch[k], rch), (name, k)
+
+
+@pytest.mark.parametrize("name", sorted(TAPES))
+def test_eval_gradients(ctx, oracle, name):
+    tape = TAPES[name]()
+    th = ctx.upload(tape)
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-1, 1, (2048, 3)).astype(np.float32)
+    got = ctx.eval_gradients(th, p, 0.5)
+    ref = np.array([oracle.eval_gradient(tape, *q, 0.5) for q in p], dtype=np.float32)
+    finite = np.isfinite(ref) & np.isfinite(got)
+    assert np.array_equal(np.isfinite(ref), np.isfinite(got))
+    assert ulp_diff(got[finite], ref[finite]).max(initial=0) <= 2
+
+
+CASES = [
+    ("spheres8", [8, 8, 4]),       # 256^3, the plain small workload
+    ("spheres8", [4, 4, 4, 4]),    # 4 levels
+    ("csg24", [8, 8, 4]),
+    ("csg24", [16, 8, 8]),         # 1024^3, the headline workload
+    ("csg24", [12, 8]),            # non power-of-two branching
+    ("trig", [8, 8, 4]),
+    ("trig", [16, 4]),
+]
+
+
+@pytest.mark.parametrize("name,dims", CASES)
+def test_voxelize_canonical_bit_exact(ctx, oracle, name, dims):
+    tape = TAPES[name]()
+    grid = shapes.make_grid(dims, tape_time=0.5 if name == "trig" else 0.0)
+    th = ctx.upload(tape)
+    tree = ctx.voxelize(th, grid)
+    got = ctx.export(tree)
+    ref = oracle.Tree(tape, grid)
+    want = ref.canonical()
+    info = ctx.tree_info(tree)
+    assert list(info.node_count[:len(dims)]) == ref.counts()
+    assert got.shape == want.shape and np.array_equal(got, want)
+    ctx.tree_free(tree)
+
+
+def test_voxelize_deterministic_and_reusable(ctx, oracle):
+    tape = shapes.random_csg(24, 2)
+    grid = shapes.make_grid([16, 8, 8])
+    th = ctx.upload(tape)
+    tree = ctx.tree_create()
+    ctx.voxelize(th, grid, tree)
+    first = ctx.export(tree)
+    # a different shape into the same tree, then the first again
+    t1 = ctx.upload(shapes.spheres_union(8, 1))
+    ctx.voxelize(t1, shapes.make_grid([8, 8, 4]), tree)
+    ctx.voxelize(th, grid, tree)
+    assert np.array_equal(ctx.export(tree), first)
+    ctx.tree_free(tree)
+
+
+def test_edge_shapes(ctx, oracle):
+    b = Builder()
+    empty = compile_tape(b, b.add(b.const(1.0), b.mul(b.x(), b.const(0.0))))
+    b2 = Builder()
+    full = compile_tape(b2, b2.sub(b2.mul(b2.x(), b2.const(0.0)), b2.const(1.0)))
+    for tape in (empty, full):
+        grid = shapes.make_grid([8, 4])
+        th = ctx.upload(tape)
+        tree = ctx.voxelize(th, grid)
+        assert np.array_equal(ctx.export(tree), oracle.Tree(tape, grid).canonical())
+
+
+def test_bad_tapes_rejected(ctx):
+    good = shapes.spheres_union(2, 3)
+    w = good.words.copy()
+    bad_op = w.copy(); bad_op[1] = (bad_op[1] & ~0xFF) | 13
+    bad_slot = w.copy(); bad_slot[2] = (bad_slot[2] & ~0xFF00) | (200 << 8)
+    bad_const = w.copy(); bad_const[1] = (bad_const[1] & 0xFFFF) | (250 << 16)
+    for words in (bad_op, bad_slot, bad_const):
+        with pytest.raises(lib.FrepError) as e:
+            ctx.upload_raw(words, good.consts, good.slot_count)
+        assert e.value.status == lib.FREP_BAD_TAPE
+    with pytest.raises(lib.FrepError) as e:
+        ctx.upload_raw(w[:5], good.consts, good.slot_count)
+    assert e.value.status == lib.FREP_BAD_TAPE
+    th = ctx.upload(good)
+    with pytest.raises(lib.FrepError) as e:
+        ctx.voxelize(th, shapes.make_grid([6, 4]))
+    assert e.value.status == lib.FREP_UNSUPPORTED
+
+
+RENDERS = [
+    ("spheres8", [8, 8, 4], dict(pos=(0, 0, 3), fwd=(0, 0, -1), up=(0, 1, 0), w=128, h=96,
+                                 ortho=False, spp=1, jitter=False)),
+    ("spheres8", [8, 8, 4], dict(pos=(0, 0, 3), fwd=(0, 0, -1), up=(0, 1, 0), w=96, h=96,
+                                 ortho=True, spp=1, jitter=False)),
+    ("csg24", [16, 8, 8], dict(pos=(1.8, 1.2, 2.4), fwd=(-0.55, -0.35, -0.75), up=(0, 1, 0),
+                               w=160, h=90, ortho=False, spp=2, jitter=True)),
+    ("trig", [8, 8, 4], dict(pos=(0.3, 2.5, 0.2), fwd=(-0.1, -1, -0.05), up=(0, 0, 1), w=96,
+                             h=80, ortho=False, spp=1, jitter=True)),
+]
+
+
+@pytest.mark.parametrize("name,dims,cfg", RENDERS)
+def test_render_bit_exact(ctx, oracle, name, dims, cfg):
+    tape = TAPES[name]()
+    grid = shapes.make_grid(dims, tape_time=0.5 if name == "trig" else 0.0)
+    th = ctx.upload(tape)
+    tree = ctx.voxelize(th, grid)
+    basis = shapes.camera_basis(cfg["pos"], cfg["fwd"], cfg["up"], 45.0, cfg["w"], cfg["h"],
+                                ortho=cfg["ortho"], extent=(2.4, 2.4))
+    kw = dict(spp=cfg["spp"], ortho=cfg["ortho"], jitter=cfg["jitter"], seed=1234)
+    rgba, hits, nrm = ctx.render(tree, lib.make_camera(basis, cfg["w"], cfg["h"], **kw))
+    ohits, onrm, orgba = oracle.Tree(tape, grid).render(
+        oracle.make_camera(basis, cfg["w"], cfg["h"], **kw))
+    assert (hits != MISS).sum() > 0
+    assert np.array_equal(hits, ohits)
+    hit = hits != MISS
+    assert ulp_diff(nrm[hit], onrm[hit]).max(initial=0) <= 2
+    assert np.array_equal(rgba, orgba)
+    ctx.tree_free(tree)

@@ -219,7 +219,8 @@ frep_status frep_init(int device, frep_ctx** out) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     c->cu_count = (uint32_t)prop.multiProcessorCount;
-    c->level_grid = c->cu_count * 4;
+    // persistent level kernels: 8 blocks of 4 wavefronts per CU
+    c->level_grid = c->cu_count * 8;
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     *out = c.release();
@@ -426,6 +427,9 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             a.pool_cap = t->pool_cap;
             a.level = L;
             a.dim = grid->dims[L];
+            a.dim_shift = 0;
+            if ((a.dim & (a.dim - 1)) == 0)
+                while ((1u << a.dim_shift) < a.dim) ++a.dim_shift;
             a.node_cap = t->node_cap[L];
             a.child_cap = last ? 0 : t->node_cap[L + 1];
             a.ns = t->ns[L];
@@ -436,8 +440,11 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             a.low[1] = grid->low[1];
             a.low[2] = grid->low[2];
             a.tape_time = grid->tape_time;
-            if (last) HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, ctx->level_grid));
-            else HIP_TRY(launch_level_interval(ctx->stream, a, t->maxs, ctx->level_grid));
+            if (last)
+                HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, ctx->cu_count, ctx->level_grid));
+            else
+                HIP_TRY(launch_level_interval(ctx->stream, a, t->maxs, ctx->cu_count,
+                                              ctx->level_grid));
             if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
         }
         HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4, hipMemcpyDeviceToHost,

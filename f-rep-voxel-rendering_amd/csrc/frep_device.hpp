@@ -107,96 +107,105 @@ __device__ __forceinline__ float dev_cosf(float x) {
 }
 
 // ---------------------------------------------------------------------------
-// Point rules (vulkan/shaders/tape/point.comp:19-59).
+// Point rules (vulkan/shaders/tape/point.comp:19-59), K points per lane.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float point_op(uint32_t op, float a, float b, float c) {
-    switch (op) {
-        case OP_CONST: return c;
-        case OP_SIN:   return dev_sinf(a);
-        case OP_COS:   return dev_cosf(a);
-        case OP_ADD:   return a + b;
-        case OP_SUB:   return a - b;
-        case OP_MUL:   return a * b;
-        case OP_DIV:   return a / b;
-        case OP_MIN:   return fmin_glm(a, b);
-        case OP_MAX:   return fmax_glm(a, b);
-        case OP_EXP:   return dev_expf(a);
-        case OP_NEG:   return -a;
-        case OP_SQRT:  return __builtin_sqrtf(a);
-        default:       return a;
-    }
+// Reads the next four tape words at once (one 16-byte scalar load); words
+// past the end of a tape lie in the padding every tape buffer carries.
+__device__ __forceinline__ void fetch4(ctape_t words, uint32_t i, uint32_t (&w)[4]) {
+    w[0] = words[i]; w[1] = words[i + 1]; w[2] = words[i + 2]; w[3] = words[i + 3];
 }
 
-// Evaluates a tape at K points per lane.  s[slot][k] lives in VGPRs when
-// MAXS*K is small enough for the compiler to keep the array in registers;
-// instruction fields are wave-uniform (scalar loads, scalar decode).
-template <int MAXS, int K>
-__device__ __forceinline__ void eval_point_k(ctape_t tape, cconst_t consts, float (&s)[MAXS][K]) {
-    const uint32_t n = tape[0];
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t w = tape[1 + i];
-        // slot fields masked to the register file; CONST uses the raw index
-        const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-        const uint32_t a = ((w >> 16) & 0xFFu) & (MAXS - 1), b = (w >> 24) & (MAXS - 1);
-        float r[K];
-        switch (op) {
-            case OP_CONST: {
-                const float c = consts[w >> 16];
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = c;
-            } break;
-            case OP_ADD:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = s[a][k] + s[b][k];
-                break;
-            case OP_SUB:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = s[a][k] - s[b][k];
-                break;
-            case OP_MUL:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = s[a][k] * s[b][k];
-                break;
-            case OP_MIN:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = fmin_glm(s[a][k], s[b][k]);
-                break;
-            case OP_MAX:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = fmax_glm(s[a][k], s[b][k]);
-                break;
-            case OP_SQRT:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = __builtin_sqrtf(s[a][k]);
-                break;
-            case OP_NEG:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = -s[a][k];
-                break;
-            case OP_DIV:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = s[a][k] / s[b][k];
-                break;
-            case OP_SIN:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = dev_sinf(s[a][k]);
-                break;
-            case OP_COS:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = dev_cosf(s[a][k]);
-                break;
-            case OP_EXP:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = dev_expf(s[a][k]);
-                break;
-            default:
-#pragma unroll
-                for (int k = 0; k < K; ++k) r[k] = s[a][k];
-                break;
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) s[o][k] = r[k];
+#define FREP_BANKS8(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+
+// Evaluates a tape at K <= 8 points per lane.  Each point has its own slot
+// array: the compiler keeps every array of <= 32 registers in VGPRs and
+// indexes it with the wave-uniform slot number (s_set_gpr_idx), whereas a
+// single MAXS*K array would spill.  Operand reads happen inside the op
+// switch and the indexed write once after it, so no register file is copied
+// at the switch's join.  init(k) -> float3 gives point k; out(k, v) receives
+// its density.
+template <int MAXS, int K, class Init, class Out>
+__device__ __forceinline__ void run_point_tape(ctape_t tape, cconst_t consts, float t, Init init,
+                                               Out out) {
+    static_assert(K >= 1 && K <= 8, "1..8 points per lane");
+    float s0[MAXS], s1[MAXS], s2[MAXS], s3[MAXS], s4[MAXS], s5[MAXS], s6[MAXS], s7[MAXS];
+#define FREP_INIT(k)                                                         \
+    if constexpr (k < K) {                                                   \
+        const float3 p = init(k);                                            \
+        s##k[0] = p.x; s##k[1] = p.y; s##k[2] = p.z; s##k[3] = t;            \
     }
+    FREP_BANKS8(FREP_INIT)
+#undef FREP_INIT
+    const uint32_t n = tape[0];
+    for (uint32_t i = 0; i < n; i += 4) {
+        uint32_t wq[4];
+        fetch4(tape, 1 + i, wq);
+        const uint32_t m = n - i < 4 ? n - i : 4;
+        // the four words rotate through wq[0] so the body is emitted once
+        for (uint32_t j = 0; j < m; ++j) {
+            {
+                const uint32_t w = wq[0];
+                wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
+                // slot fields masked to the register file; CONST uses the raw
+                // 16-bit pool index
+                const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+                const uint32_t a = ((w >> 16) & 0xFFu) & (MAXS - 1), b = (w >> 24) & (MAXS - 1);
+                float r0, r1, r2, r3, r4, r5, r6, r7;
+#define FREP_B(k, EXPR) if constexpr (k < K) { r##k = EXPR; }
+#define FREP_ADD(k) FREP_B(k, s##k[a] + s##k[b])
+#define FREP_SUB(k) FREP_B(k, s##k[a] - s##k[b])
+#define FREP_MUL(k) FREP_B(k, s##k[a] * s##k[b])
+#define FREP_MIN(k) FREP_B(k, fmin_glm(s##k[a], s##k[b]))
+#define FREP_MAX(k) FREP_B(k, fmax_glm(s##k[a], s##k[b]))
+#define FREP_SQRT(k) FREP_B(k, __builtin_sqrtf(s##k[a]))
+#define FREP_NEG(k) FREP_B(k, -s##k[a])
+#define FREP_COPY(k) FREP_B(k, s##k[a])
+#define FREP_CST(k) FREP_B(k, c)
+#define FREP_DIV(k) FREP_B(k, s##k[a] / s##k[b])
+#define FREP_SIN(k) FREP_B(k, dev_sinf(s##k[a]))
+#define FREP_COS(k) FREP_B(k, dev_cosf(s##k[a]))
+#define FREP_EXP(k) FREP_B(k, dev_expf(s##k[a]))
+                switch (op) {
+                    case OP_CONST: { const float c = consts[w >> 16]; FREP_BANKS8(FREP_CST) } break;
+                    case OP_ADD: { FREP_BANKS8(FREP_ADD) } break;
+                    case OP_SUB: { FREP_BANKS8(FREP_SUB) } break;
+                    case OP_MUL: { FREP_BANKS8(FREP_MUL) } break;
+                    case OP_MIN: { FREP_BANKS8(FREP_MIN) } break;
+                    case OP_MAX: { FREP_BANKS8(FREP_MAX) } break;
+                    case OP_SQRT: { FREP_BANKS8(FREP_SQRT) } break;
+                    case OP_NEG: { FREP_BANKS8(FREP_NEG) } break;
+                    case OP_COPY: { FREP_BANKS8(FREP_COPY) } break;
+                    default:
+                        // rare ops behind one more uniform branch
+                        if (op == OP_DIV) { FREP_BANKS8(FREP_DIV) }
+                        else if (op == OP_SIN) { FREP_BANKS8(FREP_SIN) }
+                        else if (op == OP_COS) { FREP_BANKS8(FREP_COS) }
+                        else { FREP_BANKS8(FREP_EXP) }
+                        break;
+                }
+#define FREP_WR(k) if constexpr (k < K) { s##k[o] = r##k; }
+                FREP_BANKS8(FREP_WR)
+#undef FREP_WR
+#undef FREP_B
+#undef FREP_ADD
+#undef FREP_SUB
+#undef FREP_MUL
+#undef FREP_MIN
+#undef FREP_MAX
+#undef FREP_SQRT
+#undef FREP_NEG
+#undef FREP_COPY
+#undef FREP_CST
+#undef FREP_DIV
+#undef FREP_SIN
+#undef FREP_COS
+#undef FREP_EXP
+            }
+        }
+    }
+#define FREP_OUT(k) if constexpr (k < K) { out(k, s##k[0]); }
+    FREP_BANKS8(FREP_OUT)
+#undef FREP_OUT
 }
 
 // ---------------------------------------------------------------------------

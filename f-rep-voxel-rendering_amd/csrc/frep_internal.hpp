@@ -52,6 +52,7 @@ struct LevelArgs {
     uint64_t pool_cap;        // words
     uint32_t level;
     uint32_t dim;
+    uint32_t dim_shift;       // log2(dim) when dim is a power of two, else 0
     uint32_t node_cap;        // capacity of this level
     uint32_t child_cap;       // capacity of the next level
     uint32_t ns;              // record words, this level
@@ -95,8 +96,12 @@ struct RenderArgs {
 // Launchers (frep_kernels.hip).  Each returns hipSuccess or the launch error.
 hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0, uint32_t* pool,
                             const uint32_t* root_tape, uint32_t root_off, uint32_t n_levels);
-hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid);
-hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid);
+// Level kernels run a persistent grid of min(max_grid, cu_count x resident
+// blocks per CU) blocks; max_grid bounds the per-wave scratch rows.
+hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
+                                 uint32_t cu_count, uint32_t max_grid);
+hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
+                              uint32_t cu_count, uint32_t max_grid);
 hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs);
 hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
                               const float* xyz, uint32_t n, float t, float* out, uint32_t maxs);

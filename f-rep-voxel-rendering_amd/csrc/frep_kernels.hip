@@ -32,6 +32,21 @@ __device__ __forceinline__ void store_mask64(uint32_t* dst, uint64_t m, uint32_t
     if (lane < 2) dst[lane] = lane ? (uint32_t)(m >> 32) : (uint32_t)m;
 }
 
+// Cell (x, y, z) of mask bit idx: INDEX3D = z + dim*y + dim*dim*x
+// (vulkan/shaders/utils.comp:6).  Shifts when dim is a power of two.
+__device__ __forceinline__ void cell_of(uint32_t idx, uint32_t dim, uint32_t sh, uint32_t* x,
+                                        uint32_t* y, uint32_t* z) {
+    if (sh) {
+        *x = idx >> (2 * sh);
+        *y = (idx >> sh) & (dim - 1);
+        *z = idx & (dim - 1);
+    } else {
+        *x = idx / (dim * dim);
+        *y = (idx / dim) % dim;
+        *z = idx % dim;
+    }
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -94,9 +109,9 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
         ctape_t hdr = as_const(rec);
         const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
         const uint32_t idx = sub * 64 + lane;
-        const uint32_t gx = ncx * dim + idx / dim2;
-        const uint32_t gy = ncy * dim + (idx / dim) % dim;
-        const uint32_t gz = ncz * dim + idx % dim;
+        uint32_t cx, cy, cz;
+        cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
+        const uint32_t gx = ncx * dim + cx, gy = ncy * dim + cy, gz = ncz * dim + cz;
         // main.comp:113-116, 124-126: box [pos, pos + cell_size]
         const float lx = a.low[0] + (float)gx * a.cs;
         const float ly = a.low[1] + (float)gy * a.cs;
@@ -110,8 +125,12 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
         slo[2] = lz; shi[2] = lz + a.cs;
         slo[3] = a.tape_time; shi[3] = a.tape_time;
         uint32_t mm = 0;
+        uint32_t wq[4];
         for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t w = tape[1 + i];
+            // tape words arrive four at a time and rotate through wq[0]
+            if ((i & 3u) == 0) fetch4(tape, 1 + i, wq);
+            const uint32_t w = wq[0];
+            wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
             const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
             // operand fields are slot indices except for CONST (pool index);
             // masking keeps every register-file access in range
@@ -234,7 +253,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
     const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
     uint32_t n_nodes = uni(load_counter(a.counters + CNT_NODES + a.level));
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
-    const uint32_t dim = a.dim, dim2 = dim * dim, cells = dim2 * dim;
+    const uint32_t dim = a.dim, cells = dim * dim * dim;
     const uint32_t cpw = cells / (64 * K);
     const uint32_t total = n_nodes * cpw;
     cconst_t consts = as_const(a.consts);
@@ -244,24 +263,20 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
         uint32_t* rec = a.nodes + (size_t)node * a.ns;
         ctape_t hdr = as_const(rec);
         const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
-        float s[MAXS][K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t idx = (sub * K + k) * 64 + lane;
-            const uint32_t gx = ncx * dim + idx / dim2;
-            const uint32_t gy = ncy * dim + (idx / dim) % dim;
-            const uint32_t gz = ncz * dim + idx % dim;
-            s[0][k] = a.low[0] + (float)gx * a.cs;
-            s[1][k] = a.low[1] + (float)gy * a.cs;
-            s[2][k] = a.low[2] + (float)gz * a.cs;
-            s[3][k] = a.tape_time;
-        }
-        eval_point_k<MAXS, K>(as_const(a.pool + toff), consts, s);
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint64_t B = __ballot(s[0][k] < 0.0f);
-            store_mask64(rec + 4 + (sub * K + k) * 2, B, lane);
-        }
+        auto init = [&](int k) {
+            const uint32_t idx = (sub * K + (uint32_t)k) * 64 + lane;
+            uint32_t cx, cy, cz;
+            cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
+            // main.comp:77-79: grid low + float(node.coords * dim + cell) * cell size
+            return make_float3(a.low[0] + (float)(ncx * dim + cx) * a.cs,
+                               a.low[1] + (float)(ncy * dim + cy) * a.cs,
+                               a.low[2] + (float)(ncz * dim + cz) * a.cs);
+        };
+        auto out = [&](int k, float d) {
+            const uint64_t B = __ballot(d < 0.0f);
+            store_mask64(rec + 4 + (sub * K + (uint32_t)k) * 2, B, lane);
+        };
+        run_point_tape<MAXS, K>(as_const(a.pool + toff), consts, a.tape_time, init, out);
     }
 }
 
@@ -508,13 +523,9 @@ __global__ __launch_bounds__(256) void k_eval_points(const uint32_t* tape, const
                                                      float* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t j = i < n ? i : n - 1;
-    float s[MAXS][1];
-    s[0][0] = xyz[3 * j + 0];
-    s[1][0] = xyz[3 * j + 1];
-    s[2][0] = xyz[3 * j + 2];
-    s[3][0] = t;
-    eval_point_k<MAXS, 1>(as_const(tape), as_const(consts), s);
-    if (i < n) out[i] = s[0][0];
+    auto init = [&](int) { return make_float3(xyz[3 * j + 0], xyz[3 * j + 1], xyz[3 * j + 2]); };
+    auto put = [&](int, float d) { if (i < n) out[i] = d; };
+    run_point_tape<MAXS, 1>(as_const(tape), as_const(consts), t, init, put);
 }
 
 template <int MAXS>
@@ -589,38 +600,69 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
     return hipGetLastError();
 }
 
-hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid) {
-    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<grid, kLevelBlock, 0, st>>>(a)));
+// Persistent grid of a level kernel: as many 256-thread blocks as can be
+// resident at once (occupancy of that instantiation x CU count), so every
+// block starts immediately and no block waits for another to retire.
+template <class Kern>
+uint32_t resident_grid(Kern kern, uint32_t cu_count, uint32_t max_grid) {
+    int bpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kLevelBlock, 0) != hipSuccess ||
+        bpc < 1)
+        bpc = 1;
+    const uint32_t g = cu_count * (uint32_t)bpc;
+    return g < max_grid ? g : max_grid;
+}
+
+template <int M>
+uint32_t interval_grid(uint32_t cu_count, uint32_t max_grid) {
+    static const uint32_t g = resident_grid(k_level_interval<M>, cu_count, max_grid);
+    return g;
+}
+
+template <int M, int K>
+uint32_t point_grid(uint32_t cu_count, uint32_t max_grid) {
+    static const uint32_t g = resident_grid(k_level_point<M, K>, cu_count, max_grid);
+    return g;
+}
+
+hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
+                                 uint32_t cu_count, uint32_t max_grid) {
+    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<interval_grid<M>(cu_count, max_grid),
+                                                    kLevelBlock, 0, st>>>(a)));
     return hipGetLastError();
 }
 
-// Voxels per lane of the point kernel for a slot-file size (keeps the slot
-// array within the register budget the compiler promotes).
-__host__ __device__ constexpr int point_k_for(int maxs) { return maxs <= 8 ? 4 : (maxs <= 16 ? 2 : 1); }
-
-hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs, uint32_t grid) {
-    const uint32_t cpn = a.dim * a.dim * a.dim / 64;
-#define FREP_POINT_CALL                                                              \
-    {                                                                                \
-        constexpr int KM = point_k_for(M);                                           \
-        if (KM >= 4 && cpn % 4 == 0)                                                 \
-            k_level_point<M, (KM >= 4 ? 4 : 1)><<<grid, kLevelBlock, 0, st>>>(a);   \
-        else if (KM >= 2 && cpn % 2 == 0)                                            \
-            k_level_point<M, (KM >= 2 ? 2 : 1)><<<grid, kLevelBlock, 0, st>>>(a);   \
-        else                                                                         \
-            k_level_point<M, 1><<<grid, kLevelBlock, 0, st>>>(a);                    \
-    }
-    FREP_DISPATCH_MAXS(maxs, FREP_POINT_CALL);
-#undef FREP_POINT_CALL
-    return hipGetLastError();
+// Largest voxels-per-lane count for a slot-file size: K banks of MAXS
+// registers each, at most 128 slot registers per lane (MAXS >= 64 banks do
+// not fit in registers, so one bank is used).
+__host__ __device__ constexpr int point_k_for(int maxs) {
+    return maxs <= 16 ? 8 : (maxs <= 32 ? 4 : 1);
 }
 
 uint32_t point_k_used(uint32_t maxs, uint32_t dim) {
-    const uint32_t cpn = dim * dim * dim / 64;
-    const int km = point_k_for((int)maxs);
-    if (km >= 4 && cpn % 4 == 0) return 4;
-    if (km >= 2 && cpn % 2 == 0) return 2;
-    return 1;
+    const uint32_t cpn = dim * dim * dim / 64;   // wavefront chunks per node
+    uint32_t k = (uint32_t)point_k_for((int)maxs);
+    while (k > 1 && cpn % k != 0) k /= 2;
+    return k;
+}
+
+hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
+                              uint32_t cu_count, uint32_t max_grid) {
+    const uint32_t k = point_k_used(maxs, a.dim);
+#define FREP_POINT_LAUNCH(KK)                                                                 \
+    k_level_point<M, KK><<<point_grid<M, KK>(cu_count, max_grid), kLevelBlock, 0, st>>>(a)
+#define FREP_POINT_CALL                                                                       \
+    {                                                                                         \
+        constexpr int KM = point_k_for(M);                                                    \
+        if (k == 8) FREP_POINT_LAUNCH((KM >= 8 ? 8 : 1));                                     \
+        else if (k == 4) FREP_POINT_LAUNCH((KM >= 4 ? 4 : 1));                                \
+        else if (k == 2) FREP_POINT_LAUNCH((KM >= 2 ? 2 : 1));                                \
+        else FREP_POINT_LAUNCH(1);                                                            \
+    }
+    FREP_DISPATCH_MAXS(maxs, FREP_POINT_CALL);
+#undef FREP_POINT_CALL
+#undef FREP_POINT_LAUNCH
+    return hipGetLastError();
 }
 
 hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs) {

@@ -164,10 +164,16 @@ def test_edge_shapes(ctx, oracle):
 
 def test_bad_tapes_rejected(ctx):
     good = shapes.spheres_union(2, 3)
-    w = good.words.copy()
-    bad_op = w.copy(); bad_op[1] = (bad_op[1] & ~0xFF) | 13
-    bad_slot = w.copy(); bad_slot[2] = (bad_slot[2] & ~0xFF00) | (200 << 8)
-    bad_const = w.copy(); bad_const[1] = (bad_const[1] & 0xFFFF) | (250 << 16)
+    w = [int(v) for v in good.words]
+
+    def patched(i, value):
+        v = list(w)
+        v[i] = value
+        return np.array(v, dtype=np.uint32)
+
+    bad_op = patched(1, (w[1] & 0xFFFFFF00) | 13)                  # op 13 does not exist
+    bad_slot = patched(2, (w[2] & 0xFFFF00FF) | (200 << 8))        # out slot >= slot_count
+    bad_const = patched(1, (w[1] & 0xFFFF) | (250 << 16))          # pool index >= n_consts
     for words in (bad_op, bad_slot, bad_const):
         with pytest.raises(lib.FrepError) as e:
             ctx.upload_raw(words, good.consts, good.slot_count)

@@ -73,8 +73,6 @@ struct frep_ctx {
     std::mutex mu;
     std::vector<std::unique_ptr<TapeObj>> tapes;
     std::vector<std::unique_ptr<TreeObj>> trees;
-    uint4* d_scratch = nullptr;
-    size_t scratch_bytes = 0;
     bool timing = false;
     hipEvent_t ev[kMaxLevels + 4] = {};
     frep_timing last{};
@@ -149,16 +147,6 @@ frep_status check_grid(frep_ctx* ctx, const frep_grid* g) {
         return fail(ctx, FREP_UNSUPPORTED, "finest resolution above 2^20 per axis");
     if (!(g->world_size > 0.0f) || !std::isfinite(g->world_size))
         return fail(ctx, FREP_INVALID_ARG, "world_size must be positive and finite");
-    return FREP_OK;
-}
-
-frep_status ensure_scratch(frep_ctx* ctx, size_t bytes) {
-    if (bytes <= ctx->scratch_bytes) return FREP_OK;
-    if (ctx->d_scratch) HIP_TRY(hipFree(ctx->d_scratch));
-    ctx->d_scratch = nullptr;
-    ctx->scratch_bytes = 0;
-    HIP_TRY(hipMalloc(&ctx->d_scratch, bytes));
-    ctx->scratch_bytes = bytes;
     return FREP_OK;
 }
 
@@ -242,7 +230,6 @@ void frep_shutdown(frep_ctx* ctx) {
         if (t->d_counters) (void)hipFree(t->d_counters);
         if (t->h_counters) (void)hipHostFree(t->h_counters);
     }
-    if (ctx->d_scratch) (void)hipFree(ctx->d_scratch);
     if (ctx->d_out) (void)hipFree(ctx->d_out);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
@@ -395,12 +382,11 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
     if (nc)
         HIP_TRY(hipMemcpyAsync(t->d_consts, tp->d_consts, (size_t)nc * 4, hipMemcpyDeviceToDevice,
                                ctx->stream));
-    const size_t waves = (size_t)ctx->level_grid * (kLevelBlock / 64);
+    // min/max choice rows live in the interval kernel's LDS
     const uint32_t mm_stride = std::max<uint32_t>(tp->n_mm, 1);
-    {
-        const frep_status s = ensure_scratch(ctx, waves * mm_stride * sizeof(uint4));
-        if (s != FREP_OK) return s;
-    }
+    if (nl > 1 && mm_stride > kMaxMinMax)
+        return fail(ctx, FREP_UNSUPPORTED,
+                    "tape has more than " + std::to_string(kMaxMinMax) + " MIN/MAX instructions");
     t->retries = 0;
     for (uint32_t attempt = 0;; ++attempt) {
         for (uint32_t L = 0; L < nl; ++L) {
@@ -423,7 +409,6 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             a.pool = t->d_pool;
             a.consts = t->d_consts;
             a.counters = t->d_counters;
-            a.choice_scratch = ctx->d_scratch;
             a.pool_cap = t->pool_cap;
             a.level = L;
             a.dim = grid->dims[L];

@@ -117,13 +117,27 @@ __device__ __forceinline__ void fetch4(ctape_t words, uint32_t i, uint32_t (&w)[
 
 #define FREP_BANKS8(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
-// Evaluates a tape at K <= 8 points per lane.  Each point has its own slot
-// array: the compiler keeps every array of <= 32 registers in VGPRs and
-// indexes it with the wave-uniform slot number (s_set_gpr_idx), whereas a
-// single MAXS*K array would spill.  Operand reads happen inside the op
-// switch and the indexed write once after it, so no register file is copied
-// at the switch's join.  init(k) -> float3 gives point k; out(k, v) receives
-// its density.
+// Scheduling fence: keeps each operand's register reads together so the
+// compiler opens one s_set_gpr_idx window per operand instead of one per
+// register move.
+#define FREP_FENCE() __builtin_amdgcn_sched_barrier(0)
+
+// Evaluates a tape at K <= 8 points per lane.
+//
+// Register file: each point has its own slot array.  The compiler keeps every
+// array of <= 32 registers in VGPRs and indexes it with the wave-uniform slot
+// number (s_set_gpr_idx / M0), whereas a single MAXS*K array would spill to
+// scratch.  Per tape instruction the K reads of operand A share one index
+// window, the K reads of B another, and the K writes of the result a third,
+// so an ADD costs 3K register moves + K adds + ~6 scalar instructions.
+//
+// Decode: instruction words arrive four at a time with one scalar load, and
+// the next group is requested before the current one is interpreted.
+//
+// Dispatch: a balanced test on the op-code bits (4 uniform branches) instead
+// of a switch, which the compiler lowers to a long compare chain.
+//
+// init(k) -> float3 gives point k; out(k, v) receives its density.
 template <int MAXS, int K, class Init, class Out>
 __device__ __forceinline__ void run_point_tape(ctape_t tape, cconst_t consts, float t, Init init,
                                                Out out) {
@@ -137,70 +151,95 @@ __device__ __forceinline__ void run_point_tape(ctape_t tape, cconst_t consts, fl
     FREP_BANKS8(FREP_INIT)
 #undef FREP_INIT
     const uint32_t n = tape[0];
+    uint32_t nq[4];
+    fetch4(tape, 1, nq);
     for (uint32_t i = 0; i < n; i += 4) {
-        uint32_t wq[4];
-        fetch4(tape, 1 + i, wq);
+        uint32_t wq[4] = {nq[0], nq[1], nq[2], nq[3]};
+        if (i + 4 < n) fetch4(tape, 5 + i, nq);                 // prefetch next group
         const uint32_t m = n - i < 4 ? n - i : 4;
         // the four words rotate through wq[0] so the body is emitted once
         for (uint32_t j = 0; j < m; ++j) {
-            {
-                const uint32_t w = wq[0];
-                wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
-                // slot fields masked to the register file; CONST uses the raw
-                // 16-bit pool index
-                const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-                const uint32_t a = ((w >> 16) & 0xFFu) & (MAXS - 1), b = (w >> 24) & (MAXS - 1);
-                float r0, r1, r2, r3, r4, r5, r6, r7;
+            const uint32_t w = wq[0];
+            wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
+            // slot fields masked to the register file; CONST uses the raw
+            // 16-bit pool index
+            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+            const uint32_t a = ((w >> 16) & 0xFFu) & (MAXS - 1), b = (w >> 24) & (MAXS - 1);
+#define FREP_RA(k) float A##k = 0.0f; if constexpr (k < K) { A##k = s##k[a]; }
+#define FREP_RB(k) float B##k = 0.0f; if constexpr (k < K) { B##k = s##k[b]; }
 #define FREP_B(k, EXPR) if constexpr (k < K) { r##k = EXPR; }
-#define FREP_ADD(k) FREP_B(k, s##k[a] + s##k[b])
-#define FREP_SUB(k) FREP_B(k, s##k[a] - s##k[b])
-#define FREP_MUL(k) FREP_B(k, s##k[a] * s##k[b])
-#define FREP_MIN(k) FREP_B(k, fmin_glm(s##k[a], s##k[b]))
-#define FREP_MAX(k) FREP_B(k, fmax_glm(s##k[a], s##k[b]))
-#define FREP_SQRT(k) FREP_B(k, __builtin_sqrtf(s##k[a]))
-#define FREP_NEG(k) FREP_B(k, -s##k[a])
-#define FREP_COPY(k) FREP_B(k, s##k[a])
+#define FREP_ADD(k) FREP_B(k, A##k + B##k)
+#define FREP_SUB(k) FREP_B(k, A##k - B##k)
+#define FREP_MUL(k) FREP_B(k, A##k * B##k)
+#define FREP_DIV(k) FREP_B(k, A##k / B##k)
+#define FREP_MIN(k) FREP_B(k, fmin_glm(A##k, B##k))
+#define FREP_MAX(k) FREP_B(k, fmax_glm(A##k, B##k))
+#define FREP_SQRT(k) FREP_B(k, __builtin_sqrtf(A##k))
+#define FREP_NEG(k) FREP_B(k, -A##k)
+#define FREP_COPY(k) FREP_B(k, A##k)
 #define FREP_CST(k) FREP_B(k, c)
-#define FREP_DIV(k) FREP_B(k, s##k[a] / s##k[b])
-#define FREP_SIN(k) FREP_B(k, dev_sinf(s##k[a]))
-#define FREP_COS(k) FREP_B(k, dev_cosf(s##k[a]))
-#define FREP_EXP(k) FREP_B(k, dev_expf(s##k[a]))
-                switch (op) {
-                    case OP_CONST: { const float c = consts[w >> 16]; FREP_BANKS8(FREP_CST) } break;
-                    case OP_ADD: { FREP_BANKS8(FREP_ADD) } break;
-                    case OP_SUB: { FREP_BANKS8(FREP_SUB) } break;
-                    case OP_MUL: { FREP_BANKS8(FREP_MUL) } break;
-                    case OP_MIN: { FREP_BANKS8(FREP_MIN) } break;
-                    case OP_MAX: { FREP_BANKS8(FREP_MAX) } break;
-                    case OP_SQRT: { FREP_BANKS8(FREP_SQRT) } break;
-                    case OP_NEG: { FREP_BANKS8(FREP_NEG) } break;
-                    case OP_COPY: { FREP_BANKS8(FREP_COPY) } break;
-                    default:
-                        // rare ops behind one more uniform branch
-                        if (op == OP_DIV) { FREP_BANKS8(FREP_DIV) }
-                        else if (op == OP_SIN) { FREP_BANKS8(FREP_SIN) }
-                        else if (op == OP_COS) { FREP_BANKS8(FREP_COS) }
-                        else { FREP_BANKS8(FREP_EXP) }
-                        break;
+#define FREP_SIN(k) FREP_B(k, dev_sinf(A##k))
+#define FREP_COS(k) FREP_B(k, dev_cosf(A##k))
+#define FREP_EXP(k) FREP_B(k, dev_expf(A##k))
+            FREP_BANKS8(FREP_RA)
+            FREP_FENCE();
+            float r0 = 0, r1 = 0, r2 = 0, r3 = 0, r4 = 0, r5 = 0, r6 = 0, r7 = 0;
+            // op codes: 0 CONST 1 SIN 2 COS 3 ADD 4 SUB 5 MUL 6 DIV 7 MIN
+            //           8 MAX 9 EXP 10 NEG 11 SQRT 12 COPY
+            if (op & 8u) {
+                if (op & 4u) {
+                    FREP_BANKS8(FREP_COPY)
+                } else if (op & 2u) {
+                    if (op & 1u) { FREP_BANKS8(FREP_SQRT) } else { FREP_BANKS8(FREP_NEG) }
+                } else if (op & 1u) {
+                    FREP_BANKS8(FREP_EXP)
+                } else {
+                    FREP_BANKS8(FREP_RB)
+                    FREP_FENCE();
+                    FREP_BANKS8(FREP_MAX)
                 }
+            } else if (op & 4u) {
+                FREP_BANKS8(FREP_RB)
+                FREP_FENCE();
+                if (op & 2u) {
+                    if (op & 1u) { FREP_BANKS8(FREP_MIN) } else { FREP_BANKS8(FREP_DIV) }
+                } else {
+                    if (op & 1u) { FREP_BANKS8(FREP_MUL) } else { FREP_BANKS8(FREP_SUB) }
+                }
+            } else if (op & 2u) {
+                if (op & 1u) {
+                    FREP_BANKS8(FREP_RB)
+                    FREP_FENCE();
+                    FREP_BANKS8(FREP_ADD)
+                } else {
+                    FREP_BANKS8(FREP_COS)
+                }
+            } else if (op & 1u) {
+                FREP_BANKS8(FREP_SIN)
+            } else {
+                const float c = consts[w >> 16];
+                FREP_BANKS8(FREP_CST)
+            }
+            FREP_FENCE();
 #define FREP_WR(k) if constexpr (k < K) { s##k[o] = r##k; }
-                FREP_BANKS8(FREP_WR)
+            FREP_BANKS8(FREP_WR)
 #undef FREP_WR
+#undef FREP_RA
+#undef FREP_RB
 #undef FREP_B
 #undef FREP_ADD
 #undef FREP_SUB
 #undef FREP_MUL
+#undef FREP_DIV
 #undef FREP_MIN
 #undef FREP_MAX
 #undef FREP_SQRT
 #undef FREP_NEG
 #undef FREP_COPY
 #undef FREP_CST
-#undef FREP_DIV
 #undef FREP_SIN
 #undef FREP_COS
 #undef FREP_EXP
-            }
         }
     }
 #define FREP_OUT(k) if constexpr (k < K) { out(k, s##k[0]); }

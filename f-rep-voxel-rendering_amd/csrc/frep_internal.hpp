@@ -48,7 +48,6 @@ struct LevelArgs {
     uint32_t* pool;           // tape pool
     const float* consts;      // constant pool of the shape
     uint32_t* counters;       // CNT_* block
-    uint4* choice_scratch;    // per-wave min/max choice ballots
     uint64_t pool_cap;        // words
     uint32_t level;
     uint32_t dim;
@@ -57,7 +56,7 @@ struct LevelArgs {
     uint32_t child_cap;       // capacity of the next level
     uint32_t ns;              // record words, this level
     uint32_t child_ns;        // record words, next level
-    uint32_t mm_stride;       // uint4 entries per wave in choice_scratch
+    uint32_t mm_stride;       // LDS choice rows per wavefront (>= MIN/MAX count)
     float cs;                 // cell size at this level
     float low[3];
     float tape_time;
@@ -112,7 +111,13 @@ hipError_t launch_eval_gradients(hipStream_t st, const uint32_t* tape, const flo
                                  const float* xyz, uint32_t n, float t, float* out, uint32_t maxs);
 hipError_t launch_math(hipStream_t st, uint32_t fn, const float* x, uint32_t n, float* out);
 
-// Waves per block of the persistent level kernels.
+// Threads per block of the persistent level kernels (4 wavefronts).
 constexpr uint32_t kLevelBlock = 256;
+
+// Min/max choice rows of the interval kernel live in LDS: 16 bytes per
+// MIN/MAX instruction per wavefront.  64 KiB per block caps a tape at 1024
+// MIN/MAX instructions.
+constexpr size_t kMaxChoiceLds = 64 * 1024;
+constexpr uint32_t kMaxMinMax = (uint32_t)(kMaxChoiceLds / 16 / (kLevelBlock / 64));
 
 }  // namespace frep

@@ -91,17 +91,22 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 template <int MAXS>
 __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
     constexpr int AW = (MAXS + 31) / 32;
+    // min/max choice ballots of this wavefront's current chunk: mm_stride rows
+    // of {FIRST lanes, SECOND lanes} (two 64-bit masks) per wavefront
+    extern __shared__ uint4 lds_choice[];
     if (load_counter(a.counters + CNT_OVERFLOW) != 0) return;
     const uint32_t lane = lane_id();
-    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
+    const uint32_t wib = uni(threadIdx.x >> 6);
+    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + wib;
     const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
     uint32_t n_nodes = uni(load_counter(a.counters + CNT_NODES + a.level));
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
-    const uint32_t dim = a.dim, dim2 = dim * dim, cells = dim2 * dim, W = cells / 32;
+    const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
     const uint32_t cpn = cells / 64;
     const uint32_t total = n_nodes * cpn;
-    uint4* scr = a.choice_scratch + (size_t)gwave * a.mm_stride;
+    uint4* scr = lds_choice + (size_t)wib * a.mm_stride;
     cconst_t consts = as_const(a.consts);
+    const float kPi = 3.14159265358979323846f;
 
     for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
         const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
@@ -119,36 +124,93 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
         ctape_t tape = as_const(a.pool + toff);
         const uint32_t n = tape[0];
 
+        // ---- forward pass: interval evaluation (interval.comp:121-160) ----
         float slo[MAXS], shi[MAXS];
         slo[0] = lx; shi[0] = lx + a.cs;
         slo[1] = ly; shi[1] = ly + a.cs;
         slo[2] = lz; shi[2] = lz + a.cs;
         slo[3] = a.tape_time; shi[3] = a.tape_time;
         uint32_t mm = 0;
-        uint32_t wq[4];
-        for (uint32_t i = 0; i < n; ++i) {
-            // tape words arrive four at a time and rotate through wq[0]
-            if ((i & 3u) == 0) fetch4(tape, 1 + i, wq);
-            const uint32_t w = wq[0];
-            wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
-            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-            // operand fields are slot indices except for CONST (pool index);
-            // masking keeps every register-file access in range
-            const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
-            const Itv A{slo[ia], shi[ia]}, B{slo[ib], shi[ib]};
-            const float c = (op == OP_CONST) ? consts[w >> 16] : 0.0f;
-            const Itv r = interval_op(op, A, B, c);
-            if (op == OP_MIN || op == OP_MAX) {
-                const uint32_t ch = mm_choice(op, A, B);
-                const uint64_t F = __ballot((ch & MM_FIRST) != 0);
-                const uint64_t S = __ballot((ch & MM_SECOND) != 0);
-                scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
-                                     (uint32_t)(S >> 32));
-                ++mm;
+        uint32_t nq[4];
+        fetch4(tape, 1, nq);
+        for (uint32_t i0 = 0; i0 < n; i0 += 4) {
+            uint32_t wq[4] = {nq[0], nq[1], nq[2], nq[3]};
+            if (i0 + 4 < n) fetch4(tape, 5 + i0, nq);          // prefetch next group
+            const uint32_t m = n - i0 < 4 ? n - i0 : 4;
+            for (uint32_t j = 0; j < m; ++j) {
+                const uint32_t w = wq[0];
+                wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
+                // operand fields are slot indices except for CONST (pool
+                // index); masking keeps every register-file access in range
+                const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+                const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+                const Itv A{slo[ia], shi[ia]};
+                FREP_FENCE();
+                Itv r{0.0f, 0.0f};
+                // op codes: 0 CONST 1 SIN 2 COS 3 ADD 4 SUB 5 MUL 6 DIV 7 MIN
+                //           8 MAX 9 EXP 10 NEG 11 SQRT 12 COPY
+                if (op & 8u) {
+                    if (op & 4u) {
+                        r = A;
+                    } else if (op & 2u) {
+                        if (op & 1u)
+                            r = Itv{__builtin_sqrtf(fmax_glm(0.0f, A.lo)),
+                                    __builtin_sqrtf(fmax_glm(0.0f, A.hi))};
+                        else
+                            r = Itv{-A.hi, -A.lo};
+                    } else if (op & 1u) {
+                        r = Itv{dev_expf(A.lo), dev_expf(A.hi)};
+                    } else {
+                        const Itv B{slo[ib], shi[ib]};
+                        FREP_FENCE();
+                        r = Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
+                        const uint32_t ch = mm_choice(OP_MAX, A, B);
+                        const uint64_t F = __ballot((ch & MM_FIRST) != 0);
+                        const uint64_t S = __ballot((ch & MM_SECOND) != 0);
+                        scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
+                                             (uint32_t)(S >> 32));
+                        ++mm;
+                    }
+                } else if (op & 4u) {
+                    const Itv B{slo[ib], shi[ib]};
+                    FREP_FENCE();
+                    if (op & 2u) {
+                        if (op & 1u) {
+                            r = Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)};
+                            const uint32_t ch = mm_choice(OP_MIN, A, B);
+                            const uint64_t F = __ballot((ch & MM_FIRST) != 0);
+                            const uint64_t S = __ballot((ch & MM_SECOND) != 0);
+                            scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
+                                                 (uint32_t)(S >> 32));
+                            ++mm;
+                        } else {
+                            r = interval_op(OP_DIV, A, B, 0.0f);
+                        }
+                    } else if (op & 1u) {
+                        r = imul(A, B);
+                    } else {
+                        r = Itv{A.lo - B.hi, A.hi - B.lo};
+                    }
+                } else if (op & 2u) {
+                    if (op & 1u) {
+                        const Itv B{slo[ib], shi[ib]};
+                        FREP_FENCE();
+                        r = Itv{A.lo + B.lo, A.hi + B.hi};
+                    } else {
+                        r = interval_op(OP_COS, A, A, 0.0f);
+                    }
+                } else if (op & 1u) {
+                    r = interval_op(OP_SIN, A, A, 0.0f);
+                } else {
+                    const float c = consts[w >> 16];
+                    r = Itv{c, c};
+                }
+                FREP_FENCE();
+                slo[o] = r.lo;
+                shi[o] = r.hi;
             }
-            slo[o] = r.lo;
-            shi[o] = r.hi;
         }
+        (void)kPi;
         // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
         const float ilo = slo[0], ihi = shi[0];
         const bool full = ihi < 0.0f;
@@ -181,6 +243,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
                     atomicOr(a.counters + CNT_OVERFLOW,
                              (nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL));
             } else if (amb) {
+                // ---- backward pass: liveness + compaction (shorten.comp:164-223) ----
                 const uint64_t mybase = pbase + (uint64_t)rank * stride;
                 uint32_t* out = a.pool + mybase;
                 uint32_t act[AW];
@@ -188,17 +251,27 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
                 for (int k = 0; k < AW; ++k) act[k] = 0;
                 act[0] = 1u;                                   // slot 0 is the result
                 uint32_t kept = 0;
-                int32_t j = (int32_t)mm - 1;
+                int32_t jm = (int32_t)mm - 1;
+                // words i-3 .. i arrive together; the group below the first
+                // instruction reads into the slack before every pool tape
+                uint32_t bq[4];
                 for (int32_t i = (int32_t)n - 1; i >= 0; --i) {
-                    const uint32_t w = tape[1 + i];
+                    // signed offset: for i < 3 the group starts up to two
+                    // words before the tape (inside the pool's slack)
+                    if ((((int32_t)n - 1 - i) & 3) == 0) {
+                        ctape_t g = tape + (ptrdiff_t)(i - 2);
+                        bq[0] = g[0]; bq[1] = g[1]; bq[2] = g[2]; bq[3] = g[3];
+                    }
+                    const uint32_t w = bq[3];
+                    bq[3] = bq[2]; bq[2] = bq[1]; bq[1] = bq[0];
                     const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
                     const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1);
                     const uint32_t ib = (w >> 24) & (MAXS - 1);
                     const bool is_mm = (op == OP_MIN || op == OP_MAX);
                     uint32_t ch = 0;
                     if (is_mm) {
-                        const uint4 v = scr[j];
-                        --j;
+                        const uint4 v = scr[jm];
+                        --jm;
                         const uint64_t F = ((uint64_t)v.y << 32) | v.x;
                         const uint64_t S = ((uint64_t)v.w << 32) | v.z;
                         ch = (uint32_t)((F >> lane) & 1u) | ((uint32_t)((S >> lane) & 1u) << 1);
@@ -604,31 +677,51 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
 // resident at once (occupancy of that instantiation x CU count), so every
 // block starts immediately and no block waits for another to retire.
 template <class Kern>
-uint32_t resident_grid(Kern kern, uint32_t cu_count, uint32_t max_grid) {
+uint32_t resident_grid(Kern kern, uint32_t cu_count, uint32_t max_grid, size_t lds) {
     int bpc = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kLevelBlock, 0) != hipSuccess ||
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kLevelBlock, lds) != hipSuccess ||
         bpc < 1)
         bpc = 1;
     const uint32_t g = cu_count * (uint32_t)bpc;
     return g < max_grid ? g : max_grid;
 }
 
+// The occupancy of an interval kernel depends on its dynamic LDS (choice
+// rows); the last answer is kept per instantiation and thread.
 template <int M>
-uint32_t interval_grid(uint32_t cu_count, uint32_t max_grid) {
-    static const uint32_t g = resident_grid(k_level_interval<M>, cu_count, max_grid);
+uint32_t interval_grid(uint32_t cu_count, uint32_t max_grid, size_t lds) {
+    thread_local size_t last_lds = ~(size_t)0;
+    thread_local uint32_t last_cu = 0, last_max = 0, g = 0;
+    if (lds != last_lds || cu_count != last_cu || max_grid != last_max) {
+        g = resident_grid(k_level_interval<M>, cu_count, max_grid, lds);
+        last_lds = lds;
+        last_cu = cu_count;
+        last_max = max_grid;
+    }
     return g;
 }
 
 template <int M, int K>
 uint32_t point_grid(uint32_t cu_count, uint32_t max_grid) {
-    static const uint32_t g = resident_grid(k_level_point<M, K>, cu_count, max_grid);
+    thread_local uint32_t last_cu = 0, last_max = 0, g = 0;
+    if (cu_count != last_cu || max_grid != last_max) {
+        g = resident_grid(k_level_point<M, K>, cu_count, max_grid, 0);
+        last_cu = cu_count;
+        last_max = max_grid;
+    }
     return g;
+}
+
+size_t interval_lds_bytes(uint32_t mm_stride) {
+    return (size_t)(kLevelBlock / 64) * mm_stride * sizeof(uint4);
 }
 
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                                  uint32_t cu_count, uint32_t max_grid) {
-    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<interval_grid<M>(cu_count, max_grid),
-                                                    kLevelBlock, 0, st>>>(a)));
+    const size_t lds = interval_lds_bytes(a.mm_stride);
+    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
+    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<interval_grid<M>(cu_count, max_grid, lds),
+                                                    kLevelBlock, lds, st>>>(a)));
     return hipGetLastError();
 }
 

@@ -222,6 +222,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
         store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
 
         uint32_t child_id = kNoChild;
+        uint32_t kept_len = 0;
         if (Bamb != 0) {
             const uint32_t namb = (uint32_t)__popcll(Bamb);
             const uint32_t rank = lane_rank(Bamb);
@@ -305,8 +306,22 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
                 crec[1] = gx;
                 crec[2] = gy;
                 crec[3] = gz;
-                atomicAdd(a.counters + CNT_LEN_SUM + a.level + 1, kept);
-                atomicMax(a.counters + CNT_LEN_MAX + a.level + 1, kept);
+                kept_len = kept;
+            }
+            // tape-length stats (voxelizer/main.comp:146-147): reduced over the
+            // wavefront first, then one atomic each instead of one per lane
+            if (nodes_ok && pool_ok) {
+                uint32_t sum = kept_len, mx = kept_len;
+#pragma unroll
+                for (int d = 32; d >= 1; d >>= 1) {
+                    sum += (uint32_t)__shfl_xor((int)sum, d);
+                    const uint32_t o2 = (uint32_t)__shfl_xor((int)mx, d);
+                    mx = o2 > mx ? o2 : mx;
+                }
+                if (lane == 0) {
+                    atomicAdd(a.counters + CNT_LEN_SUM + a.level + 1, sum);
+                    atomicMax(a.counters + CNT_LEN_MAX + a.level + 1, mx);
+                }
             }
         }
         rec[4 + 2 * W + idx] = child_id;

@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -17,6 +18,7 @@
 #include <vector>
 
 #include "../../include/frep.h"
+#include "frep_asm.hpp"
 #include "frep_internal.hpp"
 
 namespace frep {
@@ -35,6 +37,7 @@ struct TapeObj {
     std::vector<uint32_t> words;       // [n, instrs]
     std::vector<float> consts;
     uint32_t slot_count = 0, maxs = 0, n_mm = 0;
+    bool asm_ok = false;               // all ops in the assembly interpreter's set
     uint32_t* d_words = nullptr;
     float* d_consts = nullptr;
 };
@@ -53,6 +56,7 @@ struct TreeObj {
     uint32_t* d_counters = nullptr;
     uint32_t* h_counters = nullptr;    // pinned mirror
     uint32_t slot_count = 0, maxs = 0;
+    bool asm_ok = false;
     uint32_t retries = 0;
 };
 
@@ -74,6 +78,7 @@ struct frep_ctx {
     std::vector<std::unique_ptr<TapeObj>> tapes;
     std::vector<std::unique_ptr<TreeObj>> trees;
     bool timing = false;
+    bool no_asm = false;               // FREP_NO_ASM=1: C++ interpreter only
     hipEvent_t ev[kMaxLevels + 4] = {};
     frep_timing last{};
     // render output staging
@@ -209,6 +214,8 @@ frep_status frep_init(int device, frep_ctx** out) {
     c->cu_count = (uint32_t)prop.multiProcessorCount;
     // persistent level kernels: 8 blocks of 4 wavefronts per CU
     c->level_grid = c->cu_count * 8;
+    const char* na = std::getenv("FREP_NO_ASM");
+    c->no_asm = na && na[0] == '1';
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     *out = c.release();
@@ -275,9 +282,11 @@ frep_status frep_tape_upload(frep_ctx* ctx, const uint32_t* words, uint32_t n_wo
     t->consts.assign(consts, consts + n_consts);
     t->slot_count = slot_count;
     t->maxs = maxs_for(slot_count);
+    t->asm_ok = true;
     for (uint32_t k = 0; k < n; ++k) {
         const uint32_t op = words[1 + k] & 0xFFu;
         t->n_mm += (op == OP_MIN || op == OP_MAX);
+        t->asm_ok = t->asm_ok && asm_op_ok(op);
     }
     // padded device copies: scalar loads may read a few words past the end
     HIP_TRY(hipMalloc(&t->d_words, (size_t)(n + 1 + kPoolPad) * 4));
@@ -354,6 +363,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
     t->grid = *grid;
     t->slot_count = tp->slot_count;
     t->maxs = tp->maxs;
+    t->asm_ok = tp->asm_ok && !ctx->no_asm;
     // cell sizes by sequential float division (voxelizer.cpp:210-216)
     float c = grid->world_size;
     for (uint32_t L = 0; L < nl; ++L) {
@@ -426,7 +436,8 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             a.low[2] = grid->low[2];
             a.tape_time = grid->tape_time;
             if (last)
-                HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, ctx->cu_count, ctx->level_grid));
+                HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, ctx->cu_count, ctx->level_grid,
+                                           t->asm_ok));
             else
                 HIP_TRY(launch_level_interval(ctx->stream, a, t->maxs, ctx->cu_count,
                                               ctx->level_grid));
@@ -683,7 +694,7 @@ frep_status frep_eval_points(frep_ctx* ctx, frep_tape_t th, const float* xyz, ui
     HIP_TRY(hipMalloc(&dout.p, (size_t)n * 4));
     HIP_TRY(hipMemcpyAsync(din.p, xyz, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream));
     HIP_TRY(launch_eval_points(ctx->stream, tp->d_words, tp->d_consts, (const float*)din.p, n, t,
-                               (float*)dout.p, tp->maxs));
+                               (float*)dout.p, tp->maxs, tp->asm_ok && !ctx->no_asm));
     HIP_TRY(hipMemcpyAsync(out, dout.p, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     return FREP_OK;

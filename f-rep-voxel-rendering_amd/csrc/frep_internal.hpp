@@ -99,11 +99,13 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
 // blocks per CU) blocks; max_grid bounds the per-wave scratch rows.
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                                  uint32_t cu_count, uint32_t max_grid);
+// asm_ok: every op of the tape is in the assembly interpreter's set.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                              uint32_t cu_count, uint32_t max_grid);
+                              uint32_t cu_count, uint32_t max_grid, bool asm_ok);
 hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs);
 hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
-                              const float* xyz, uint32_t n, float t, float* out, uint32_t maxs);
+                              const float* xyz, uint32_t n, float t, float* out, uint32_t maxs,
+                              bool asm_ok);
 hipError_t launch_eval_intervals(hipStream_t st, const uint32_t* tape, const float* consts,
                                  const float* lo, const float* hi, uint32_t n, float t,
                                  float* out, uint8_t* choices, uint32_t n_mm, uint32_t maxs);

@@ -6,6 +6,7 @@
 // whole voxelization is enqueued without a host round trip per level (the
 // reference waits on a fence and reads the child count back after every
 // level: vulkan/src/engine/voxelizer.cpp:323-328).
+#include "frep_asm.hpp"
 #include "frep_device.hpp"
 #include "frep_internal.hpp"
 
@@ -333,8 +334,9 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
 // (vulkan/shaders/voxelizer/main.comp:74-83).  Each lane evaluates K voxels
 // of the same node (same tape) so instruction decode is shared K ways.
 // ---------------------------------------------------------------------------
-template <int MAXS, int K>
+template <int MAXS, int K, bool ASM = false>
 __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
+    static_assert(!ASM || (K == 8 && (MAXS == 8 || MAXS == 16)), "assembly path shape");
     if (load_counter(a.counters + CNT_OVERFLOW) != 0) return;
     const uint32_t lane = lane_id();
     const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
@@ -364,7 +366,20 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
             const uint64_t B = __ballot(d < 0.0f);
             store_mask64(rec + 4 + (sub * K + (uint32_t)k) * 2, B, lane);
         };
-        run_point_tape<MAXS, K>(as_const(a.pool + toff), consts, a.tape_time, init, out);
+        if constexpr (ASM) {
+            v8f xs, ys, zs;
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const float3 p = init(k);
+                xs[k] = p.x; ys[k] = p.y; zs[k] = p.z;
+            }
+            const v8f d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time,
+                                                    xs, ys, zs);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) out(k, d[k]);
+        } else {
+            run_point_tape<MAXS, K>(as_const(a.pool + toff), consts, a.tape_time, init, out);
+        }
     }
 }
 
@@ -616,6 +631,27 @@ __global__ __launch_bounds__(256) void k_eval_points(const uint32_t* tape, const
     run_point_tape<MAXS, 1>(as_const(tape), as_const(consts), t, init, put);
 }
 
+// Same through the assembly interpreter: 8 points per lane, point
+// blockIdx.x*2048 + k*256 + threadIdx.x for k = 0..7.
+template <int MAXS>
+__global__ __launch_bounds__(256) void k_eval_points_asm(const uint32_t* tape, const float* consts,
+                                                         const float* xyz, uint32_t n, float t,
+                                                         float* out) {
+    v8f xs, ys, zs;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t p = blockIdx.x * 2048u + (uint32_t)k * 256u + threadIdx.x;
+        const uint32_t q = p < n ? p : n - 1;
+        xs[k] = xyz[3 * q]; ys[k] = xyz[3 * q + 1]; zs[k] = xyz[3 * q + 2];
+    }
+    const v8f d = run_point_tape_asm8<MAXS>(tape, consts, t, xs, ys, zs);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t p = blockIdx.x * 2048u + (uint32_t)k * 256u + threadIdx.x;
+        if (p < n) out[p] = d[k];
+    }
+}
+
 template <int MAXS>
 __global__ __launch_bounds__(256) void k_eval_intervals(const uint32_t* tape_, const float* consts_,
                                                         const float* lo, const float* hi,
@@ -716,11 +752,11 @@ uint32_t interval_grid(uint32_t cu_count, uint32_t max_grid, size_t lds) {
     return g;
 }
 
-template <int M, int K>
+template <int M, int K, bool ASM>
 uint32_t point_grid(uint32_t cu_count, uint32_t max_grid) {
     thread_local uint32_t last_cu = 0, last_max = 0, g = 0;
     if (cu_count != last_cu || max_grid != last_max) {
-        g = resident_grid(k_level_point<M, K>, cu_count, max_grid, 0);
+        g = resident_grid(k_level_point<M, K, ASM>, cu_count, max_grid, 0);
         last_cu = cu_count;
         last_max = max_grid;
     }
@@ -755,10 +791,21 @@ uint32_t point_k_used(uint32_t maxs, uint32_t dim) {
 }
 
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                              uint32_t cu_count, uint32_t max_grid) {
+                              uint32_t cu_count, uint32_t max_grid, bool asm_ok) {
     const uint32_t k = point_k_used(maxs, a.dim);
+    // hand-scheduled interpreter: 8 points per lane, slot file of 8 or 16
+    if (asm_ok && k == 8 && maxs == 8) {
+        k_level_point<8, 8, true><<<point_grid<8, 8, true>(cu_count, max_grid), kLevelBlock, 0,
+                                    st>>>(a);
+        return hipGetLastError();
+    }
+    if (asm_ok && k == 8 && maxs == 16) {
+        k_level_point<16, 8, true><<<point_grid<16, 8, true>(cu_count, max_grid), kLevelBlock, 0,
+                                     st>>>(a);
+        return hipGetLastError();
+    }
 #define FREP_POINT_LAUNCH(KK)                                                                 \
-    k_level_point<M, KK><<<point_grid<M, KK>(cu_count, max_grid), kLevelBlock, 0, st>>>(a)
+    k_level_point<M, KK><<<point_grid<M, KK, false>(cu_count, max_grid), kLevelBlock, 0, st>>>(a)
 #define FREP_POINT_CALL                                                                       \
     {                                                                                         \
         constexpr int KM = point_k_for(M);                                                    \
@@ -780,7 +827,14 @@ hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs) {
 }
 
 hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
-                              const float* xyz, uint32_t n, float t, float* out, uint32_t maxs) {
+                              const float* xyz, uint32_t n, float t, float* out, uint32_t maxs,
+                              bool asm_ok) {
+    if (asm_ok && (maxs == 8 || maxs == 16)) {
+        const uint32_t blocks = (n + 2047) / 2048;
+        if (maxs == 8) k_eval_points_asm<8><<<blocks, 256, 0, st>>>(tape, consts, xyz, n, t, out);
+        else k_eval_points_asm<16><<<blocks, 256, 0, st>>>(tape, consts, xyz, n, t, out);
+        return hipGetLastError();
+    }
     const uint32_t blocks = (n + 255) / 256;
     FREP_DISPATCH_MAXS(maxs, (k_eval_points<M><<<blocks, 256, 0, st>>>(tape, consts, xyz, n, t, out)));
     return hipGetLastError();

@@ -217,3 +217,32 @@ def test_render_bit_exact(ctx, oracle, name, dims, cfg):
     assert ulp_diff(nrm[hit], onrm[hit]).max(initial=0) <= 2
     assert np.array_equal(rgba, orgba)
     ctx.tree_free(tree)
+
+
+def test_asm_and_cpp_interpreters_agree(oracle):
+    """The hand-scheduled point interpreter (8 points per lane) and the C++
+    interpreter give identical trees and point values (FREP_NO_ASM=1 forces
+    the C++ path in a second context)."""
+    import os
+    tape = shapes.random_csg(24, 2)
+    grid = shapes.make_grid([16, 8, 8])
+    os.environ["FREP_NO_ASM"] = "1"
+    try:
+        plain = lib.Context(0)
+    finally:
+        del os.environ["FREP_NO_ASM"]
+    fast = lib.Context(0)
+    try:
+        outs = []
+        for c in (fast, plain):
+            th = c.upload(tape)
+            outs.append(c.export(c.voxelize(th, grid)))
+        assert np.array_equal(outs[0], outs[1])
+        rng = np.random.default_rng(9)
+        p = rng.uniform(-1.1, 1.1, (5000, 3)).astype(np.float32)
+        a = fast.eval_points(fast.upload(tape), p)
+        b = plain.eval_points(plain.upload(tape), p)
+        assert same_bits(a, b)
+    finally:
+        fast.close()
+        plain.close()

@@ -191,8 +191,26 @@ frep_status ensure_pool(frep_ctx* ctx, TreeObj* t, uint64_t words) {
 
 uint32_t clamp_cap(uint64_t v, uint64_t bound) {
     v = std::min(v, bound);
-    v = std::min<uint64_t>(v, 0x7FFFFFFFull);
+    v = std::min<uint64_t>(v, kMaxLevelNodes);
     return (uint32_t)std::max<uint64_t>(v, 1);
+}
+
+uint64_t host_alloc(const TreeObj* t, uint32_t L) {
+    uint64_t v;
+    std::memcpy(&v, t->h_counters + CNT_ALLOC + 2 * L, 8);
+    return v;
+}
+
+// Node count of level L from the counter mirror (level 0 is the root).
+uint32_t host_nodes(const TreeObj* t, uint32_t L) {
+    return L == 0 ? 1u : alloc_nodes(host_alloc(t, L));
+}
+
+// First pool word past the tapes of every level.
+uint64_t host_pool_top(const TreeObj* t) {
+    uint64_t top = t->h_counters[CNT_BASE1];
+    for (uint32_t L = 1; L < t->grid.n_levels; ++L) top += alloc_words(host_alloc(t, L));
+    return top;
 }
 
 }  // namespace
@@ -450,13 +468,24 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         if (ovf == 0) break;
         if (attempt > nl + 4) return fail(ctx, FREP_OUT_OF_MEMORY, "tree did not fit after regrowth");
         ++t->retries;
+        bool grew = false;
         for (uint32_t L = 1; L < nl; ++L) {
-            const uint32_t req = t->h_counters[CNT_NODES + L];
-            if (req > t->node_cap[L]) want[L] = clamp_cap((uint64_t)req + req / 4 + 64, ~0ull);
+            const uint32_t req = host_nodes(t, L);
+            if (req > t->node_cap[L]) {
+                if (req >= kMaxLevelNodes)
+                    return fail(ctx, FREP_OUT_OF_MEMORY, "more than 2^26 nodes in one level");
+                want[L] = clamp_cap((uint64_t)req + req / 4 + 64, ~0ull);
+                grew = true;
+            }
         }
-        uint64_t top;
-        std::memcpy(&top, t->h_counters + CNT_POOL_TOP, 8);
-        if (top > t->pool_cap) pool_want = top + top / 4 + 4096;
+        const uint64_t top = host_pool_top(t);
+        if (top > t->pool_cap) {
+            pool_want = top + top / 4 + 4096;
+            grew = true;
+        }
+        // a flagged overflow with every request within capacity means a
+        // packed counter wrapped: the tree cannot be represented
+        if (!grew) return fail(ctx, FREP_OUT_OF_MEMORY, "node or tape counter overflow");
         if (pool_want > kMaxPoolWords)
             return fail(ctx, FREP_OUT_OF_MEMORY, "tape pool exceeds 2^32 words");
     }
@@ -478,15 +507,33 @@ frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t h, frep_tree_info* out
     if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
     *out = frep_tree_info{};
     out->n_levels = t->grid.n_levels;
+    HIP_TRY(hipSetDevice(ctx->device));
+    // tape-length stats (what the reference's stats buffer records,
+    // vulkan/shaders/voxelizer/main.comp:146-147): read back node headers
+    // and the tapes' size words
+    const uint64_t top = host_pool_top(t);
+    std::vector<uint32_t> pool(top);
+    if (top) HIP_TRY(hipMemcpy(pool.data(), t->d_pool, top * 4, hipMemcpyDeviceToHost));
     uint64_t bytes = 0;
     for (uint32_t L = 0; L < t->grid.n_levels; ++L) {
+        const uint32_t n = host_nodes(t, L);
         out->dims[L] = t->grid.dims[L];
-        out->node_count[L] = t->h_counters[CNT_NODES + L];
-        out->tape_len_sum[L] = t->h_counters[CNT_LEN_SUM + L];
-        out->tape_len_max[L] = t->h_counters[CNT_LEN_MAX + L];
+        out->node_count[L] = n;
+        std::vector<uint32_t> rec((size_t)n * t->ns[L]);
+        if (n) HIP_TRY(hipMemcpy(rec.data(), t->d_nodes[L], rec.size() * 4, hipMemcpyDeviceToHost));
+        uint64_t sum = 0;
+        uint32_t mx = 0;
+        for (uint32_t j = 0; j < n; ++j) {
+            const uint32_t off = rec[(size_t)j * t->ns[L]];
+            const uint32_t len = off < top ? pool[off] : 0;
+            sum += len;
+            mx = std::max(mx, len);
+        }
+        out->tape_len_sum[L] = (uint32_t)std::min<uint64_t>(sum, 0xFFFFFFFFull);
+        out->tape_len_max[L] = mx;
         bytes += (uint64_t)t->node_cap[L] * t->ns[L] * 4;
     }
-    std::memcpy(&out->tape_pool_words, t->h_counters + CNT_POOL_TOP, 8);
+    out->tape_pool_words = top;
     bytes += (t->pool_cap + kPoolPad) * 4 + (uint64_t)t->consts_cap * 4;
     out->device_bytes = bytes;
     out->slot_count = t->slot_count;
@@ -502,14 +549,13 @@ frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t h, uint32_t* buf, uint64
     if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t nl = t->grid.n_levels;
-    uint64_t top;
-    std::memcpy(&top, t->h_counters + CNT_POOL_TOP, 8);
+    const uint64_t top = host_pool_top(t);
     std::vector<uint32_t> pool(top);
     HIP_TRY(hipMemcpy(pool.data(), t->d_pool, top * 4, hipMemcpyDeviceToHost));
     std::vector<std::vector<uint32_t>> recs(nl);
     std::vector<uint32_t> cnt(nl);
     for (uint32_t L = 0; L < nl; ++L) {
-        cnt[L] = t->h_counters[CNT_NODES + L];
+        cnt[L] = host_nodes(t, L);
         recs[L].resize((size_t)cnt[L] * t->ns[L]);
         if (cnt[L])
             HIP_TRY(hipMemcpy(recs[L].data(), t->d_nodes[L], recs[L].size() * 4,

@@ -1,5 +1,8 @@
 // frep_asm.hpp -- hand-scheduled CDNA point interpreter (gfx950).
 //
+// M0 (the index register behind s_set_gpr_idx) is reserved by the compiler,
+// which sets it before each of its own uses, so it is not listed as clobbered.
+//
 // Evaluates one tape at 8 points per lane with the whole instruction loop in
 // a single inline-assembly block, for tapes whose ops are all in
 // {CONST, ADD, SUB, MUL, MIN, MAX, NEG, SQRT, COPY} (the signed-distance
@@ -239,7 +242,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
                        [p3] "=&s"(p3)
                      : [tape] "s"(tp), [cst] "s"(cp), [t] "v"(t), [maxs] "n"(MAXS)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_99, FREP_ASM_CLOBBER_100_163,
-                       "vcc", "m0", "memory");
+                       "vcc", "memory");
     } else {
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [n] "=&s"(n),
@@ -248,7 +251,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
                        [tiny] "=&s"(tiny), [p0] "=&s"(p0), [p1] "=&s"(p1), [p2] "=&s"(p2),
                        [p3] "=&s"(p3)
                      : [tape] "s"(tp), [cst] "s"(cp), [t] "v"(t), [maxs] "n"(MAXS)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_99, "vcc", "m0", "memory");
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_99, "vcc", "memory");
     }
     return x;
 }

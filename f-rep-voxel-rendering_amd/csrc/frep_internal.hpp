@@ -18,17 +18,23 @@ enum : uint32_t {
 };
 enum : uint32_t { MM_NONE = 0, MM_FIRST = 1, MM_SECOND = 2, MM_BOTH = 3 };
 
-// Device counter block of one tree (u32 words unless noted).
-//   [0..7]   node count per level (level 0 = 1)
-//   [8..9]   u64 tape-pool top (words)
-//   [10]     overflow flags: 1 = node capacity, 2 = tape pool
-//   [16..23] sum of child tape lengths per level (stats)
-//   [24..31] max child tape length per level (stats)
-enum : uint32_t {
-    CNT_NODES = 0, CNT_POOL_TOP = 8, CNT_OVERFLOW = 10, CNT_LEN_SUM = 16, CNT_LEN_MAX = 24,
-    CNT_WORDS = 32
-};
+// Device counter block of one tree (u32 words).
+//   [2L, 2L+1]  u64 allocation counter of level L (L = 1..7):
+//               (nodes << kAllocShift) | tape-pool words used by level L.
+//               One packed atomic per wavefront reserves both child ids and
+//               tape space.  Level 0 always has exactly one node.
+//   [16]        pool offset where level-1 tapes start
+//   [17]        overflow flags: 1 = node capacity, 2 = tape pool
+// Level L's tapes occupy [base_L, base_L + words_L) with base_1 = [16] and
+// base_{L+1} = base_L + words_L.
+enum : uint32_t { CNT_ALLOC = 0, CNT_BASE1 = 16, CNT_OVERFLOW = 17, CNT_WORDS = 32 };
 enum : uint32_t { OVF_NODES = 1u, OVF_POOL = 2u };
+constexpr uint32_t kAllocShift = 38;
+constexpr uint64_t kAllocWordMask = (1ull << kAllocShift) - 1;
+constexpr uint32_t kMaxLevelNodes = (1u << (64 - kAllocShift)) - 1;   // 2^26 - 1
+
+__host__ __device__ inline uint32_t alloc_nodes(uint64_t v) { return (uint32_t)(v >> kAllocShift); }
+__host__ __device__ inline uint64_t alloc_words(uint64_t v) { return v & kAllocWordMask; }
 
 // Node record (u32 words), per level, in the reference's order
 // (vulkan/shaders/node.comp:4-15, vulkan/include/engine/voxel_storage.h:63-88):

@@ -28,6 +28,17 @@ __device__ __forceinline__ uint32_t load_counter(const uint32_t* p) {
     return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+__device__ __forceinline__ uint64_t load_alloc(const uint32_t* counters, uint32_t level) {
+    return __hip_atomic_load((const unsigned long long*)(counters + CNT_ALLOC + 2 * level),
+                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Node count of a level (level 0 is the root alone), written by the
+// previous level's kernel.
+__device__ __forceinline__ uint32_t level_nodes(const uint32_t* counters, uint32_t level) {
+    return level == 0 ? 1u : uni(alloc_nodes(load_alloc(counters, level)));
+}
+
 // Write the 64-bit ballot as two mask words (lanes 0 and 1, one word each).
 __device__ __forceinline__ void store_mask64(uint32_t* dst, uint64_t m, uint32_t lane) {
     if (lane < 2) dst[lane] = lane ? (uint32_t)(m >> 32) : (uint32_t)m;
@@ -58,13 +69,7 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
     const uint32_t tid = threadIdx.x;
     const uint32_t n = root_tape[0];
     for (uint32_t i = tid; i <= n; i += blockDim.x) pool[root_off + i] = root_tape[i];
-    if (tid < CNT_WORDS) {
-        uint32_t v = 0;
-        if (tid == CNT_NODES) v = 1;
-        if (tid == CNT_POOL_TOP) v = root_off + ((n + 1 + 7u) & ~7u);
-        if (tid == CNT_LEN_SUM || tid == CNT_LEN_MAX) v = n;
-        counters[tid] = v;
-    }
+    if (tid < CNT_WORDS) counters[tid] = tid == CNT_BASE1 ? root_off + ((n + 1 + 7u) & ~7u) : 0u;
     if (tid == 0) {
         nodes0[0] = root_off;
         nodes0[1] = nodes0[2] = nodes0[3] = 0;
@@ -100,8 +105,12 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
     const uint32_t wib = uni(threadIdx.x >> 6);
     const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + wib;
     const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
-    uint32_t n_nodes = uni(load_counter(a.counters + CNT_NODES + a.level));
+    uint32_t n_nodes = level_nodes(a.counters, a.level);
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
+    // tapes of level+1 start after those of levels 1..level
+    uint64_t child_base = load_counter(a.counters + CNT_BASE1);
+    for (uint32_t j = 1; j <= a.level; ++j) child_base += alloc_words(load_alloc(a.counters, j));
+    child_base = ((uint64_t)uni((uint32_t)(child_base >> 32)) << 32) | uni((uint32_t)child_base);
     const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
     const uint32_t cpn = cells / 64;
     const uint32_t total = n_nodes * cpn;
@@ -223,21 +232,20 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
         store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
 
         uint32_t child_id = kNoChild;
-        uint32_t kept_len = 0;
         if (Bamb != 0) {
             const uint32_t namb = (uint32_t)__popcll(Bamb);
             const uint32_t rank = lane_rank(Bamb);
             const uint32_t stride = n + 1;
-            uint32_t cbase = 0;
-            unsigned long long pbase = 0;
-            if (lane == 0) {
-                cbase = atomicAdd(a.counters + CNT_NODES + a.level + 1, namb);
-                pbase = atomicAdd((unsigned long long*)(a.counters + CNT_POOL_TOP),
-                                  (unsigned long long)namb * stride);
-            }
-            cbase = __shfl(cbase, 0);
-            pbase = ((unsigned long long)(uint32_t)__shfl((uint32_t)(pbase >> 32), 0) << 32) |
-                    (uint32_t)__shfl((uint32_t)pbase, 0);
+            // one packed atomic reserves namb child ids and namb*stride words
+            unsigned long long old = 0;
+            if (lane == 0)
+                old = atomicAdd((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
+                                ((unsigned long long)namb << kAllocShift) |
+                                    ((unsigned long long)namb * stride));
+            old = ((unsigned long long)uni((uint32_t)__shfl((int)(uint32_t)(old >> 32), 0)) << 32) |
+                  uni((uint32_t)__shfl((int)(uint32_t)old, 0));
+            const uint32_t cbase = alloc_nodes(old);
+            const uint64_t pbase = child_base + alloc_words(old);
             const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
             const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
             if (!(nodes_ok && pool_ok)) {
@@ -307,22 +315,6 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
                 crec[1] = gx;
                 crec[2] = gy;
                 crec[3] = gz;
-                kept_len = kept;
-            }
-            // tape-length stats (voxelizer/main.comp:146-147): reduced over the
-            // wavefront first, then one atomic each instead of one per lane
-            if (nodes_ok && pool_ok) {
-                uint32_t sum = kept_len, mx = kept_len;
-#pragma unroll
-                for (int d = 32; d >= 1; d >>= 1) {
-                    sum += (uint32_t)__shfl_xor((int)sum, d);
-                    const uint32_t o2 = (uint32_t)__shfl_xor((int)mx, d);
-                    mx = o2 > mx ? o2 : mx;
-                }
-                if (lane == 0) {
-                    atomicAdd(a.counters + CNT_LEN_SUM + a.level + 1, sum);
-                    atomicMax(a.counters + CNT_LEN_MAX + a.level + 1, mx);
-                }
             }
         }
         rec[4 + 2 * W + idx] = child_id;
@@ -341,7 +333,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
     const uint32_t lane = lane_id();
     const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
     const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
-    uint32_t n_nodes = uni(load_counter(a.counters + CNT_NODES + a.level));
+    uint32_t n_nodes = level_nodes(a.counters, a.level);
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
     const uint32_t dim = a.dim, cells = dim * dim * dim;
     const uint32_t cpw = cells / (64 * K);

@@ -115,6 +115,29 @@ __device__ __forceinline__ void fetch4(ctape_t words, uint32_t i, uint32_t (&w)[
     w[0] = words[i]; w[1] = words[i + 1]; w[2] = words[i + 2]; w[3] = words[i + 3];
 }
 
+// Lane l of a VGPR as a wave-uniform value (v_readlane_b32; ignores EXEC).
+__device__ __forceinline__ uint32_t rdlane(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ float rdlanef(float v, uint32_t l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)l));
+}
+
+// Tape window: 64 consecutive instruction words staged across the lanes of
+// one VGPR (lane l holds word s + l), so an instruction is fetched with one
+// v_readlane instead of a scalar load whose latency the interpreter would
+// wait on.  Words outside [0, n) read as 0xFFFFFFFF (op 255: no constant).
+// Must be called with every lane of the wavefront active.
+__device__ __forceinline__ uint32_t window_load(const uint32_t* words, int32_t s, uint32_t n,
+                                                uint32_t lane) {
+    const int32_t i = s + (int32_t)lane;
+    return (i >= 0 && i < (int32_t)n) ? words[i] : 0xFFFFFFFFu;
+}
+// Constant operand of each CONST word of a window (gathered once per window).
+__device__ __forceinline__ float window_consts(uint32_t w, const float* consts) {
+    return (w & 0xFFu) == 0u ? consts[w >> 16] : 0.0f;
+}
+
 #define FREP_BANKS8(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
 
 // Scheduling fence: keeps each operand's register reads together so the

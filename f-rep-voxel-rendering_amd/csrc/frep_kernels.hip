@@ -131,8 +131,8 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
         const float lx = a.low[0] + (float)gx * a.cs;
         const float ly = a.low[1] + (float)gy * a.cs;
         const float lz = a.low[2] + (float)gz * a.cs;
-        ctape_t tape = as_const(a.pool + toff);
-        const uint32_t n = tape[0];
+        const uint32_t n = as_const(a.pool + toff)[0];
+        const uint32_t* tw = a.pool + toff + 1;             // instruction words
 
         // ---- forward pass: interval evaluation (interval.comp:121-160) ----
         float slo[MAXS], shi[MAXS];
@@ -141,15 +141,15 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
         slo[2] = lz; shi[2] = lz + a.cs;
         slo[3] = a.tape_time; shi[3] = a.tape_time;
         uint32_t mm = 0;
-        uint32_t nq[4];
-        fetch4(tape, 1, nq);
-        for (uint32_t i0 = 0; i0 < n; i0 += 4) {
-            uint32_t wq[4] = {nq[0], nq[1], nq[2], nq[3]};
-            if (i0 + 4 < n) fetch4(tape, 5 + i0, nq);          // prefetch next group
-            const uint32_t m = n - i0 < 4 ? n - i0 : 4;
+        // three windows in flight: the current one (with its constants), the
+        // next (constants requested), the one after (words requested)
+        uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
+        uint32_t fw2 = window_load(tw, 128, n, lane);
+        float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
+        for (uint32_t s = 0; s < n; s += 64) {
+            const uint32_t m = n - s < 64 ? n - s : 64;
             for (uint32_t j = 0; j < m; ++j) {
-                const uint32_t w = wq[0];
-                wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
+                const uint32_t w = rdlane(fw0, j);
                 // operand fields are slot indices except for CONST (pool
                 // index); masking keeps every register-file access in range
                 const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
@@ -212,15 +212,19 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
                 } else if (op & 1u) {
                     r = interval_op(OP_SIN, A, A, 0.0f);
                 } else {
-                    const float c = consts[w >> 16];
+                    const float c = rdlanef(fc0, j);
                     r = Itv{c, c};
                 }
                 FREP_FENCE();
                 slo[o] = r.lo;
                 shi[o] = r.hi;
             }
+            fw0 = fw1; fc0 = fc1;
+            fw1 = fw2; fc1 = window_consts(fw2, a.consts);
+            fw2 = window_load(tw, (int32_t)s + 192, n, lane);
         }
         (void)kPi;
+        (void)consts;
         // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
         const float ilo = slo[0], ihi = shi[0];
         const bool full = ihi < 0.0f;
@@ -252,69 +256,78 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
                 if (lane == 0)
                     atomicOr(a.counters + CNT_OVERFLOW,
                              (nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL));
-            } else if (amb) {
+            } else {
                 // ---- backward pass: liveness + compaction (shorten.comp:164-223) ----
+                // Runs with the whole wavefront active (the tape windows are
+                // loaded by every lane); only ambiguous lanes keep anything.
                 const uint64_t mybase = pbase + (uint64_t)rank * stride;
                 uint32_t* out = a.pool + mybase;
                 uint32_t act[AW];
 #pragma unroll
                 for (int k = 0; k < AW; ++k) act[k] = 0;
-                act[0] = 1u;                                   // slot 0 is the result
+                act[0] = amb ? 1u : 0u;                        // slot 0 is the result
                 uint32_t kept = 0;
                 int32_t jm = (int32_t)mm - 1;
-                // words i-3 .. i arrive together; the group below the first
-                // instruction reads into the slack before every pool tape
-                uint32_t bq[4];
-                for (int32_t i = (int32_t)n - 1; i >= 0; --i) {
-                    // signed offset: for i < 3 the group starts up to two
-                    // words before the tape (inside the pool's slack)
-                    if ((((int32_t)n - 1 - i) & 3) == 0) {
-                        ctape_t g = tape + (ptrdiff_t)(i - 2);
-                        bq[0] = g[0]; bq[1] = g[1]; bq[2] = g[2]; bq[3] = g[3];
+                uint4 row = scr[jm > 0 ? jm : 0];              // next choice row, read ahead
+                int32_t e = (int32_t)n;                        // window = words [e-64, e)
+                uint32_t bw0 = window_load(tw, e - 64, n, lane);
+                uint32_t bw1 = window_load(tw, e - 128, n, lane);
+                uint32_t bw2 = window_load(tw, e - 192, n, lane);
+                while (e > 0) {
+                    const int32_t s = e - 64;
+                    const int32_t lo = s > 0 ? s : 0;
+                    for (int32_t i = e - 1; i >= lo; --i) {
+                        const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
+                        const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+                        const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1);
+                        const uint32_t ib = (w >> 24) & (MAXS - 1);
+                        const bool is_mm = (op == OP_MIN || op == OP_MAX);
+                        uint32_t ch = 0;
+                        if (is_mm) {
+                            const uint4 v = row;
+                            --jm;
+                            row = scr[jm > 0 ? jm : 0];
+                            const uint64_t F = ((uint64_t)v.y << 32) | v.x;
+                            const uint64_t S = ((uint64_t)v.w << 32) | v.z;
+                            ch = (uint32_t)((F >> lane) & 1u) |
+                                 ((uint32_t)((S >> lane) & 1u) << 1);
+                        }
+                        if ((act[o >> 5] >> (o & 31u)) & 1u) {
+                            act[o >> 5] &= ~(1u << (o & 31u));  // output first (shorten.comp:179)
+                            uint32_t sel;
+                            if (op == OP_CONST) sel = MM_NONE;
+                            else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
+                                sel = MM_BOTH;
+                            else if (is_mm) sel = ch;
+                            else sel = MM_FIRST;
+                            if (sel & MM_FIRST) act[ia >> 5] |= 1u << (ia & 31u);
+                            if (sel & MM_SECOND) act[ib >> 5] |= 1u << (ib & 31u);
+                            uint32_t ow = w;
+                            if (is_mm && ch == MM_FIRST) ow = (w & ~0xFFu) | OP_COPY;
+                            else if (is_mm && ch == MM_SECOND)
+                                ow = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
+                            out[n - kept] = ow;
+                            ++kept;
+                        }
                     }
-                    const uint32_t w = bq[3];
-                    bq[3] = bq[2]; bq[2] = bq[1]; bq[1] = bq[0];
-                    const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-                    const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1);
-                    const uint32_t ib = (w >> 24) & (MAXS - 1);
-                    const bool is_mm = (op == OP_MIN || op == OP_MAX);
-                    uint32_t ch = 0;
-                    if (is_mm) {
-                        const uint4 v = scr[jm];
-                        --jm;
-                        const uint64_t F = ((uint64_t)v.y << 32) | v.x;
-                        const uint64_t S = ((uint64_t)v.w << 32) | v.z;
-                        ch = (uint32_t)((F >> lane) & 1u) | ((uint32_t)((S >> lane) & 1u) << 1);
-                    }
-                    if ((act[o >> 5] >> (o & 31u)) & 1u) {
-                        act[o >> 5] &= ~(1u << (o & 31u));     // output first (shorten.comp:179)
-                        uint32_t sel;
-                        if (op == OP_CONST) sel = MM_NONE;
-                        else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
-                            sel = MM_BOTH;
-                        else if (is_mm) sel = ch;
-                        else sel = MM_FIRST;
-                        if (sel & MM_FIRST) act[ia >> 5] |= 1u << (ia & 31u);
-                        if (sel & MM_SECOND) act[ib >> 5] |= 1u << (ib & 31u);
-                        uint32_t ow = w;
-                        if (is_mm && ch == MM_FIRST) ow = (w & ~0xFFu) | OP_COPY;
-                        else if (is_mm && ch == MM_SECOND)
-                            ow = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
-                        out[n - kept] = ow;
-                        ++kept;
-                    }
+                    bw0 = bw1;
+                    bw1 = bw2;
+                    bw2 = window_load(tw, e - 256, n, lane);
+                    e -= 64;
                 }
-                uint32_t final_off = toff;
-                if (kept < n) {
-                    out[n - kept] = kept;
-                    final_off = (uint32_t)(mybase + n - kept);
+                if (amb) {
+                    uint32_t final_off = toff;
+                    if (kept < n) {
+                        out[n - kept] = kept;
+                        final_off = (uint32_t)(mybase + n - kept);
+                    }
+                    child_id = cbase + rank;
+                    uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
+                    crec[0] = final_off;
+                    crec[1] = gx;
+                    crec[2] = gy;
+                    crec[3] = gz;
                 }
-                child_id = cbase + rank;
-                uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
-                crec[0] = final_off;
-                crec[1] = gx;
-                crec[2] = gy;
-                crec[3] = gz;
             }
         }
         rec[4 + 2 * W + idx] = child_id;

@@ -52,6 +52,49 @@ def cpu_baseline(tape, grid, budget_s=12.0):
                       f"dense Tape::Eval semantics with per-call slot allocation"}
 
 
+def bench_batched(ctx, shapes, reps=3):
+    """The 64-shape mix (sphere unions of 4..400 primitives at 128^3..1024^3):
+    requests/s of one frep_voxelize per shape (plain) against one
+    frep_voxelize_batch over all 64 (batched), same trees, same results."""
+    mix = shapes.batch_mix(64)
+    ths = [ctx.upload(t) for t, _, _ in mix]
+    grids = [g for _, g, _ in mix]
+    trees = [ctx.tree_create() for _ in mix]
+    ctx.voxelize_batch(ths, grids, trees)            # grows every tree's buffers
+    for th, g, tr in zip(ths, grids, trees):
+        ctx.voxelize(th, g, tr)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        for th, g, tr in zip(ths, grids, trees):
+            ctx.voxelize(th, g, tr)
+    plain = (time.perf_counter() - t0) / reps
+    ctx.set_timing(True)
+    bms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.voxelize_batch(ths, grids, trees)
+        bms += ctx.timing().batch_ms
+    batched = (time.perf_counter() - t0) / reps
+    ctx.set_timing(False)
+    voxels = float(sum(r ** 3 for _, _, r in mix))
+    out = {
+        "workload": "64 sphere unions, 4..400 primitives (log-uniform, seeds 1000+k), "
+                    "resolutions {128,256,512,1024}^3",
+        "tape_ops_total": sum(t.n for t, _, _ in mix),
+        "plain_req_per_s": round(len(mix) / plain, 2),
+        "batched_req_per_s": round(len(mix) / batched, 2),
+        "speedup": round(plain / batched, 3),
+        "batched_ms": round(batched * 1e3, 3),
+        "batched_gpu_ms": round(bms / reps, 3),
+        "batched_gvoxel_evals_per_s": round(voxels / batched / 1e9, 2),
+    }
+    for tr in trees:
+        ctx.tree_free(tr)
+    for th in ths:
+        ctx.tape_free(th)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -60,6 +103,7 @@ def main():
     ap.add_argument("--workload", default="csg24_1024")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -149,6 +193,8 @@ def main():
         ctx.set_timing(False)
         out["render"] = {"frame": "1920x1080 perspective, 1 spp", "kernel_ms": round(rms / reps, 4),
                          "frames_per_s_incl_copy": round(1.0 / rwall, 2)}
+    if not args.no_batched:
+        out["batched"] = bench_batched(ctx, shapes)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tape, grid)
     ctx.close()

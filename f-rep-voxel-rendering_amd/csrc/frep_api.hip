@@ -84,6 +84,14 @@ struct frep_ctx {
     // render output staging
     void* d_out = nullptr;
     size_t out_bytes = 0;
+    // batched voxelization: argument tables (pinned + device) and the
+    // gathered counter blocks
+    void* h_stage = nullptr;
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    uint32_t* h_cnt = nullptr;
+    uint32_t* d_cnt = nullptr;
+    size_t cnt_words = 0;
 };
 
 namespace {
@@ -256,6 +264,10 @@ void frep_shutdown(frep_ctx* ctx) {
         if (t->h_counters) (void)hipHostFree(t->h_counters);
     }
     if (ctx->d_out) (void)hipFree(ctx->d_out);
+    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
+    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
+    if (ctx->h_cnt) (void)hipHostFree(ctx->h_cnt);
+    if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
     for (auto& e : ctx->ev)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -356,16 +368,23 @@ frep_status frep_tree_free(frep_ctx* ctx, frep_tree_t h) {
     return FREP_OK;
 }
 
-frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, frep_tree_t h) {
-    if (!ctx) return FREP_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    TapeObj* tp = get_tape(ctx, th);
-    TreeObj* t = get_tree(ctx, h);
-    if (!tp) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
-    if (!t) return fail(ctx, FREP_INVALID_ARG, "unknown tree handle");
-    const frep_status gs = check_grid(ctx, grid);
-    if (gs != FREP_OK) return gs;
-    HIP_TRY(hipSetDevice(ctx->device));
+}  // extern "C"
+
+namespace {
+
+// One voxelization of a batch (or the only one of a plain call): capacities
+// requested for the next attempt.
+struct Job {
+    TapeObj* tp = nullptr;
+    TreeObj* t = nullptr;
+    uint32_t want[kMaxLevels] = {};
+    uint64_t pool_want = 0;
+    uint32_t mm_stride = 1;
+};
+
+// Sets up tree t for tape tp over grid: record sizes, cell sizes, first
+// capacity guesses, the tree's copy of the constant pool (enqueued).
+frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid* grid, Job* j) {
     const uint32_t nl = grid->n_levels;
     const bool same_shape = t->built && t->grid.n_levels == nl &&
                             std::equal(grid->dims, grid->dims + nl, t->grid.dims);
@@ -391,15 +410,14 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
     }
     // capacities: upper bound n_{L+1} <= n_L * dim_L^3, first guess capped
     uint64_t bound = 1;
-    uint32_t want[kMaxLevels];
     for (uint32_t L = 0; L < nl; ++L) {
         const uint64_t guess = L == 0 ? 1 : std::max<uint64_t>(t->node_cap[L], 1u << 16);
-        want[L] = clamp_cap(guess, bound);
+        j->want[L] = clamp_cap(guess, bound);
         const uint64_t cells = (uint64_t)grid->dims[L] * grid->dims[L] * grid->dims[L];
         bound = std::min<uint64_t>(bound * cells, 0x7FFFFFFFull);
     }
-    uint64_t pool_want = std::max<uint64_t>(t->pool_cap, (uint64_t)kRootOff + (tp->words[0] + 1) * 64ull + (1u << 20));
-    // constants
+    j->pool_want = std::max<uint64_t>(
+        t->pool_cap, (uint64_t)kRootOff + (tp->words[0] + 1) * 64ull + (1u << 20));
     const uint32_t nc = (uint32_t)tp->consts.size();
     if (!t->d_consts || t->consts_cap < nc + kPoolPad) {
         if (t->d_consts) HIP_TRY(hipFree(t->d_consts));
@@ -411,18 +429,238 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         HIP_TRY(hipMemcpyAsync(t->d_consts, tp->d_consts, (size_t)nc * 4, hipMemcpyDeviceToDevice,
                                ctx->stream));
     // min/max choice rows live in the interval kernel's LDS
-    const uint32_t mm_stride = std::max<uint32_t>(tp->n_mm, 1);
-    if (nl > 1 && mm_stride > kMaxMinMax)
+    j->mm_stride = std::max<uint32_t>(tp->n_mm, 1);
+    if (nl > 1 && j->mm_stride > kMaxMinMax)
         return fail(ctx, FREP_UNSUPPORTED,
                     "tape has more than " + std::to_string(kMaxMinMax) + " MIN/MAX instructions");
     t->retries = 0;
-    for (uint32_t attempt = 0;; ++attempt) {
-        for (uint32_t L = 0; L < nl; ++L) {
-            const frep_status s = ensure_level(ctx, t, L, want[L]);
-            if (s != FREP_OK) return s;
+    j->tp = tp;
+    j->t = t;
+    return FREP_OK;
+}
+
+frep_status ensure_job(frep_ctx* ctx, Job* j) {
+    for (uint32_t L = 0; L < j->t->grid.n_levels; ++L) {
+        const frep_status s = ensure_level(ctx, j->t, L, j->want[L]);
+        if (s != FREP_OK) return s;
+    }
+    return ensure_pool(ctx, j->t, j->pool_want);
+}
+
+// Arguments of level L of tree t (pk / use_asm are set by the caller).
+LevelArgs level_args(const TreeObj* t, uint32_t L, uint32_t mm_stride) {
+    const frep_grid& g = t->grid;
+    const bool last = (L + 1 == g.n_levels);
+    LevelArgs a{};
+    a.nodes = t->d_nodes[L];
+    a.child_nodes = last ? nullptr : t->d_nodes[L + 1];
+    a.pool = t->d_pool;
+    a.consts = t->d_consts;
+    a.counters = t->d_counters;
+    a.pool_cap = t->pool_cap;
+    a.level = L;
+    a.dim = g.dims[L];
+    a.dim_shift = 0;
+    if ((a.dim & (a.dim - 1)) == 0)
+        while ((1u << a.dim_shift) < a.dim) ++a.dim_shift;
+    a.node_cap = t->node_cap[L];
+    a.child_cap = last ? 0 : t->node_cap[L + 1];
+    a.ns = t->ns[L];
+    a.child_ns = last ? 0 : t->ns[L + 1];
+    a.mm_stride = mm_stride;
+    a.cs = t->cs[L];
+    a.low[0] = g.low[0];
+    a.low[1] = g.low[1];
+    a.low[2] = g.low[2];
+    a.tape_time = g.tape_time;
+    a.pk = 1;
+    a.use_asm = 0;
+    return a;
+}
+
+// After a flagged overflow (counters in t->h_counters): grow what the
+// counters asked for.  Fails when nothing can grow.
+frep_status regrow_job(frep_ctx* ctx, Job* j, uint32_t attempt) {
+    TreeObj* t = j->t;
+    const uint32_t nl = t->grid.n_levels;
+    if (attempt > nl + 4) return fail(ctx, FREP_OUT_OF_MEMORY, "tree did not fit after regrowth");
+    ++t->retries;
+    bool grew = false;
+    for (uint32_t L = 1; L < nl; ++L) {
+        const uint32_t req = host_nodes(t, L);
+        if (req > t->node_cap[L]) {
+            if (req >= kMaxLevelNodes)
+                return fail(ctx, FREP_OUT_OF_MEMORY, "more than 2^26 nodes in one level");
+            j->want[L] = clamp_cap((uint64_t)req + req / 4 + 64, ~0ull);
+            grew = true;
         }
+    }
+    const uint64_t top = host_pool_top(t);
+    if (top > t->pool_cap) {
+        j->pool_want = top + top / 4 + 4096;
+        grew = true;
+    }
+    // a flagged overflow with every request within capacity means a packed
+    // counter wrapped: the tree cannot be represented
+    if (!grew) return fail(ctx, FREP_OUT_OF_MEMORY, "node or tape counter overflow");
+    if (j->pool_want > kMaxPoolWords)
+        return fail(ctx, FREP_OUT_OF_MEMORY, "tape pool exceeds 2^32 words");
+    return FREP_OK;
+}
+
+size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
+
+// Enqueues the batched voxelization of jobs[idx[0..m)] in groups of
+// kBatchMax trees (one init launch, one or two level launches per level and
+// one counter gather per group), reads every counter block back with one
+// copy and waits.
+frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<uint32_t>& idx) {
+    const uint32_t m = (uint32_t)idx.size();
+    if (m == 0) return FREP_OK;
+    const uint32_t groups = (m + kBatchMax - 1) / kBatchMax;
+    // staging layout per group: InitArgs[cnt] | for each level: interval
+    // LevelArgs, point LevelArgs
+    struct Launch {
+        uint32_t level;
+        bool point;
+        uint32_t count, maxs, mm;
+        size_t off;
+    };
+    struct Group {
+        uint32_t first, count;
+        size_t init_off;
+        std::vector<Launch> launches;
+    };
+    std::vector<Group> gs(groups);
+    size_t bytes = 0;
+    for (uint32_t g = 0; g < groups; ++g) {
+        Group& G = gs[g];
+        G.first = g * kBatchMax;
+        G.count = std::min<uint32_t>(kBatchMax, m - G.first);
+        G.init_off = bytes;
+        bytes = align16(bytes + G.count * sizeof(InitArgs));
+        uint32_t max_nl = 0;
+        for (uint32_t k = 0; k < G.count; ++k)
+            max_nl = std::max(max_nl, jobs[idx[G.first + k]].t->grid.n_levels);
+        for (uint32_t L = 0; L < max_nl; ++L) {
+            for (int pt = 0; pt < 2; ++pt) {
+                Launch la{L, pt == 1, 0, 8, 1, bytes};
+                for (uint32_t k = 0; k < G.count; ++k) {
+                    const Job& j = jobs[idx[G.first + k]];
+                    const uint32_t nl = j.t->grid.n_levels;
+                    if (L >= nl || (L + 1 == nl) != (pt == 1)) continue;
+                    ++la.count;
+                    la.maxs = std::max(la.maxs, j.t->maxs);
+                    la.mm = std::max(la.mm, j.mm_stride);
+                }
+                if (la.count) {
+                    bytes = align16(bytes + la.count * sizeof(LevelArgs));
+                    G.launches.push_back(la);
+                }
+            }
+        }
+    }
+    if (ctx->stage_bytes < bytes) {
+        if (ctx->h_stage) HIP_TRY(hipHostFree(ctx->h_stage));
+        if (ctx->d_stage) HIP_TRY(hipFree(ctx->d_stage));
+        ctx->h_stage = nullptr;
+        ctx->d_stage = nullptr;
+        ctx->stage_bytes = 0;
+        const size_t cap = bytes + bytes / 2 + 4096;
+        HIP_TRY(hipHostMalloc(&ctx->h_stage, cap, hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&ctx->d_stage, cap));
+        ctx->stage_bytes = cap;
+    }
+    if (ctx->cnt_words < (size_t)m * CNT_WORDS) {
+        if (ctx->h_cnt) HIP_TRY(hipHostFree(ctx->h_cnt));
+        if (ctx->d_cnt) HIP_TRY(hipFree(ctx->d_cnt));
+        ctx->h_cnt = nullptr;
+        ctx->d_cnt = nullptr;
+        ctx->cnt_words = 0;
+        const size_t cap = (size_t)std::max<uint32_t>(m, kBatchMax) * CNT_WORDS;
+        HIP_TRY(hipHostMalloc(&ctx->h_cnt, cap * 4, hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&ctx->d_cnt, cap * 4));
+        ctx->cnt_words = cap;
+    }
+    uint8_t* hs = static_cast<uint8_t*>(ctx->h_stage);
+    for (const Group& G : gs) {
+        InitArgs* ia = reinterpret_cast<InitArgs*>(hs + G.init_off);
+        for (uint32_t k = 0; k < G.count; ++k) {
+            const Job& j = jobs[idx[G.first + k]];
+            ia[k] = InitArgs{j.t->d_counters, j.t->d_nodes[0], j.t->d_pool, j.tp->d_words, kRootOff, 0};
+        }
+        for (const Launch& la : G.launches) {
+            LevelArgs* la_args = reinterpret_cast<LevelArgs*>(hs + la.off);
+            uint32_t c = 0;
+            for (uint32_t k = 0; k < G.count; ++k) {
+                const Job& j = jobs[idx[G.first + k]];
+                const uint32_t nl = j.t->grid.n_levels;
+                if (la.level >= nl || (la.level + 1 == nl) != la.point) continue;
+                LevelArgs a = level_args(j.t, la.level, la.mm);
+                if (la.point) {
+                    a.pk = point_k_used(la.maxs, a.dim);
+                    a.use_asm = (j.t->asm_ok && a.pk == 8 && la.maxs <= 16) ? 1u : 0u;
+                }
+                la_args[c++] = a;
+            }
+        }
+    }
+    uint8_t* ds = static_cast<uint8_t*>(ctx->d_stage);
+    if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+    HIP_TRY(hipMemcpyAsync(ds, hs, bytes, hipMemcpyHostToDevice, ctx->stream));
+    for (uint32_t g = 0; g < groups; ++g) {
+        const Group& G = gs[g];
+        const InitArgs* d_init = reinterpret_cast<const InitArgs*>(ds + G.init_off);
+        HIP_TRY(launch_init_batch(ctx->stream, d_init, G.count));
+        for (const Launch& la : G.launches) {
+            BatchArgs b{reinterpret_cast<const LevelArgs*>(ds + la.off), la.count, la.mm};
+            if (la.point)
+                HIP_TRY(launch_level_point_batch(ctx->stream, b, la.maxs, ctx->cu_count,
+                                                 ctx->level_grid));
+            else
+                HIP_TRY(launch_level_interval_batch(ctx->stream, b, la.maxs, ctx->cu_count,
+                                                    ctx->level_grid));
+        }
+        HIP_TRY(launch_gather_counters(ctx->stream, d_init, G.count,
+                                       ctx->d_cnt + (size_t)G.first * CNT_WORDS));
+    }
+    HIP_TRY(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, (size_t)m * CNT_WORDS * 4,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    for (uint32_t k = 0; k < m; ++k)
+        std::memcpy(jobs[idx[k]].t->h_counters, ctx->h_cnt + (size_t)k * CNT_WORDS, CNT_WORDS * 4);
+    if (ctx->timing) {
+        float ms = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        ctx->last.batch_ms += ms;
+    }
+    return FREP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, frep_tree_t h) {
+    if (!ctx) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TapeObj* tp = get_tape(ctx, th);
+    TreeObj* t = get_tree(ctx, h);
+    if (!tp) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
+    if (!t) return fail(ctx, FREP_INVALID_ARG, "unknown tree handle");
+    const frep_status gs = check_grid(ctx, grid);
+    if (gs != FREP_OK) return gs;
+    HIP_TRY(hipSetDevice(ctx->device));
+    const uint32_t nl = grid->n_levels;
+    Job j;
+    {
+        const frep_status s = prepare_job(ctx, tp, t, grid, &j);
+        if (s != FREP_OK) return s;
+    }
+    for (uint32_t attempt = 0;; ++attempt) {
         {
-            const frep_status s = ensure_pool(ctx, t, pool_want);
+            const frep_status s = ensure_job(ctx, &j);
             if (s != FREP_OK) return s;
         }
         if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
@@ -430,30 +668,8 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
                                  kRootOff, nl));
         if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
         for (uint32_t L = 0; L < nl; ++L) {
-            const bool last = (L + 1 == nl);
-            LevelArgs a{};
-            a.nodes = t->d_nodes[L];
-            a.child_nodes = last ? nullptr : t->d_nodes[L + 1];
-            a.pool = t->d_pool;
-            a.consts = t->d_consts;
-            a.counters = t->d_counters;
-            a.pool_cap = t->pool_cap;
-            a.level = L;
-            a.dim = grid->dims[L];
-            a.dim_shift = 0;
-            if ((a.dim & (a.dim - 1)) == 0)
-                while ((1u << a.dim_shift) < a.dim) ++a.dim_shift;
-            a.node_cap = t->node_cap[L];
-            a.child_cap = last ? 0 : t->node_cap[L + 1];
-            a.ns = t->ns[L];
-            a.child_ns = last ? 0 : t->ns[L + 1];
-            a.mm_stride = mm_stride;
-            a.cs = t->cs[L];
-            a.low[0] = grid->low[0];
-            a.low[1] = grid->low[1];
-            a.low[2] = grid->low[2];
-            a.tape_time = grid->tape_time;
-            if (last)
+            const LevelArgs a = level_args(t, L, j.mm_stride);
+            if (L + 1 == nl)
                 HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, ctx->cu_count, ctx->level_grid,
                                            t->asm_ok));
             else
@@ -464,30 +680,9 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4, hipMemcpyDeviceToHost,
                                ctx->stream));
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        const uint32_t ovf = t->h_counters[CNT_OVERFLOW];
-        if (ovf == 0) break;
-        if (attempt > nl + 4) return fail(ctx, FREP_OUT_OF_MEMORY, "tree did not fit after regrowth");
-        ++t->retries;
-        bool grew = false;
-        for (uint32_t L = 1; L < nl; ++L) {
-            const uint32_t req = host_nodes(t, L);
-            if (req > t->node_cap[L]) {
-                if (req >= kMaxLevelNodes)
-                    return fail(ctx, FREP_OUT_OF_MEMORY, "more than 2^26 nodes in one level");
-                want[L] = clamp_cap((uint64_t)req + req / 4 + 64, ~0ull);
-                grew = true;
-            }
-        }
-        const uint64_t top = host_pool_top(t);
-        if (top > t->pool_cap) {
-            pool_want = top + top / 4 + 4096;
-            grew = true;
-        }
-        // a flagged overflow with every request within capacity means a
-        // packed counter wrapped: the tree cannot be represented
-        if (!grew) return fail(ctx, FREP_OUT_OF_MEMORY, "node or tape counter overflow");
-        if (pool_want > kMaxPoolWords)
-            return fail(ctx, FREP_OUT_OF_MEMORY, "tape pool exceeds 2^32 words");
+        if (t->h_counters[CNT_OVERFLOW] == 0) break;
+        const frep_status s = regrow_job(ctx, &j, attempt);
+        if (s != FREP_OK) return s;
     }
     if (ctx->timing) {
         ctx->last = frep_timing{};
@@ -497,6 +692,58 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             HIP_TRY(hipEventElapsedTime(&ctx->last.level_ms[L], ctx->ev[1 + L], ctx->ev[2 + L]));
     }
     t->built = true;
+    return FREP_OK;
+}
+
+frep_status frep_voxelize_batch(frep_ctx* ctx, uint32_t n, const frep_tape_t* tapes,
+                                const frep_grid* grids, const frep_tree_t* trees) {
+    if (!ctx) return FREP_INVALID_ARG;
+    if (n && (!tapes || !grids || !trees)) return fail(ctx, FREP_INVALID_ARG, "null argument");
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    std::vector<Job> jobs(n);
+    std::vector<frep_tree_t> seen(trees, trees + n);
+    std::sort(seen.begin(), seen.end());
+    if (std::adjacent_find(seen.begin(), seen.end()) != seen.end())
+        return fail(ctx, FREP_INVALID_ARG, "a tree appears twice in one batch");
+    for (uint32_t k = 0; k < n; ++k) {
+        if (!get_tape(ctx, tapes[k])) return fail(ctx, FREP_INVALID_ARG, "unknown tape handle");
+        if (!get_tree(ctx, trees[k])) return fail(ctx, FREP_INVALID_ARG, "unknown tree handle");
+        const frep_status gs = check_grid(ctx, &grids[k]);
+        if (gs != FREP_OK) return gs;
+        if (grids[k].n_levels > 1 && std::max<uint32_t>(get_tape(ctx, tapes[k])->n_mm, 1) > kMaxMinMax)
+            return fail(ctx, FREP_UNSUPPORTED,
+                        "tape has more than " + std::to_string(kMaxMinMax) + " MIN/MAX instructions");
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    for (uint32_t k = 0; k < n; ++k) {
+        const frep_status s =
+            prepare_job(ctx, get_tape(ctx, tapes[k]), get_tree(ctx, trees[k]), &grids[k], &jobs[k]);
+        if (s != FREP_OK) return s;
+    }
+    if (ctx->timing) ctx->last = frep_timing{};
+    std::vector<uint32_t> pending(n);
+    for (uint32_t k = 0; k < n; ++k) pending[k] = k;
+    for (uint32_t attempt = 0; !pending.empty(); ++attempt) {
+        for (uint32_t k : pending) {
+            const frep_status s = ensure_job(ctx, &jobs[k]);
+            if (s != FREP_OK) return s;
+        }
+        {
+            const frep_status s = run_batch(ctx, jobs, pending);
+            if (s != FREP_OK) return s;
+        }
+        std::vector<uint32_t> again;
+        for (uint32_t k : pending) {
+            if (jobs[k].t->h_counters[CNT_OVERFLOW] == 0) {
+                jobs[k].t->built = true;
+                continue;
+            }
+            const frep_status s = regrow_job(ctx, &jobs[k], attempt);
+            if (s != FREP_OK) return s;
+            again.push_back(k);
+        }
+        pending.swap(again);
+    }
     return FREP_OK;
 }
 

@@ -24,11 +24,16 @@ enum : uint32_t { MM_NONE = 0, MM_FIRST = 1, MM_SECOND = 2, MM_BOTH = 3 };
 //               One packed atomic per wavefront reserves both child ids and
 //               tape space.  Level 0 always has exactly one node.
 //   [16]        pool offset where level-1 tapes start
-//   [17]        overflow flags: 1 = node capacity, 2 = tape pool
+//   [17]        overflow flags, two bits per level L (bits 2L, 2L+1): the
+//               launch of level L ran out of node capacity (OVF_NODES) or
+//               tape pool (OVF_POOL) for level L+1
 // Level L's tapes occupy [base_L, base_L + words_L) with base_1 = [16] and
 // base_{L+1} = base_L + words_L.
 enum : uint32_t { CNT_ALLOC = 0, CNT_BASE1 = 16, CNT_OVERFLOW = 17, CNT_WORDS = 32 };
 enum : uint32_t { OVF_NODES = 1u, OVF_POOL = 2u };
+// Overflow bits of the levels before `level`: a launch of `level` has nothing
+// to do for a tree whose earlier level overflowed (the host rebuilds it).
+__host__ __device__ inline uint32_t ovf_before(uint32_t level) { return (1u << (2 * level)) - 1u; }
 constexpr uint32_t kAllocShift = 38;
 constexpr uint64_t kAllocWordMask = (1ull << kAllocShift) - 1;
 constexpr uint32_t kMaxLevelNodes = (1u << (64 - kAllocShift)) - 1;   // 2^26 - 1
@@ -66,7 +71,28 @@ struct LevelArgs {
     float cs;                 // cell size at this level
     float low[3];
     float tape_time;
+    uint32_t pk;              // point level: voxels per lane (8, 4 or 1)
+    uint32_t use_asm;         // point level: assembly interpreter (pk == 8, slot file <= 16)
 };
+
+// One tree of a batch: what the init kernel needs.
+struct InitArgs {
+    uint32_t* counters;
+    uint32_t* nodes0;
+    uint32_t* pool;
+    const uint32_t* root_tape;
+    uint32_t root_off;
+    uint32_t pad;
+};
+
+// A batched level launch: one persistent grid over the chunks of every
+// shape in the list (at most kBatchMax shapes, one per lane of the plan).
+struct BatchArgs {
+    const LevelArgs* shapes;  // device array
+    uint32_t n_shapes;
+    uint32_t mm_stride;       // LDS choice rows per wavefront (max over shapes)
+};
+constexpr uint32_t kBatchMax = 64;
 
 struct RenderLevel {
     const uint32_t* nodes;
@@ -108,6 +134,17 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
 // asm_ok: every op of the tape is in the assembly interpreter's set.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                               uint32_t cu_count, uint32_t max_grid, bool asm_ok);
+// Batched forms: init of n trees, one level over many shapes, and a copy of
+// the n counter blocks into out (n * CNT_WORDS words) for one readback.
+// max_grid caps the persistent grid (the autotuner varies it).
+hipError_t launch_init_batch(hipStream_t st, const InitArgs* d_args, uint32_t n);
+hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
+                                       uint32_t cu_count, uint32_t max_grid);
+hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
+                                    uint32_t cu_count, uint32_t max_grid);
+hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32_t n,
+                                  uint32_t* out);
+uint32_t point_k_used(uint32_t maxs, uint32_t dim);
 hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs);
 hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
                               const float* xyz, uint32_t n, float t, float* out, uint32_t maxs,

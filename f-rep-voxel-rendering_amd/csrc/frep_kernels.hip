@@ -10,6 +10,9 @@
 #include "frep_device.hpp"
 #include "frep_internal.hpp"
 
+#include <map>
+#include <utility>
+
 namespace frep {
 
 namespace {
@@ -95,242 +98,249 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 // (shorten.comp:188-191).
 // ---------------------------------------------------------------------------
 template <int MAXS>
-__global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
+__device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
+                                               uint64_t child_base, uint32_t lane, uint4* scr) {
     constexpr int AW = (MAXS + 31) / 32;
+    const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
+    uint32_t* rec = a.nodes + (size_t)node * a.ns;
+    ctape_t hdr = as_const(rec);
+    const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
+    const uint32_t idx = sub * 64 + lane;
+    uint32_t cx, cy, cz;
+    cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
+    const uint32_t gx = ncx * dim + cx, gy = ncy * dim + cy, gz = ncz * dim + cz;
+    // main.comp:113-116, 124-126: box [pos, pos + cell_size]
+    const float lx = a.low[0] + (float)gx * a.cs;
+    const float ly = a.low[1] + (float)gy * a.cs;
+    const float lz = a.low[2] + (float)gz * a.cs;
+    const uint32_t n = as_const(a.pool + toff)[0];
+    const uint32_t* tw = a.pool + toff + 1;             // instruction words
+
+    // ---- forward pass: interval evaluation (interval.comp:121-160) ----
+    float slo[MAXS], shi[MAXS];
+    slo[0] = lx; shi[0] = lx + a.cs;
+    slo[1] = ly; shi[1] = ly + a.cs;
+    slo[2] = lz; shi[2] = lz + a.cs;
+    slo[3] = a.tape_time; shi[3] = a.tape_time;
+    uint32_t mm = 0;
+    // three windows in flight: the current one (with its constants), the
+    // next (constants requested), the one after (words requested)
+    uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
+    uint32_t fw2 = window_load(tw, 128, n, lane);
+    float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
+    for (uint32_t s = 0; s < n; s += 64) {
+        const uint32_t m = n - s < 64 ? n - s : 64;
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t w = rdlane(fw0, j);
+            // operand fields are slot indices except for CONST (pool
+            // index); masking keeps every register-file access in range
+            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+            const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+            const Itv A{slo[ia], shi[ia]};
+            FREP_FENCE();
+            Itv r{0.0f, 0.0f};
+            // op codes: 0 CONST 1 SIN 2 COS 3 ADD 4 SUB 5 MUL 6 DIV 7 MIN
+            //           8 MAX 9 EXP 10 NEG 11 SQRT 12 COPY
+            if (op & 8u) {
+                if (op & 4u) {
+                    r = A;
+                } else if (op & 2u) {
+                    if (op & 1u)
+                        r = Itv{__builtin_sqrtf(fmax_glm(0.0f, A.lo)),
+                                __builtin_sqrtf(fmax_glm(0.0f, A.hi))};
+                    else
+                        r = Itv{-A.hi, -A.lo};
+                } else if (op & 1u) {
+                    r = Itv{dev_expf(A.lo), dev_expf(A.hi)};
+                } else {
+                    const Itv B{slo[ib], shi[ib]};
+                    FREP_FENCE();
+                    r = Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
+                    const uint32_t ch = mm_choice(OP_MAX, A, B);
+                    const uint64_t F = __ballot((ch & MM_FIRST) != 0);
+                    const uint64_t S = __ballot((ch & MM_SECOND) != 0);
+                    scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
+                                         (uint32_t)(S >> 32));
+                    ++mm;
+                }
+            } else if (op & 4u) {
+                const Itv B{slo[ib], shi[ib]};
+                FREP_FENCE();
+                if (op & 2u) {
+                    if (op & 1u) {
+                        r = Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)};
+                        const uint32_t ch = mm_choice(OP_MIN, A, B);
+                        const uint64_t F = __ballot((ch & MM_FIRST) != 0);
+                        const uint64_t S = __ballot((ch & MM_SECOND) != 0);
+                        scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
+                                             (uint32_t)(S >> 32));
+                        ++mm;
+                    } else {
+                        r = interval_op(OP_DIV, A, B, 0.0f);
+                    }
+                } else if (op & 1u) {
+                    r = imul(A, B);
+                } else {
+                    r = Itv{A.lo - B.hi, A.hi - B.lo};
+                }
+            } else if (op & 2u) {
+                if (op & 1u) {
+                    const Itv B{slo[ib], shi[ib]};
+                    FREP_FENCE();
+                    r = Itv{A.lo + B.lo, A.hi + B.hi};
+                } else {
+                    r = interval_op(OP_COS, A, A, 0.0f);
+                }
+            } else if (op & 1u) {
+                r = interval_op(OP_SIN, A, A, 0.0f);
+            } else {
+                const float c = rdlanef(fc0, j);
+                r = Itv{c, c};
+            }
+            FREP_FENCE();
+            slo[o] = r.lo;
+            shi[o] = r.hi;
+        }
+        fw0 = fw1; fc0 = fc1;
+        fw1 = fw2; fc1 = window_consts(fw2, a.consts);
+        fw2 = window_load(tw, (int32_t)s + 192, n, lane);
+    }
+    // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
+    const float ilo = slo[0], ihi = shi[0];
+    const bool full = ihi < 0.0f;
+    const bool empty = ilo > 0.0f;
+    const bool amb = !full && !empty;
+    const uint64_t Bfull = __ballot(full);
+    const uint64_t Bamb = __ballot(amb);
+    store_mask64(rec + 4 + sub * 2, Bfull, lane);
+    store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
+
+    uint32_t child_id = kNoChild;
+    if (Bamb != 0) {
+        const uint32_t namb = (uint32_t)__popcll(Bamb);
+        const uint32_t rank = lane_rank(Bamb);
+        const uint32_t stride = n + 1;
+        // one packed atomic reserves namb child ids and namb*stride words
+        unsigned long long old = 0;
+        if (lane == 0)
+            old = atomicAdd((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
+                            ((unsigned long long)namb << kAllocShift) |
+                                ((unsigned long long)namb * stride));
+        old = ((unsigned long long)uni((uint32_t)__shfl((int)(uint32_t)(old >> 32), 0)) << 32) |
+              uni((uint32_t)__shfl((int)(uint32_t)old, 0));
+        const uint32_t cbase = alloc_nodes(old);
+        const uint64_t pbase = child_base + alloc_words(old);
+        const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
+        const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
+        if (!(nodes_ok && pool_ok)) {
+            if (lane == 0)
+                atomicOr(a.counters + CNT_OVERFLOW,
+                         ((nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL)) << (2 * a.level));
+        } else {
+            // ---- backward pass: liveness + compaction (shorten.comp:164-223) ----
+            // Runs with the whole wavefront active (the tape windows are
+            // loaded by every lane); only ambiguous lanes keep anything.
+            const uint64_t mybase = pbase + (uint64_t)rank * stride;
+            uint32_t* out = a.pool + mybase;
+            uint32_t act[AW];
+#pragma unroll
+            for (int k = 0; k < AW; ++k) act[k] = 0;
+            act[0] = amb ? 1u : 0u;                        // slot 0 is the result
+            uint32_t kept = 0;
+            int32_t jm = (int32_t)mm - 1;
+            uint4 row = scr[jm > 0 ? jm : 0];              // next choice row, read ahead
+            int32_t e = (int32_t)n;                        // window = words [e-64, e)
+            uint32_t bw0 = window_load(tw, e - 64, n, lane);
+            uint32_t bw1 = window_load(tw, e - 128, n, lane);
+            uint32_t bw2 = window_load(tw, e - 192, n, lane);
+            while (e > 0) {
+                const int32_t s = e - 64;
+                const int32_t lo = s > 0 ? s : 0;
+                for (int32_t i = e - 1; i >= lo; --i) {
+                    const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
+                    const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+                    const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1);
+                    const uint32_t ib = (w >> 24) & (MAXS - 1);
+                    const bool is_mm = (op == OP_MIN || op == OP_MAX);
+                    uint32_t ch = 0;
+                    if (is_mm) {
+                        const uint4 v = row;
+                        --jm;
+                        row = scr[jm > 0 ? jm : 0];
+                        const uint64_t F = ((uint64_t)v.y << 32) | v.x;
+                        const uint64_t S = ((uint64_t)v.w << 32) | v.z;
+                        ch = (uint32_t)((F >> lane) & 1u) |
+                             ((uint32_t)((S >> lane) & 1u) << 1);
+                    }
+                    if ((act[o >> 5] >> (o & 31u)) & 1u) {
+                        act[o >> 5] &= ~(1u << (o & 31u));  // output first (shorten.comp:179)
+                        uint32_t sel;
+                        if (op == OP_CONST) sel = MM_NONE;
+                        else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
+                            sel = MM_BOTH;
+                        else if (is_mm) sel = ch;
+                        else sel = MM_FIRST;
+                        if (sel & MM_FIRST) act[ia >> 5] |= 1u << (ia & 31u);
+                        if (sel & MM_SECOND) act[ib >> 5] |= 1u << (ib & 31u);
+                        uint32_t ow = w;
+                        if (is_mm && ch == MM_FIRST) ow = (w & ~0xFFu) | OP_COPY;
+                        else if (is_mm && ch == MM_SECOND)
+                            ow = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
+                        out[n - kept] = ow;
+                        ++kept;
+                    }
+                }
+                bw0 = bw1;
+                bw1 = bw2;
+                bw2 = window_load(tw, e - 256, n, lane);
+                e -= 64;
+            }
+            if (amb) {
+                uint32_t final_off = toff;
+                if (kept < n) {
+                    out[n - kept] = kept;
+                    final_off = (uint32_t)(mybase + n - kept);
+                }
+                child_id = cbase + rank;
+                uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
+                crec[0] = final_off;
+                crec[1] = gx;
+                crec[2] = gy;
+                crec[3] = gz;
+            }
+        }
+    }
+    rec[4 + 2 * W + idx] = child_id;
+}
+
+// Node count of `level` and first pool word of level+1's tapes (tapes of
+// level+1 start after those of levels 1..level).
+__device__ __forceinline__ uint64_t child_tape_base(const uint32_t* counters, uint32_t level) {
+    uint64_t b = load_counter(counters + CNT_BASE1);
+    for (uint32_t j = 1; j <= level; ++j) b += alloc_words(load_alloc(counters, j));
+    return b;
+}
+
+template <int MAXS>
+__global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
     // min/max choice ballots of this wavefront's current chunk: mm_stride rows
     // of {FIRST lanes, SECOND lanes} (two 64-bit masks) per wavefront
     extern __shared__ uint4 lds_choice[];
-    if (load_counter(a.counters + CNT_OVERFLOW) != 0) return;
+    if ((load_counter(a.counters + CNT_OVERFLOW) & ovf_before(a.level)) != 0) return;
     const uint32_t lane = lane_id();
     const uint32_t wib = uni(threadIdx.x >> 6);
     const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + wib;
     const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
     uint32_t n_nodes = level_nodes(a.counters, a.level);
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
-    // tapes of level+1 start after those of levels 1..level
-    uint64_t child_base = load_counter(a.counters + CNT_BASE1);
-    for (uint32_t j = 1; j <= a.level; ++j) child_base += alloc_words(load_alloc(a.counters, j));
+    uint64_t child_base = child_tape_base(a.counters, a.level);
     child_base = ((uint64_t)uni((uint32_t)(child_base >> 32)) << 32) | uni((uint32_t)child_base);
-    const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
-    const uint32_t cpn = cells / 64;
+    const uint32_t cpn = a.dim * a.dim * a.dim / 64;
     const uint32_t total = n_nodes * cpn;
     uint4* scr = lds_choice + (size_t)wib * a.mm_stride;
-    cconst_t consts = as_const(a.consts);
-    const float kPi = 3.14159265358979323846f;
-
     for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
         const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
-        uint32_t* rec = a.nodes + (size_t)node * a.ns;
-        ctape_t hdr = as_const(rec);
-        const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
-        const uint32_t idx = sub * 64 + lane;
-        uint32_t cx, cy, cz;
-        cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
-        const uint32_t gx = ncx * dim + cx, gy = ncy * dim + cy, gz = ncz * dim + cz;
-        // main.comp:113-116, 124-126: box [pos, pos + cell_size]
-        const float lx = a.low[0] + (float)gx * a.cs;
-        const float ly = a.low[1] + (float)gy * a.cs;
-        const float lz = a.low[2] + (float)gz * a.cs;
-        const uint32_t n = as_const(a.pool + toff)[0];
-        const uint32_t* tw = a.pool + toff + 1;             // instruction words
-
-        // ---- forward pass: interval evaluation (interval.comp:121-160) ----
-        float slo[MAXS], shi[MAXS];
-        slo[0] = lx; shi[0] = lx + a.cs;
-        slo[1] = ly; shi[1] = ly + a.cs;
-        slo[2] = lz; shi[2] = lz + a.cs;
-        slo[3] = a.tape_time; shi[3] = a.tape_time;
-        uint32_t mm = 0;
-        // three windows in flight: the current one (with its constants), the
-        // next (constants requested), the one after (words requested)
-        uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
-        uint32_t fw2 = window_load(tw, 128, n, lane);
-        float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
-        for (uint32_t s = 0; s < n; s += 64) {
-            const uint32_t m = n - s < 64 ? n - s : 64;
-            for (uint32_t j = 0; j < m; ++j) {
-                const uint32_t w = rdlane(fw0, j);
-                // operand fields are slot indices except for CONST (pool
-                // index); masking keeps every register-file access in range
-                const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-                const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
-                const Itv A{slo[ia], shi[ia]};
-                FREP_FENCE();
-                Itv r{0.0f, 0.0f};
-                // op codes: 0 CONST 1 SIN 2 COS 3 ADD 4 SUB 5 MUL 6 DIV 7 MIN
-                //           8 MAX 9 EXP 10 NEG 11 SQRT 12 COPY
-                if (op & 8u) {
-                    if (op & 4u) {
-                        r = A;
-                    } else if (op & 2u) {
-                        if (op & 1u)
-                            r = Itv{__builtin_sqrtf(fmax_glm(0.0f, A.lo)),
-                                    __builtin_sqrtf(fmax_glm(0.0f, A.hi))};
-                        else
-                            r = Itv{-A.hi, -A.lo};
-                    } else if (op & 1u) {
-                        r = Itv{dev_expf(A.lo), dev_expf(A.hi)};
-                    } else {
-                        const Itv B{slo[ib], shi[ib]};
-                        FREP_FENCE();
-                        r = Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
-                        const uint32_t ch = mm_choice(OP_MAX, A, B);
-                        const uint64_t F = __ballot((ch & MM_FIRST) != 0);
-                        const uint64_t S = __ballot((ch & MM_SECOND) != 0);
-                        scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
-                                             (uint32_t)(S >> 32));
-                        ++mm;
-                    }
-                } else if (op & 4u) {
-                    const Itv B{slo[ib], shi[ib]};
-                    FREP_FENCE();
-                    if (op & 2u) {
-                        if (op & 1u) {
-                            r = Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)};
-                            const uint32_t ch = mm_choice(OP_MIN, A, B);
-                            const uint64_t F = __ballot((ch & MM_FIRST) != 0);
-                            const uint64_t S = __ballot((ch & MM_SECOND) != 0);
-                            scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
-                                                 (uint32_t)(S >> 32));
-                            ++mm;
-                        } else {
-                            r = interval_op(OP_DIV, A, B, 0.0f);
-                        }
-                    } else if (op & 1u) {
-                        r = imul(A, B);
-                    } else {
-                        r = Itv{A.lo - B.hi, A.hi - B.lo};
-                    }
-                } else if (op & 2u) {
-                    if (op & 1u) {
-                        const Itv B{slo[ib], shi[ib]};
-                        FREP_FENCE();
-                        r = Itv{A.lo + B.lo, A.hi + B.hi};
-                    } else {
-                        r = interval_op(OP_COS, A, A, 0.0f);
-                    }
-                } else if (op & 1u) {
-                    r = interval_op(OP_SIN, A, A, 0.0f);
-                } else {
-                    const float c = rdlanef(fc0, j);
-                    r = Itv{c, c};
-                }
-                FREP_FENCE();
-                slo[o] = r.lo;
-                shi[o] = r.hi;
-            }
-            fw0 = fw1; fc0 = fc1;
-            fw1 = fw2; fc1 = window_consts(fw2, a.consts);
-            fw2 = window_load(tw, (int32_t)s + 192, n, lane);
-        }
-        (void)kPi;
-        (void)consts;
-        // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
-        const float ilo = slo[0], ihi = shi[0];
-        const bool full = ihi < 0.0f;
-        const bool empty = ilo > 0.0f;
-        const bool amb = !full && !empty;
-        const uint64_t Bfull = __ballot(full);
-        const uint64_t Bamb = __ballot(amb);
-        store_mask64(rec + 4 + sub * 2, Bfull, lane);
-        store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
-
-        uint32_t child_id = kNoChild;
-        if (Bamb != 0) {
-            const uint32_t namb = (uint32_t)__popcll(Bamb);
-            const uint32_t rank = lane_rank(Bamb);
-            const uint32_t stride = n + 1;
-            // one packed atomic reserves namb child ids and namb*stride words
-            unsigned long long old = 0;
-            if (lane == 0)
-                old = atomicAdd((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
-                                ((unsigned long long)namb << kAllocShift) |
-                                    ((unsigned long long)namb * stride));
-            old = ((unsigned long long)uni((uint32_t)__shfl((int)(uint32_t)(old >> 32), 0)) << 32) |
-                  uni((uint32_t)__shfl((int)(uint32_t)old, 0));
-            const uint32_t cbase = alloc_nodes(old);
-            const uint64_t pbase = child_base + alloc_words(old);
-            const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
-            const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
-            if (!(nodes_ok && pool_ok)) {
-                if (lane == 0)
-                    atomicOr(a.counters + CNT_OVERFLOW,
-                             (nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL));
-            } else {
-                // ---- backward pass: liveness + compaction (shorten.comp:164-223) ----
-                // Runs with the whole wavefront active (the tape windows are
-                // loaded by every lane); only ambiguous lanes keep anything.
-                const uint64_t mybase = pbase + (uint64_t)rank * stride;
-                uint32_t* out = a.pool + mybase;
-                uint32_t act[AW];
-#pragma unroll
-                for (int k = 0; k < AW; ++k) act[k] = 0;
-                act[0] = amb ? 1u : 0u;                        // slot 0 is the result
-                uint32_t kept = 0;
-                int32_t jm = (int32_t)mm - 1;
-                uint4 row = scr[jm > 0 ? jm : 0];              // next choice row, read ahead
-                int32_t e = (int32_t)n;                        // window = words [e-64, e)
-                uint32_t bw0 = window_load(tw, e - 64, n, lane);
-                uint32_t bw1 = window_load(tw, e - 128, n, lane);
-                uint32_t bw2 = window_load(tw, e - 192, n, lane);
-                while (e > 0) {
-                    const int32_t s = e - 64;
-                    const int32_t lo = s > 0 ? s : 0;
-                    for (int32_t i = e - 1; i >= lo; --i) {
-                        const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
-                        const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-                        const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1);
-                        const uint32_t ib = (w >> 24) & (MAXS - 1);
-                        const bool is_mm = (op == OP_MIN || op == OP_MAX);
-                        uint32_t ch = 0;
-                        if (is_mm) {
-                            const uint4 v = row;
-                            --jm;
-                            row = scr[jm > 0 ? jm : 0];
-                            const uint64_t F = ((uint64_t)v.y << 32) | v.x;
-                            const uint64_t S = ((uint64_t)v.w << 32) | v.z;
-                            ch = (uint32_t)((F >> lane) & 1u) |
-                                 ((uint32_t)((S >> lane) & 1u) << 1);
-                        }
-                        if ((act[o >> 5] >> (o & 31u)) & 1u) {
-                            act[o >> 5] &= ~(1u << (o & 31u));  // output first (shorten.comp:179)
-                            uint32_t sel;
-                            if (op == OP_CONST) sel = MM_NONE;
-                            else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
-                                sel = MM_BOTH;
-                            else if (is_mm) sel = ch;
-                            else sel = MM_FIRST;
-                            if (sel & MM_FIRST) act[ia >> 5] |= 1u << (ia & 31u);
-                            if (sel & MM_SECOND) act[ib >> 5] |= 1u << (ib & 31u);
-                            uint32_t ow = w;
-                            if (is_mm && ch == MM_FIRST) ow = (w & ~0xFFu) | OP_COPY;
-                            else if (is_mm && ch == MM_SECOND)
-                                ow = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
-                            out[n - kept] = ow;
-                            ++kept;
-                        }
-                    }
-                    bw0 = bw1;
-                    bw1 = bw2;
-                    bw2 = window_load(tw, e - 256, n, lane);
-                    e -= 64;
-                }
-                if (amb) {
-                    uint32_t final_off = toff;
-                    if (kept < n) {
-                        out[n - kept] = kept;
-                        final_off = (uint32_t)(mybase + n - kept);
-                    }
-                    child_id = cbase + rank;
-                    uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
-                    crec[0] = final_off;
-                    crec[1] = gx;
-                    crec[2] = gy;
-                    crec[3] = gz;
-                }
-            }
-        }
-        rec[4 + 2 * W + idx] = child_id;
+        interval_chunk<MAXS>(a, node, sub, child_base, lane, scr);
     }
 }
 
@@ -339,53 +349,217 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
 // (vulkan/shaders/voxelizer/main.comp:74-83).  Each lane evaluates K voxels
 // of the same node (same tape) so instruction decode is shared K ways.
 // ---------------------------------------------------------------------------
+template <int MAXS, int K, bool ASM>
+__device__ __forceinline__ void point_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
+                                            uint32_t lane) {
+    static_assert(!ASM || (K == 8 && (MAXS == 8 || MAXS == 16)), "assembly path shape");
+    const uint32_t dim = a.dim;
+    uint32_t* rec = a.nodes + (size_t)node * a.ns;
+    ctape_t hdr = as_const(rec);
+    const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
+    auto init = [&](int k) {
+        const uint32_t idx = (sub * K + (uint32_t)k) * 64 + lane;
+        uint32_t cx, cy, cz;
+        cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
+        // main.comp:77-79: grid low + float(node.coords * dim + cell) * cell size
+        return make_float3(a.low[0] + (float)(ncx * dim + cx) * a.cs,
+                           a.low[1] + (float)(ncy * dim + cy) * a.cs,
+                           a.low[2] + (float)(ncz * dim + cz) * a.cs);
+    };
+    auto out = [&](int k, float d) {
+        const uint64_t B = __ballot(d < 0.0f);
+        store_mask64(rec + 4 + (sub * K + (uint32_t)k) * 2, B, lane);
+    };
+    if constexpr (ASM) {
+        v8f xs, ys, zs;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float3 p = init(k);
+            xs[k] = p.x; ys[k] = p.y; zs[k] = p.z;
+        }
+        const v8f d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) out(k, d[k]);
+    } else {
+        run_point_tape<MAXS, K>(as_const(a.pool + toff), as_const(a.consts), a.tape_time, init,
+                                out);
+    }
+}
+
 template <int MAXS, int K, bool ASM = false>
 __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
-    static_assert(!ASM || (K == 8 && (MAXS == 8 || MAXS == 16)), "assembly path shape");
-    if (load_counter(a.counters + CNT_OVERFLOW) != 0) return;
+    if ((load_counter(a.counters + CNT_OVERFLOW) & ovf_before(a.level)) != 0) return;
     const uint32_t lane = lane_id();
     const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
     const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
     uint32_t n_nodes = level_nodes(a.counters, a.level);
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
-    const uint32_t dim = a.dim, cells = dim * dim * dim;
-    const uint32_t cpw = cells / (64 * K);
+    const uint32_t cpw = a.dim * a.dim * a.dim / (64 * K);
     const uint32_t total = n_nodes * cpw;
-    cconst_t consts = as_const(a.consts);
-
     for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
         const uint32_t node = chunk / cpw, sub = chunk - node * cpw;
-        uint32_t* rec = a.nodes + (size_t)node * a.ns;
-        ctape_t hdr = as_const(rec);
-        const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
-        auto init = [&](int k) {
-            const uint32_t idx = (sub * K + (uint32_t)k) * 64 + lane;
-            uint32_t cx, cy, cz;
-            cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
-            // main.comp:77-79: grid low + float(node.coords * dim + cell) * cell size
-            return make_float3(a.low[0] + (float)(ncx * dim + cx) * a.cs,
-                               a.low[1] + (float)(ncy * dim + cy) * a.cs,
-                               a.low[2] + (float)(ncz * dim + cz) * a.cs);
-        };
-        auto out = [&](int k, float d) {
-            const uint64_t B = __ballot(d < 0.0f);
-            store_mask64(rec + 4 + (sub * K + (uint32_t)k) * 2, B, lane);
-        };
-        if constexpr (ASM) {
-            v8f xs, ys, zs;
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                const float3 p = init(k);
-                xs[k] = p.x; ys[k] = p.y; zs[k] = p.z;
-            }
-            const v8f d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time,
-                                                    xs, ys, zs);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) out(k, d[k]);
-        } else {
-            run_point_tape<MAXS, K>(as_const(a.pool + toff), consts, a.tape_time, init, out);
-        }
+        point_chunk<MAXS, K, ASM>(a, node, sub, lane);
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched levels: one persistent grid walks the chunks of many shapes (each
+// with its own tree, tape pool, constants, dims and resolution).  Every
+// wavefront first builds the plan in registers -- lane s holds the inclusive
+// prefix sum of the chunk counts of shapes 0..s, read from each tree's device
+// counters -- and maps a global chunk index to its shape with one ballot.
+// The shape's arguments are reloaded (scalar loads) only when a wavefront
+// moves on to another shape.
+// ---------------------------------------------------------------------------
+namespace {
+
+typedef const __attribute__((address_space(4))) LevelArgs* cargs_t;
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)v, (unsigned)d);
+        if (lane >= (uint32_t)d) v += u;
+    }
+    return v;
+}
+
+// Chunks of shape `lane` at this launch's level, and the first pool word of
+// its child tapes.  Only counts that no wavefront of this launch modifies
+// enter the plan (the node count of this level was final when the previous
+// launch ended), so every wavefront maps chunk indices to shapes the same
+// way; overflow flags, which this launch may set, are checked per shape
+// while walking the chunks.
+template <bool POINT>
+__device__ __forceinline__ uint32_t plan_lane(const BatchArgs& b, uint32_t lane, uint64_t* cbase) {
+    *cbase = 0;
+    if (lane >= b.n_shapes) return 0u;
+    const LevelArgs* A = b.shapes + lane;
+    const uint32_t* cnt = A->counters;
+    uint32_t nn = A->level == 0 ? 1u : alloc_nodes(load_alloc(cnt, A->level));
+    nn = nn < A->node_cap ? nn : A->node_cap;
+    const uint32_t cells = A->dim * A->dim * A->dim;
+    if (!POINT) *cbase = child_tape_base(cnt, A->level);
+    return nn * (POINT ? cells / (64u * A->pk) : cells / 64u);
+}
+
+// Scalar (constant address space) copy of one shape's arguments.
+__device__ __forceinline__ LevelArgs load_args(cargs_t p) {
+    static_assert(sizeof(LevelArgs) % 4 == 0, "LevelArgs is copied as words");
+    typedef const __attribute__((address_space(4))) uint32_t* cw_t;
+    const cw_t w = (cw_t)p;
+    LevelArgs a;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&a);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(LevelArgs) / 4; ++i) d[i] = w[i];
+    return a;
+}
+
+__device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
+    return ((uint64_t)rdlane((uint32_t)(v >> 32), l) << 32) | rdlane((uint32_t)v, l);
+}
+
+}  // namespace
+
+template <int MAXS>
+__global__ __launch_bounds__(kLevelBlock) void k_level_interval_batch(BatchArgs b) {
+    extern __shared__ uint4 lds_choice[];
+    const uint32_t lane = lane_id();
+    const uint32_t wib = uni(threadIdx.x >> 6);
+    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + wib;
+    const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
+    uint64_t cbase;
+    const uint32_t incl = wave_incl_scan(plan_lane<false>(b, lane, &cbase), lane);
+    const uint32_t total = rdlane(incl, 63);
+    uint4* scr = lds_choice + (size_t)wib * b.mm_stride;
+    cargs_t shapes = (cargs_t)b.shapes;
+    uint32_t cur = 0xFFFFFFFFu, start = 0, cpn = 1;
+    uint64_t cb = 0;
+    bool skip = false;
+    LevelArgs a{};
+    for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
+        const uint32_t s = (uint32_t)__popcll(__ballot(incl <= chunk));
+        if (s != cur) {
+            cur = s;
+            a = load_args(shapes + s);
+            start = s ? rdlane(incl, s - 1) : 0u;
+            cb = rdlane64(cbase, s);
+            cpn = a.dim * a.dim * a.dim / 64;
+            // a shape that ran out of capacity at an earlier level is rebuilt
+            // by the host; one that runs out during this launch goes on, so
+            // its counters report what the whole level needs
+            skip = (uni(load_counter(a.counters + CNT_OVERFLOW)) & ovf_before(a.level)) != 0;
+        }
+        if (skip) continue;
+        const uint32_t local = chunk - start;
+        const uint32_t node = local / cpn, sub = local - node * cpn;
+        interval_chunk<MAXS>(a, node, sub, cb, lane, scr);
+    }
+}
+
+template <int MAXS>
+__global__ __launch_bounds__(kLevelBlock) void k_level_point_batch(BatchArgs b) {
+    constexpr int KM = MAXS <= 16 ? 8 : (MAXS <= 32 ? 4 : 1);
+    const uint32_t lane = lane_id();
+    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
+    uint64_t unused;
+    const uint32_t incl = wave_incl_scan(plan_lane<true>(b, lane, &unused), lane);
+    const uint32_t total = rdlane(incl, 63);
+    cargs_t shapes = (cargs_t)b.shapes;
+    uint32_t cur = 0xFFFFFFFFu, start = 0, cpw = 1;
+    bool skip = false;
+    LevelArgs a{};
+    for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
+        const uint32_t s = (uint32_t)__popcll(__ballot(incl <= chunk));
+        if (s != cur) {
+            cur = s;
+            a = load_args(shapes + s);
+            start = s ? rdlane(incl, s - 1) : 0u;
+            cpw = a.dim * a.dim * a.dim / (64u * a.pk);
+            skip = (uni(load_counter(a.counters + CNT_OVERFLOW)) & ovf_before(a.level)) != 0;
+        }
+        if (skip) continue;
+        const uint32_t local = chunk - start;
+        const uint32_t node = local / cpw, sub = local - node * cpw;
+        if constexpr (MAXS <= 16) {
+            if (a.use_asm) {
+                point_chunk<MAXS, 8, true>(a, node, sub, lane);
+                continue;
+            }
+        }
+        if constexpr (KM >= 8) {
+            if (a.pk == 8) {
+                point_chunk<MAXS, 8, false>(a, node, sub, lane);
+                continue;
+            }
+        }
+        if constexpr (KM >= 4) {
+            if (a.pk == 4) {
+                point_chunk<MAXS, 4, false>(a, node, sub, lane);
+                continue;
+            }
+        }
+        point_chunk<MAXS, 1, false>(a, node, sub, lane);
+    }
+}
+
+__global__ void k_init_batch(const InitArgs* args) {
+    const InitArgs a = args[blockIdx.x];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t n = a.root_tape[0];
+    for (uint32_t i = tid; i <= n; i += blockDim.x) a.pool[a.root_off + i] = a.root_tape[i];
+    if (tid < CNT_WORDS)
+        a.counters[tid] = tid == CNT_BASE1 ? a.root_off + ((n + 1 + 7u) & ~7u) : 0u;
+    if (tid == 0) {
+        a.nodes0[0] = a.root_off;
+        a.nodes0[1] = a.nodes0[2] = a.nodes0[3] = 0;
+    }
+}
+
+__global__ void k_gather_counters(const InitArgs* args, uint32_t n, uint32_t* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * CNT_WORDS) out[i] = args[i / CNT_WORDS].counters[i % CNT_WORDS];
 }
 
 // ---------------------------------------------------------------------------
@@ -822,6 +996,61 @@ hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
     FREP_DISPATCH_MAXS(maxs, FREP_POINT_CALL);
 #undef FREP_POINT_CALL
 #undef FREP_POINT_LAUNCH
+    return hipGetLastError();
+}
+
+// Resident blocks per CU of a kernel at a given dynamic LDS size, cached.
+int blocks_per_cu(const void* kern, size_t lds) {
+    thread_local std::map<std::pair<const void*, size_t>, int> cache;
+    const auto key = std::make_pair(kern, lds);
+    const auto it = cache.find(key);
+    if (it != cache.end()) return it->second;
+    int bpc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&bpc, kern, kLevelBlock, lds) != hipSuccess ||
+        bpc < 1)
+        bpc = 1;
+    cache[key] = bpc;
+    return bpc;
+}
+
+uint32_t batch_grid(const void* kern, size_t lds, uint32_t cu_count, uint32_t max_grid) {
+    const uint32_t g = cu_count * (uint32_t)blocks_per_cu(kern, lds);
+    return g < max_grid ? g : max_grid;
+}
+
+hipError_t launch_init_batch(hipStream_t st, const InitArgs* d_args, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    k_init_batch<<<n, 256, 0, st>>>(d_args);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32_t n,
+                                  uint32_t* out) {
+    if (n == 0) return hipSuccess;
+    const uint32_t threads = n * CNT_WORDS;
+    k_gather_counters<<<(threads + 255) / 256, 256, 0, st>>>(d_args, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
+                                       uint32_t cu_count, uint32_t max_grid) {
+    if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
+    const size_t lds = interval_lds_bytes(b.mm_stride);
+    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
+    FREP_DISPATCH_MAXS(maxs, (k_level_interval_batch<M><<<
+                                  batch_grid((const void*)k_level_interval_batch<M>, lds, cu_count,
+                                             max_grid),
+                                  kLevelBlock, lds, st>>>(b)));
+    return hipGetLastError();
+}
+
+hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
+                                    uint32_t cu_count, uint32_t max_grid) {
+    if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
+    FREP_DISPATCH_MAXS(maxs, (k_level_point_batch<M><<<
+                                  batch_grid((const void*)k_level_point_batch<M>, 0, cu_count,
+                                             max_grid),
+                                  kLevelBlock, 0, st>>>(b)));
     return hipGetLastError();
 }
 

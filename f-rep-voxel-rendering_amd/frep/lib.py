@@ -23,7 +23,7 @@ MAX_LEVELS = 8
 EXPORTED = [
     "frep_init", "frep_shutdown", "frep_last_error", "frep_set_timing", "frep_get_timing",
     "frep_device_info", "frep_tape_upload", "frep_tape_free", "frep_tree_create",
-    "frep_voxelize", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
+    "frep_voxelize", "frep_voxelize_batch", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
     "frep_camera_look", "frep_render", "frep_eval_points", "frep_eval_intervals",
     "frep_eval_gradients", "frep_math",
 ]
@@ -61,7 +61,7 @@ class TreeInfo(C.Structure):
 
 class Timing(C.Structure):
     _fields_ = [("n_levels", C.c_uint32), ("level_ms", C.c_float * MAX_LEVELS),
-                ("init_ms", C.c_float), ("render_ms", C.c_float)]
+                ("init_ms", C.c_float), ("render_ms", C.c_float), ("batch_ms", C.c_float)]
 
 
 _lib = None
@@ -89,6 +89,7 @@ def load():
         "frep_tape_free": (C.c_int, [vp, C.c_uint32]),
         "frep_tree_create": (C.c_int, [vp, u32p]),
         "frep_voxelize": (C.c_int, [vp, C.c_uint32, C.POINTER(Grid), C.c_uint32]),
+        "frep_voxelize_batch": (C.c_int, [vp, C.c_uint32, u32p, C.POINTER(Grid), u32p]),
         "frep_tree_info_get": (C.c_int, [vp, C.c_uint32, C.POINTER(TreeInfo)]),
         "frep_tree_export": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint64, u64p]),
         "frep_tree_free": (C.c_int, [vp, C.c_uint32]),
@@ -237,6 +238,20 @@ class Context:
         g = make_grid(grid)
         self._check(self.L.frep_voxelize(self.h, tape, C.byref(g), tree))
         return tree
+
+    def voxelize_batch(self, tapes, grids, trees=None):
+        """Voxelize many (tape, grid) pairs with one launch per level; returns the trees."""
+        n = len(tapes)
+        if trees is None:
+            trees = [self.tree_create() for _ in range(n)]
+        th = np.ascontiguousarray(tapes, dtype=np.uint32)
+        tr = np.ascontiguousarray(trees, dtype=np.uint32)
+        gs = (Grid * max(n, 1))()
+        for i, g in enumerate(grids):
+            gs[i] = make_grid(g)
+        self._check(self.L.frep_voxelize_batch(self.h, n, _p(th, C.c_uint32), gs,
+                                               _p(tr, C.c_uint32)))
+        return list(trees)
 
     def tree_info(self, tree: int) -> TreeInfo:
         info = TreeInfo()

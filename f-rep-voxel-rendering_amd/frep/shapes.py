@@ -13,6 +13,8 @@ and compiled to a tape.  Workloads:
   ``n.p - d``), internal nodes union = min, intersection = max,
   difference = max(a, -b) (vulkan/src/csg/lib.cpp:117-135) chosen uniformly;
   24 primitives, seed 2, tree 16/8/8 (1024^3).
+* ``batch_mix``      -- many sphere unions with log-uniform primitive counts
+  (4..400) at resolutions 128^3..1024^3 (branching per BATCH_RESOLUTIONS).
 * ``gyroid_blend``   -- gyroid lattice with sin/cos, thickened, clipped by a
   box and blended with spheres by a polynomial smooth minimum.
 """
@@ -158,6 +160,28 @@ def gyroid_blend(k: float = 20.0, n_spheres: int = 32, seed: int = 5,
 def make_grid(dims, low=(-1.0, -1.0, -1.0), world_size=2.0, tape_time=0.0) -> dict:
     return {"dims": list(dims), "low": [f32(v) for v in low], "world_size": f32(world_size),
             "tape_time": f32(tape_time)}
+
+
+# Resolutions of the batched mix: effective resolution -> branching per level.
+BATCH_RESOLUTIONS = {128: [8, 4, 4], 256: [8, 8, 4], 512: [8, 8, 8], 1024: [16, 8, 8]}
+
+
+def batch_mix(n_shapes: int = 64, seed_base: int = 1000, min_prims: int = 4,
+              max_prims: int = 400, resolutions=(128, 256, 512, 1024)):
+    """The batched workload: shape k is a union of spheres (as spheres_union,
+    seed seed_base + k) with round(exp(U[ln min_prims, ln max_prims]))
+    spheres, voxelized at a resolution drawn uniformly from `resolutions`.
+    Returns a list of (tape, grid, resolution)."""
+    out = []
+    for k in range(n_shapes):
+        rng = SplitMix64((seed_base + k) ^ 0x5EED)
+        lo, hi = math.log(min_prims), math.log(max_prims)
+        count = int(round(math.exp(lo + (hi - lo) * rng.uniform(0.0, 1.0))))
+        count = max(min_prims, min(max_prims, count))
+        res = resolutions[rng.randint(0, len(resolutions) - 1)]
+        tape = spheres_union(count, seed_base + k)
+        out.append((tape, make_grid(BATCH_RESOLUTIONS[res]), res))
+    return out
 
 
 # Named workloads (shape factory, grid).

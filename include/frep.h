@@ -106,6 +106,7 @@ typedef struct frep_timing {
     float level_ms[FREP_MAX_LEVELS];  /* one kernel per tree level */
     float init_ms;
     float render_ms;
+    float batch_ms;   /* frep_voxelize_batch: all launches + counter readback */
 } frep_timing;
 
 /* ---- context ---- */
@@ -132,6 +133,15 @@ frep_status frep_tree_create(frep_ctx* ctx, frep_tree_t* out);
  * Voxelizer::Voxelize).  Buffers of the tree are reused and grown as needed. */
 frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t tape, const frep_grid* grid,
                           frep_tree_t tree);
+/* Batched form of frep_voxelize (the reference voxelizes one shape per
+ * Voxelizer::Voxelize call and has no batched counterpart): voxelizes n
+ * (tape, grid, tree) triples -- shapes with their
+ * own tape lengths, dims and resolutions -- with one persistent launch per
+ * tree level (and kind) per group of up to 64 trees, and one counter
+ * readback for the whole batch.  Each tree ends up exactly as frep_voxelize
+ * would leave it.  The trees must be distinct. */
+frep_status frep_voxelize_batch(frep_ctx* ctx, uint32_t n, const frep_tape_t* tapes,
+                                const frep_grid* grids, const frep_tree_t* trees);
 frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t tree, frep_tree_info* out);
 /* Canonical blob of the tree (layout in the INTEGRATION notes); returns the
  * word count in *n_words and writes the blob when buf has room (cap words). */

@@ -35,6 +35,7 @@ def test_struct_layouts_match_header_sizes():
     assert C.sizeof(lib.Grid) == 4 * (1 + 8 + 3 + 1 + 1)
     assert C.sizeof(lib.Camera) == 4 * (3 * 4 + 2 + 6 + 4 + 1 + 16 + 16 + 4)
     assert C.sizeof(lib.TreeInfo) == 4 * (1 + 8 * 4) + 4 + 16 + 8
+    assert C.sizeof(lib.Timing) == 4 * (1 + 8 + 3)
 
 
 def test_camera_look_host_helper():

@@ -219,6 +219,59 @@ def test_render_bit_exact(ctx, oracle, name, dims, cfg):
     ctx.tree_free(tree)
 
 
+def test_voxelize_batch_heterogeneous_vs_oracle(ctx, oracle):
+    """One batch mixing tapes (slot files 8 and 16, trig ops, asm and C++
+    interpreters), level counts and branchings; every tree equals the
+    oracle's."""
+    jobs = [
+        (TAPES["spheres8"](), shapes.make_grid([8, 8, 4])),
+        (TAPES["csg24"](), shapes.make_grid([16, 8, 8])),
+        (TAPES["trig"](), shapes.make_grid([8, 8, 4], tape_time=0.5)),
+        (shapes.spheres_union(3, 11), shapes.make_grid([4, 4, 4, 4])),
+        (TAPES["csg24"](), shapes.make_grid([12, 8])),
+        (shapes.spheres_union(20, 12), shapes.make_grid([8, 4, 4])),
+        (TAPES["trig"](), shapes.make_grid([16, 4])),
+        (shapes.spheres_union(5, 13), shapes.make_grid([8])),
+    ]
+    ths = [ctx.upload(t) for t, _ in jobs]
+    trees = ctx.voxelize_batch(ths, [g for _, g in jobs])
+    for (tape, grid), tr in zip(jobs, trees):
+        ref = oracle.Tree(tape, grid)
+        assert list(ctx.tree_info(tr).node_count[:len(grid["dims"])]) == ref.counts()
+        assert np.array_equal(ctx.export(tr), ref.canonical())
+        ctx.tree_free(tr)
+
+
+def test_voxelize_batch_mix_equals_plain(ctx):
+    """The 64-shape batched mix (4..400 spheres, 128^3..1024^3): each tree
+    built in the batch equals the same shape voxelized alone."""
+    mix = shapes.batch_mix(64)
+    ths = [ctx.upload(t) for t, _, _ in mix]
+    trees = ctx.voxelize_batch(ths, [g for _, g, _ in mix])
+    alone = ctx.tree_create()
+    for (tape, grid, _), th, tr in zip(mix, ths, trees):
+        ctx.voxelize(th, grid, alone)
+        assert np.array_equal(ctx.export(tr), ctx.export(alone)), tape.n
+    # the same batch again into the same trees (buffers reused) is unchanged
+    first = ctx.export(trees[5])
+    ctx.voxelize_batch(ths, [g for _, g, _ in mix], trees)
+    assert np.array_equal(ctx.export(trees[5]), first)
+    for tr in trees + [alone]:
+        ctx.tree_free(tr)
+    for th in ths:
+        ctx.tape_free(th)
+
+
+def test_voxelize_batch_rejects_duplicate_tree(ctx):
+    tape = shapes.spheres_union(2, 3)
+    th = ctx.upload(tape)
+    tr = ctx.tree_create()
+    with pytest.raises(lib.FrepError) as e:
+        ctx.voxelize_batch([th, th], [shapes.make_grid([8, 4])] * 2, [tr, tr])
+    assert e.value.status == lib.FREP_INVALID_ARG
+    ctx.tree_free(tr)
+
+
 def test_asm_and_cpp_interpreters_agree(oracle):
     """The hand-scheduled point interpreter (8 points per lane) and the C++
     interpreter give identical trees and point values (FREP_NO_ASM=1 forces

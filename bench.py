@@ -52,15 +52,18 @@ def cpu_baseline(tape, grid, budget_s=12.0):
                       f"dense Tape::Eval semantics with per-call slot allocation"}
 
 
-def bench_batched(ctx, shapes, reps=3):
+def bench_batched(ctx, shapes, reps=5, warm=6):
     """The 64-shape mix (sphere unions of 4..400 primitives at 128^3..1024^3):
     requests/s of one frep_voxelize per shape (plain) against one
-    frep_voxelize_batch over all 64 (batched), same trees, same results."""
+    frep_voxelize_batch over all 64 (batched), same trees, same results.
+    The untimed batched calls grow every tree's buffers, then run the
+    autotuner's timed trials and capture the tuned batch's graph."""
     mix = shapes.batch_mix(64)
     ths = [ctx.upload(t) for t, _, _ in mix]
     grids = [g for _, g, _ in mix]
     trees = [ctx.tree_create() for _ in mix]
-    ctx.voxelize_batch(ths, grids, trees)            # grows every tree's buffers
+    for _ in range(warm):
+        ctx.voxelize_batch(ths, grids, trees)
     for th, g, tr in zip(ths, grids, trees):
         ctx.voxelize(th, g, tr)
     t0 = time.perf_counter()

@@ -242,6 +242,32 @@ def test_voxelize_batch_heterogeneous_vs_oracle(ctx, oracle):
         ctx.tree_free(tr)
 
 
+def test_voxelize_batch_tuned_and_graph_replays(ctx, oracle):
+    """The same batch run repeatedly: the first calls time the autotuner's
+    candidate grid sizes, later ones replay a captured graph.  Every call,
+    with other work on the library's stream in between, gives the oracle's
+    trees."""
+    jobs = [
+        (TAPES["csg24"](), shapes.make_grid([16, 8, 8])),
+        (shapes.spheres_union(20, 12), shapes.make_grid([8, 4, 4])),
+        (TAPES["trig"](), shapes.make_grid([8, 8, 4], tape_time=0.5)),
+        (shapes.spheres_union(40, 14), shapes.make_grid([8, 8, 8])),
+    ]
+    want = [oracle.Tree(t, g).canonical() for t, g in jobs]
+    ths = [ctx.upload(t) for t, _ in jobs]
+    trees = [ctx.tree_create() for _ in jobs]
+    other = ctx.upload(shapes.spheres_union(8, 1))
+    for call in range(7):
+        ctx.voxelize_batch(ths, [g for _, g in jobs], trees)
+        for tr, w in zip(trees, want):
+            assert np.array_equal(ctx.export(tr), w), call
+        # eager work between replays
+        tr = ctx.voxelize(other, shapes.make_grid([8, 8, 4]))
+        ctx.tree_free(tr)
+    for tr in trees:
+        ctx.tree_free(tr)
+
+
 def test_voxelize_batch_mix_equals_plain(ctx):
     """The 64-shape batched mix (4..400 spheres, 128^3..1024^3): each tree
     built in the batch equals the same shape voxelized alone."""

@@ -20,46 +20,12 @@
 
 #include "../../include/frep.h"
 #include "frep_asm.hpp"
+#include "frep_host.hpp"
 #include "frep_internal.hpp"
-
-namespace frep {
-uint32_t point_k_used(uint32_t maxs, uint32_t dim);
-}
 
 using namespace frep;
 
-namespace {
-
-constexpr uint32_t kRootOff = 16;      // words of slack before the root tape
-constexpr uint32_t kPoolPad = 64;      // words of slack after the pool end
-constexpr uint64_t kMaxPoolWords = 0xFFFFFF00ull;
-
-struct TapeObj {
-    std::vector<uint32_t> words;       // [n, instrs]
-    std::vector<float> consts;
-    uint32_t slot_count = 0, maxs = 0, n_mm = 0;
-    bool asm_ok = false;               // all ops in the assembly interpreter's set
-    uint32_t* d_words = nullptr;
-    float* d_consts = nullptr;
-};
-
-struct TreeObj {
-    frep_grid grid{};
-    bool built = false;
-    uint32_t* d_nodes[kMaxLevels] = {};
-    uint32_t node_cap[kMaxLevels] = {};
-    uint32_t ns[kMaxLevels] = {};
-    float cs[kMaxLevels] = {};
-    uint32_t* d_pool = nullptr;
-    uint64_t pool_cap = 0;             // usable words (allocation has kPoolPad more)
-    float* d_consts = nullptr;
-    uint32_t consts_cap = 0;
-    uint32_t* d_counters = nullptr;
-    uint32_t* h_counters = nullptr;    // pinned mirror
-    uint32_t slot_count = 0, maxs = 0;
-    bool asm_ok = false;
-    uint32_t retries = 0;
-};
+namespace frep {
 
 uint32_t maxs_for(uint32_t slot_count) {
     uint32_t m = 8;
@@ -67,62 +33,10 @@ uint32_t maxs_for(uint32_t slot_count) {
     return m;
 }
 
-}  // namespace
-
-namespace {
-// Autotuner state of one batch signature.
-struct TuneRecord {
-    uint32_t trial = 0;                // candidates timed so far
-    std::vector<float> best_ms;        // per launch
-    std::vector<uint32_t> grid;        // per launch: best persistent grid so far
-    hipGraphExec_t exec = nullptr;     // graph of the tuned batch
-};
-}  // namespace
-
-struct frep_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint32_t cu_count = 0;
-    uint32_t level_grid = 0;
-    std::string err;
-    std::mutex mu;
-    std::vector<std::unique_ptr<TapeObj>> tapes;
-    std::vector<std::unique_ptr<TreeObj>> trees;
-    bool timing = false;
-    bool no_asm = false;               // FREP_NO_ASM=1: C++ interpreter only
-    hipEvent_t ev[kMaxLevels + 4] = {};
-    frep_timing last{};
-    // render output staging
-    void* d_out = nullptr;
-    size_t out_bytes = 0;
-    // batched voxelization: argument tables (pinned + device) and the
-    // gathered counter blocks
-    void* h_stage = nullptr;
-    void* d_stage = nullptr;
-    size_t stage_bytes = 0;
-    uint32_t* h_cnt = nullptr;
-    uint32_t* d_cnt = nullptr;
-    size_t cnt_words = 0;
-    // autotuner records and graphs of batches, keyed by their argument tables
-    bool autotune = true;              // FREP_NO_AUTOTUNE=1 turns it off
-    std::unordered_map<uint64_t, TuneRecord> tune;
-    std::vector<hipEvent_t> bev;       // per-launch events of tuning trials
-};
-
-namespace {
-
 frep_status fail(frep_ctx* c, frep_status s, const std::string& msg) {
     if (c) c->err = msg;
     return s;
 }
-
-#define HIP_TRY(call)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess)                                                          \
-            return fail(ctx, e_ == hipErrorOutOfMemory ? FREP_OUT_OF_MEMORY : FREP_DEVICE, \
-                        std::string(#call) + ": " + hipGetErrorString(e_));            \
-    } while (0)
 
 TapeObj* get_tape(frep_ctx* ctx, frep_tape_t h) {
     if (h == 0 || h > ctx->tapes.size()) return nullptr;
@@ -236,7 +150,25 @@ uint64_t host_pool_top(const TreeObj* t) {
     return top;
 }
 
-}  // namespace
+uint64_t tree_device_bytes(const TreeObj* t) {
+    uint64_t bytes = 0;
+    for (uint32_t L = 0; L < kMaxLevels; ++L)
+        if (t->d_nodes[L]) bytes += (uint64_t)t->node_cap[L] * t->ns[L] * 4;
+    if (t->d_pool) bytes += (t->pool_cap + kPoolPad) * 4;
+    return bytes + (uint64_t)t->consts_cap * 4;
+}
+
+frep_status new_tree(frep_ctx* ctx, frep_tree_t* out) {
+    std::unique_ptr<TreeObj> t(new TreeObj());
+    HIP_TRY(hipMalloc(&t->d_counters, CNT_WORDS * 4));
+    HIP_TRY(hipHostMalloc(&t->h_counters, CNT_WORDS * 4, hipHostMallocDefault));
+    std::memset(t->h_counters, 0, CNT_WORDS * 4);
+    ctx->trees.push_back(std::move(t));
+    *out = (frep_tree_t)ctx->trees.size();
+    return FREP_OK;
+}
+
+}  // namespace frep
 
 extern "C" {
 
@@ -281,10 +213,7 @@ void frep_shutdown(frep_ctx* ctx) {
         if (t->h_counters) (void)hipHostFree(t->h_counters);
     }
     if (ctx->d_out) (void)hipFree(ctx->d_out);
-    if (ctx->h_stage) (void)hipHostFree(ctx->h_stage);
-    if (ctx->d_stage) (void)hipFree(ctx->d_stage);
-    if (ctx->h_cnt) (void)hipHostFree(ctx->h_cnt);
-    if (ctx->d_cnt) (void)hipFree(ctx->d_cnt);
+    free_stage(&ctx->stage);
     for (auto& kv : ctx->tune)
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     for (auto& e : ctx->bev) (void)hipEventDestroy(e);
@@ -366,13 +295,7 @@ frep_status frep_tree_create(frep_ctx* ctx, frep_tree_t* out) {
     if (!ctx || !out) return fail(ctx, FREP_INVALID_ARG, "null argument");
     std::lock_guard<std::mutex> lk(ctx->mu);
     HIP_TRY(hipSetDevice(ctx->device));
-    std::unique_ptr<TreeObj> t(new TreeObj());
-    HIP_TRY(hipMalloc(&t->d_counters, CNT_WORDS * 4));
-    HIP_TRY(hipHostMalloc(&t->h_counters, CNT_WORDS * 4, hipHostMallocDefault));
-    std::memset(t->h_counters, 0, CNT_WORDS * 4);
-    ctx->trees.push_back(std::move(t));
-    *out = (frep_tree_t)ctx->trees.size();
-    return FREP_OK;
+    return new_tree(ctx, out);
 }
 
 frep_status frep_tree_free(frep_ctx* ctx, frep_tree_t h) {
@@ -390,21 +313,41 @@ frep_status frep_tree_free(frep_ctx* ctx, frep_tree_t h) {
 
 }  // extern "C"
 
-namespace {
-
-// One voxelization of a batch (or the only one of a plain call): capacities
-// requested for the next attempt.
-struct Job {
-    TapeObj* tp = nullptr;
-    TreeObj* t = nullptr;
-    uint32_t want[kMaxLevels] = {};
-    uint64_t pool_want = 0;
-    uint32_t mm_stride = 1;
-};
+namespace frep {
 
 // Sets up tree t for tape tp over grid: record sizes, cell sizes, first
 // capacity guesses, the tree's copy of the constant pool (enqueued).
-frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid* grid, Job* j) {
+// First capacities of a voxelization: what the tree already holds, the
+// hint, or a default guess, each level capped by the most nodes it can have
+// (n_{L+1} <= n_L * dim_L^3).
+void initial_caps(const TapeObj* tp, const frep_grid* grid, const TreeObj* t, const CapHint* hint,
+                  uint32_t* want, uint64_t* pool_want) {
+    uint64_t bound = 1;
+    for (uint32_t L = 0; L < grid->n_levels; ++L) {
+        uint64_t guess = 1u << 16;
+        if (hint && hint->nodes[L]) guess = (uint64_t)hint->nodes[L] + hint->nodes[L] / 4 + 64;
+        if (t) guess = std::max<uint64_t>(guess, t->node_cap[L]);
+        want[L] = clamp_cap(L == 0 ? 1 : guess, bound);
+        const uint64_t cells = (uint64_t)grid->dims[L] * grid->dims[L] * grid->dims[L];
+        bound = std::min<uint64_t>(bound * cells, 0x7FFFFFFFull);
+    }
+    uint64_t pw = (uint64_t)kRootOff + (tp->words[0] + 1) * 64ull + (1u << 20);
+    if (hint && hint->pool_words) pw = std::max<uint64_t>(pw, hint->pool_words + hint->pool_words / 4);
+    *pool_want = std::max<uint64_t>(t ? t->pool_cap : 0, pw);
+}
+
+uint64_t estimate_job_bytes(const TapeObj* tp, const frep_grid* grid, const CapHint* hint) {
+    uint32_t want[kMaxLevels];
+    uint64_t pool_want = 0;
+    initial_caps(tp, grid, nullptr, hint, want, &pool_want);
+    uint64_t bytes = (pool_want + kPoolPad + tp->consts.size() + kPoolPad) * 4;
+    for (uint32_t L = 0; L < grid->n_levels; ++L)
+        bytes += (uint64_t)want[L] * node_words(grid->dims[L], L + 1 == grid->n_levels) * 4;
+    return bytes;
+}
+
+frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid* grid, Job* j,
+                        hipStream_t st, const CapHint* hint) {
     const uint32_t nl = grid->n_levels;
     const bool same_shape = t->built && t->grid.n_levels == nl &&
                             std::equal(grid->dims, grid->dims + nl, t->grid.dims);
@@ -428,16 +371,7 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
         t->cs[L] = c;
         t->ns[L] = node_words(grid->dims[L], L + 1 == nl);
     }
-    // capacities: upper bound n_{L+1} <= n_L * dim_L^3, first guess capped
-    uint64_t bound = 1;
-    for (uint32_t L = 0; L < nl; ++L) {
-        const uint64_t guess = L == 0 ? 1 : std::max<uint64_t>(t->node_cap[L], 1u << 16);
-        j->want[L] = clamp_cap(guess, bound);
-        const uint64_t cells = (uint64_t)grid->dims[L] * grid->dims[L] * grid->dims[L];
-        bound = std::min<uint64_t>(bound * cells, 0x7FFFFFFFull);
-    }
-    j->pool_want = std::max<uint64_t>(
-        t->pool_cap, (uint64_t)kRootOff + (tp->words[0] + 1) * 64ull + (1u << 20));
+    initial_caps(tp, grid, t, hint, j->want, &j->pool_want);
     const uint32_t nc = (uint32_t)tp->consts.size();
     if (!t->d_consts || t->consts_cap < nc + kPoolPad) {
         if (t->d_consts) HIP_TRY(hipFree(t->d_consts));
@@ -447,7 +381,7 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
     }
     if (nc)
         HIP_TRY(hipMemcpyAsync(t->d_consts, tp->d_consts, (size_t)nc * 4, hipMemcpyDeviceToDevice,
-                               ctx->stream));
+                               st));
     // min/max choice rows live in the interval kernel's LDS
     j->mm_stride = std::max<uint32_t>(tp->n_mm, 1);
     if (nl > 1 && j->mm_stride > kMaxMinMax)
@@ -545,23 +479,6 @@ size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 // (blocks per CU) and keep, per launch, the fastest; later calls run the
 // chosen sizes from a captured hipGraph (replayed with one launch).
 // ---------------------------------------------------------------------------
-struct BatchLaunch {
-    uint32_t group, level;
-    bool point;
-    uint32_t count, maxs, mm;
-    size_t off;
-};
-struct BatchGroup {
-    uint32_t first, count;
-    size_t init_off;
-};
-struct BatchPlan {
-    uint32_t m = 0;
-    std::vector<BatchGroup> groups;
-    std::vector<BatchLaunch> launches;
-    size_t bytes = 0;
-};
-
 constexpr uint32_t kTuneBlocksPerCu[] = {8, 4, 2};   // candidates, 4 wavefronts per block
 constexpr uint32_t kTuneTrials = sizeof(kTuneBlocksPerCu) / sizeof(kTuneBlocksPerCu[0]);
 constexpr size_t kTuneMaxEntries = 64;
@@ -651,34 +568,65 @@ void drop_graphs(frep_ctx* ctx) {
     ctx->tune.clear();
 }
 
-// Enqueues the plan on ctx->stream: args copy, launches with the given
-// persistent-grid caps, counter gather and readback.  With ev != nullptr an
-// event is recorded before the first launch and after each launch.
-frep_status enqueue_batch(frep_ctx* ctx, const BatchPlan& P, const std::vector<uint32_t>& grid,
-                          hipEvent_t* ev) {
-    uint8_t* ds = static_cast<uint8_t*>(ctx->d_stage);
-    HIP_TRY(hipMemcpyAsync(ds, ctx->h_stage, P.bytes, hipMemcpyHostToDevice, ctx->stream));
+frep_status ensure_stage(frep_ctx* ctx, BatchStage* S, const BatchPlan& P) {
+    if (S->stage_bytes < P.bytes) {
+        if (S->h_stage) HIP_TRY(hipHostFree(S->h_stage));
+        if (S->d_stage) HIP_TRY(hipFree(S->d_stage));
+        S->h_stage = nullptr;
+        S->d_stage = nullptr;
+        S->stage_bytes = 0;
+        const size_t cap = P.bytes + P.bytes / 2 + 4096;
+        HIP_TRY(hipHostMalloc(&S->h_stage, cap, hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&S->d_stage, cap));
+        S->stage_bytes = cap;
+    }
+    if (S->cnt_words < (size_t)P.m * CNT_WORDS) {
+        if (S->h_cnt) HIP_TRY(hipHostFree(S->h_cnt));
+        if (S->d_cnt) HIP_TRY(hipFree(S->d_cnt));
+        S->h_cnt = nullptr;
+        S->d_cnt = nullptr;
+        S->cnt_words = 0;
+        const size_t cap = (size_t)std::max<uint32_t>(P.m, kBatchMax) * CNT_WORDS;
+        HIP_TRY(hipHostMalloc(&S->h_cnt, cap * 4, hipHostMallocDefault));
+        HIP_TRY(hipMalloc(&S->d_cnt, cap * 4));
+        S->cnt_words = cap;
+    }
+    return FREP_OK;
+}
+
+void free_stage(BatchStage* S) {
+    if (S->h_stage) (void)hipHostFree(S->h_stage);
+    if (S->d_stage) (void)hipFree(S->d_stage);
+    if (S->h_cnt) (void)hipHostFree(S->h_cnt);
+    if (S->d_cnt) (void)hipFree(S->d_cnt);
+    *S = BatchStage{};
+}
+
+// With ev != nullptr an event is recorded before the first level launch and
+// after each launch.
+frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const BatchPlan& P,
+                          const std::vector<uint32_t>& grid, hipEvent_t* ev) {
+    uint8_t* ds = static_cast<uint8_t*>(S->d_stage);
+    HIP_TRY(hipMemcpyAsync(ds, S->h_stage, P.bytes, hipMemcpyHostToDevice, st));
     size_t li = 0;
     for (uint32_t g = 0; g < P.groups.size(); ++g) {
         const BatchGroup& G = P.groups[g];
         const InitArgs* d_init = reinterpret_cast<const InitArgs*>(ds + G.init_off);
-        HIP_TRY(launch_init_batch(ctx->stream, d_init, G.count));
-        if (ev && g == 0) HIP_TRY(hipEventRecord(ev[0], ctx->stream));
+        HIP_TRY(launch_init_batch(st, d_init, G.count));
+        if (ev && g == 0) HIP_TRY(hipEventRecord(ev[0], st));
         for (; li < P.launches.size() && P.launches[li].group == g; ++li) {
             const BatchLaunch& la = P.launches[li];
             BatchArgs b{reinterpret_cast<const LevelArgs*>(ds + la.off), la.count, la.mm};
             if (la.point)
-                HIP_TRY(launch_level_point_batch(ctx->stream, b, la.maxs, ctx->cu_count, grid[li]));
+                HIP_TRY(launch_level_point_batch(st, b, la.maxs, ctx->cu_count, grid[li]));
             else
-                HIP_TRY(launch_level_interval_batch(ctx->stream, b, la.maxs, ctx->cu_count,
-                                                    grid[li]));
-            if (ev) HIP_TRY(hipEventRecord(ev[1 + li], ctx->stream));
+                HIP_TRY(launch_level_interval_batch(st, b, la.maxs, ctx->cu_count, grid[li]));
+            if (ev) HIP_TRY(hipEventRecord(ev[1 + li], st));
         }
-        HIP_TRY(launch_gather_counters(ctx->stream, d_init, G.count,
-                                       ctx->d_cnt + (size_t)G.first * CNT_WORDS));
+        HIP_TRY(launch_gather_counters(st, d_init, G.count, S->d_cnt + (size_t)G.first * CNT_WORDS));
     }
-    HIP_TRY(hipMemcpyAsync(ctx->h_cnt, ctx->d_cnt, (size_t)P.m * CNT_WORDS * 4,
-                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(S->h_cnt, S->d_cnt, (size_t)P.m * CNT_WORDS * 4, hipMemcpyDeviceToHost,
+                           st));
     return FREP_OK;
 }
 
@@ -698,37 +646,18 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
                       bool tune) {
     const BatchPlan P = plan_batch(jobs, idx);
     if (P.m == 0) return FREP_OK;
-    if (ctx->stage_bytes < P.bytes || ctx->cnt_words < (size_t)P.m * CNT_WORDS) {
+    BatchStage* S = &ctx->stage;
+    if (S->stage_bytes < P.bytes || S->cnt_words < (size_t)P.m * CNT_WORDS) {
         drop_graphs(ctx);   // graphs hold the staging addresses
-        if (ctx->stage_bytes < P.bytes) {
-            if (ctx->h_stage) HIP_TRY(hipHostFree(ctx->h_stage));
-            if (ctx->d_stage) HIP_TRY(hipFree(ctx->d_stage));
-            ctx->h_stage = nullptr;
-            ctx->d_stage = nullptr;
-            ctx->stage_bytes = 0;
-            const size_t cap = P.bytes + P.bytes / 2 + 4096;
-            HIP_TRY(hipHostMalloc(&ctx->h_stage, cap, hipHostMallocDefault));
-            HIP_TRY(hipMalloc(&ctx->d_stage, cap));
-            ctx->stage_bytes = cap;
-        }
-        if (ctx->cnt_words < (size_t)P.m * CNT_WORDS) {
-            if (ctx->h_cnt) HIP_TRY(hipHostFree(ctx->h_cnt));
-            if (ctx->d_cnt) HIP_TRY(hipFree(ctx->d_cnt));
-            ctx->h_cnt = nullptr;
-            ctx->d_cnt = nullptr;
-            ctx->cnt_words = 0;
-            const size_t cap = (size_t)std::max<uint32_t>(P.m, kBatchMax) * CNT_WORDS;
-            HIP_TRY(hipHostMalloc(&ctx->h_cnt, cap * 4, hipHostMallocDefault));
-            HIP_TRY(hipMalloc(&ctx->d_cnt, cap * 4));
-            ctx->cnt_words = cap;
-        }
+        const frep_status s = ensure_stage(ctx, S, P);
+        if (s != FREP_OK) return s;
     }
-    fill_batch(P, jobs, idx, static_cast<uint8_t*>(ctx->h_stage));
+    fill_batch(P, jobs, idx, static_cast<uint8_t*>(S->h_stage));
     const size_t nl = P.launches.size();
     std::vector<uint32_t> grid(nl, ctx->level_grid);
     TuneRecord* rec = nullptr;
     if (tune && ctx->autotune) {
-        const uint64_t key = batch_key(P, static_cast<const uint8_t*>(ctx->h_stage));
+        const uint64_t key = batch_key(P, static_cast<const uint8_t*>(S->h_stage));
         auto it = ctx->tune.find(key);
         if (it == ctx->tune.end()) {
             if (ctx->tune.size() >= kTuneMaxEntries) drop_graphs(ctx);
@@ -755,7 +684,7 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
         if (!rec->exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            const frep_status s = enqueue_batch(ctx, P, grid, nullptr);
+            const frep_status s = enqueue_batch(ctx, S, ctx->stream, P, grid, nullptr);
             const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
             if (s != FREP_OK) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -768,13 +697,13 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
         }
         HIP_TRY(hipGraphLaunch(rec->exec, ctx->stream));
     } else {
-        const frep_status s = enqueue_batch(ctx, P, grid, ev);
+        const frep_status s = enqueue_batch(ctx, S, ctx->stream, P, grid, ev);
         if (s != FREP_OK) return s;
     }
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     for (uint32_t k = 0; k < P.m; ++k)
-        std::memcpy(jobs[idx[k]].t->h_counters, ctx->h_cnt + (size_t)k * CNT_WORDS, CNT_WORDS * 4);
+        std::memcpy(jobs[idx[k]].t->h_counters, S->h_cnt + (size_t)k * CNT_WORDS, CNT_WORDS * 4);
     if (timed_trial) {
         for (size_t li = 0; li < nl; ++li) {
             float ms = 0.0f;
@@ -796,7 +725,62 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
     return FREP_OK;
 }
 
-}  // namespace
+frep_status check_camera(frep_ctx* ctx, const frep_camera* cam) {
+    if (cam->width == 0 || cam->height == 0 || cam->width > 16384 || cam->height > 16384)
+        return fail(ctx, FREP_INVALID_ARG, "image size out of range");
+    if (cam->light_count > 4) return fail(ctx, FREP_INVALID_ARG, "at most 4 lights");
+    return FREP_OK;
+}
+
+RenderArgs render_args(const TreeObj* t, const frep_camera* cam, uint8_t* rgba,
+                       unsigned long long* hit_ids, float* normals) {
+    RenderArgs a{};
+    const uint32_t nl = t->grid.n_levels;
+    for (uint32_t L = 0; L < nl; ++L) {
+        a.lv[L].nodes = t->d_nodes[L];
+        a.lv[L].ns = t->ns[L];
+        a.lv[L].dim = t->grid.dims[L];
+        a.lv[L].cs = t->cs[L];
+        a.lv[L].cap = t->node_cap[L];
+    }
+    a.n_levels = nl;
+    a.pool = t->d_pool;
+    a.pool_words = t->pool_cap + kPoolPad;
+    a.consts = t->d_consts;
+    a.counters = t->d_counters;
+    for (int c = 0; c < 3; ++c) {
+        a.low[c] = t->grid.low[c];
+        a.pos[c] = cam->pos[c];
+        a.fwd[c] = cam->forward[c];
+        a.right[c] = cam->right[c];
+        a.up[c] = cam->up[c];
+    }
+    a.world_size = t->grid.world_size;
+    a.tape_time = t->grid.tape_time;
+    a.sw[0] = cam->screen_world[0];
+    a.sw[1] = cam->screen_world[1];
+    a.width = cam->width;
+    a.height = cam->height;
+    a.spp = cam->spp ? cam->spp : 1;
+    a.ortho = cam->ortho;
+    a.jitter = cam->jitter;
+    a.seed = cam->seed;
+    a.light_count = cam->light_count;
+    for (int c = 0; c < 4; ++c) {
+        a.background[c] = cam->background[c];
+        a.ambient[c] = cam->ambient[c];
+        for (int l = 0; l < 4; ++l) {
+            a.light_dir[l][c] = cam->light_dir[l][c];
+            a.light_color[l][c] = cam->light_color[l][c];
+        }
+    }
+    a.rgba = rgba;
+    a.hit_ids = hit_ids;
+    a.normals = normals;
+    return a;
+}
+
+}  // namespace frep
 
 extern "C" {
 
@@ -813,7 +797,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
     const uint32_t nl = grid->n_levels;
     Job j;
     {
-        const frep_status s = prepare_job(ctx, tp, t, grid, &j);
+        const frep_status s = prepare_job(ctx, tp, t, grid, &j, ctx->stream, nullptr);
         if (s != FREP_OK) return s;
     }
     for (uint32_t attempt = 0;; ++attempt) {
@@ -875,7 +859,8 @@ frep_status frep_voxelize_batch(frep_ctx* ctx, uint32_t n, const frep_tape_t* ta
     HIP_TRY(hipSetDevice(ctx->device));
     for (uint32_t k = 0; k < n; ++k) {
         const frep_status s =
-            prepare_job(ctx, get_tape(ctx, tapes[k]), get_tree(ctx, trees[k]), &grids[k], &jobs[k]);
+            prepare_job(ctx, get_tape(ctx, tapes[k]), get_tree(ctx, trees[k]), &grids[k], &jobs[k],
+                        ctx->stream, nullptr);
         if (s != FREP_OK) return s;
     }
     if (ctx->timing) ctx->last = frep_timing{};
@@ -919,7 +904,6 @@ frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t h, frep_tree_info* out
     const uint64_t top = host_pool_top(t);
     std::vector<uint32_t> pool(top);
     if (top) HIP_TRY(hipMemcpy(pool.data(), t->d_pool, top * 4, hipMemcpyDeviceToHost));
-    uint64_t bytes = 0;
     for (uint32_t L = 0; L < t->grid.n_levels; ++L) {
         const uint32_t n = host_nodes(t, L);
         out->dims[L] = t->grid.dims[L];
@@ -936,11 +920,9 @@ frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t h, frep_tree_info* out
         }
         out->tape_len_sum[L] = (uint32_t)std::min<uint64_t>(sum, 0xFFFFFFFFull);
         out->tape_len_max[L] = mx;
-        bytes += (uint64_t)t->node_cap[L] * t->ns[L] * 4;
     }
     out->tape_pool_words = top;
-    bytes += (t->pool_cap + kPoolPad) * 4 + (uint64_t)t->consts_cap * 4;
-    out->device_bytes = bytes;
+    out->device_bytes = tree_device_bytes(t);
     out->slot_count = t->slot_count;
     out->retries = t->retries;
     return FREP_OK;
@@ -1042,9 +1024,10 @@ frep_status frep_render(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam, ui
     std::lock_guard<std::mutex> lk(ctx->mu);
     TreeObj* t = get_tree(ctx, h);
     if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
-    if (cam->width == 0 || cam->height == 0 || cam->width > 16384 || cam->height > 16384)
-        return fail(ctx, FREP_INVALID_ARG, "image size out of range");
-    if (cam->light_count > 4) return fail(ctx, FREP_INVALID_ARG, "at most 4 lights");
+    {
+        const frep_status s = check_camera(ctx, cam);
+        if (s != FREP_OK) return s;
+    }
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t spp = cam->spp ? cam->spp : 1;
     const size_t px = (size_t)cam->width * cam->height;
@@ -1062,46 +1045,9 @@ frep_status frep_render(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam, ui
         ctx->out_bytes = total;
     }
     char* base = (char*)ctx->d_out;
-    RenderArgs a{};
-    const uint32_t nl = t->grid.n_levels;
-    for (uint32_t L = 0; L < nl; ++L) {
-        a.lv[L].nodes = t->d_nodes[L];
-        a.lv[L].ns = t->ns[L];
-        a.lv[L].dim = t->grid.dims[L];
-        a.lv[L].cs = t->cs[L];
-    }
-    a.n_levels = nl;
-    a.pool = t->d_pool;
-    a.consts = t->d_consts;
-    for (int c = 0; c < 3; ++c) {
-        a.low[c] = t->grid.low[c];
-        a.pos[c] = cam->pos[c];
-        a.fwd[c] = cam->forward[c];
-        a.right[c] = cam->right[c];
-        a.up[c] = cam->up[c];
-    }
-    a.world_size = t->grid.world_size;
-    a.tape_time = t->grid.tape_time;
-    a.sw[0] = cam->screen_world[0];
-    a.sw[1] = cam->screen_world[1];
-    a.width = cam->width;
-    a.height = cam->height;
-    a.spp = spp;
-    a.ortho = cam->ortho;
-    a.jitter = cam->jitter;
-    a.seed = cam->seed;
-    a.light_count = cam->light_count;
-    for (int c = 0; c < 4; ++c) {
-        a.background[c] = cam->background[c];
-        a.ambient[c] = cam->ambient[c];
-        for (int l = 0; l < 4; ++l) {
-            a.light_dir[l][c] = cam->light_dir[l][c];
-            a.light_color[l][c] = cam->light_color[l][c];
-        }
-    }
-    a.rgba = rgba ? (uint8_t*)base : nullptr;
-    a.hit_ids = hit_ids ? (unsigned long long*)(base + off_hit) : nullptr;
-    a.normals = normals ? (float*)(base + off_nrm) : nullptr;
+    const RenderArgs a = render_args(t, cam, rgba ? (uint8_t*)base : nullptr,
+                                     hit_ids ? (unsigned long long*)(base + off_hit) : nullptr,
+                                     normals ? (float*)(base + off_nrm) : nullptr);
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 2], ctx->stream));
     HIP_TRY(launch_render(ctx->stream, a, t->maxs));
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 3], ctx->stream));

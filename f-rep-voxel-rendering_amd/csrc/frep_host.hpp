@@ -1,0 +1,182 @@
+// frep_host.hpp -- host-side objects of the library: uploaded tapes, trees,
+// the context, and the plans of batched voxelizations.  Shared by the C ABI
+// (frep_api.hip) and the request server (frep_server.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <memory>
+#include <mutex>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/frep.h"
+#include "frep_internal.hpp"
+
+namespace frep {
+
+constexpr uint32_t kRootOff = 16;      // words of slack before the root tape
+constexpr uint32_t kPoolPad = 64;      // words of slack after the pool end
+constexpr uint64_t kMaxPoolWords = 0xFFFFFF00ull;
+
+struct TapeObj {
+    std::vector<uint32_t> words;       // [n, instrs]
+    std::vector<float> consts;
+    uint32_t slot_count = 0, maxs = 0, n_mm = 0;
+    bool asm_ok = false;               // all ops in the assembly interpreter's set
+    uint32_t* d_words = nullptr;
+    float* d_consts = nullptr;
+};
+
+struct TreeObj {
+    frep_grid grid{};
+    bool built = false;
+    uint32_t* d_nodes[kMaxLevels] = {};
+    uint32_t node_cap[kMaxLevels] = {};
+    uint32_t ns[kMaxLevels] = {};
+    float cs[kMaxLevels] = {};
+    uint32_t* d_pool = nullptr;
+    uint64_t pool_cap = 0;             // usable words (allocation has kPoolPad more)
+    float* d_consts = nullptr;
+    uint32_t consts_cap = 0;
+    uint32_t* d_counters = nullptr;
+    uint32_t* h_counters = nullptr;    // pinned mirror
+    uint32_t slot_count = 0, maxs = 0;
+    bool asm_ok = false;
+    uint32_t retries = 0;
+};
+
+// Autotuner state of one batch signature.
+struct TuneRecord {
+    uint32_t trial = 0;                // candidates timed so far
+    std::vector<float> best_ms;        // per launch
+    std::vector<uint32_t> grid;        // per launch: best persistent grid so far
+    hipGraphExec_t exec = nullptr;     // graph of the tuned batch
+};
+
+// Argument tables (pinned + device) and gathered counter blocks of one
+// batched voxelization in flight.
+struct BatchStage {
+    void* h_stage = nullptr;
+    void* d_stage = nullptr;
+    size_t stage_bytes = 0;
+    uint32_t* h_cnt = nullptr;
+    uint32_t* d_cnt = nullptr;
+    size_t cnt_words = 0;
+};
+
+// Capacities to start a tree with (node counts per level, tape-pool words),
+// e.g. learned from earlier trees of the same grid.
+struct CapHint {
+    uint32_t nodes[kMaxLevels] = {};
+    uint64_t pool_words = 0;
+};
+
+// One voxelization of a batch (or the only one of a plain call): capacities
+// requested for the next attempt.
+struct Job {
+    TapeObj* tp = nullptr;
+    TreeObj* t = nullptr;
+    uint32_t want[kMaxLevels] = {};
+    uint64_t pool_want = 0;
+    uint32_t mm_stride = 1;
+};
+
+// Launch list of a batched voxelization: groups of <= kBatchMax trees, and
+// per group one interval and/or one point launch per level.
+struct BatchLaunch {
+    uint32_t group, level;
+    bool point;
+    uint32_t count, maxs, mm;
+    size_t off;                        // byte offset of its LevelArgs table
+};
+struct BatchGroup {
+    uint32_t first, count;
+    size_t init_off;                   // byte offset of its InitArgs table
+};
+struct BatchPlan {
+    uint32_t m = 0;
+    std::vector<BatchGroup> groups;
+    std::vector<BatchLaunch> launches;
+    size_t bytes = 0;
+};
+
+}  // namespace frep
+
+struct frep_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t cu_count = 0;
+    uint32_t level_grid = 0;
+    std::string err;
+    std::mutex mu;
+    std::vector<std::unique_ptr<frep::TapeObj>> tapes;
+    std::vector<std::unique_ptr<frep::TreeObj>> trees;
+    bool timing = false;
+    bool no_asm = false;               // FREP_NO_ASM=1: C++ interpreter only
+    hipEvent_t ev[frep::kMaxLevels + 4] = {};
+    frep_timing last{};
+    // render output staging
+    void* d_out = nullptr;
+    size_t out_bytes = 0;
+    // staging of the synchronous batched voxelization
+    frep::BatchStage stage;
+    // autotuner records and graphs of batches, keyed by their argument tables
+    bool autotune = true;              // FREP_NO_AUTOTUNE=1 turns it off
+    std::unordered_map<uint64_t, frep::TuneRecord> tune;
+    std::vector<hipEvent_t> bev;       // per-launch events of tuning trials
+};
+
+// Returns from the enclosing function (which has a frep_ctx* ctx) with the
+// status of a failed HIP call and its message recorded in the context.
+#define HIP_TRY(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::frep::fail(ctx, e_ == hipErrorOutOfMemory ? FREP_OUT_OF_MEMORY      \
+                                                               : FREP_DEVICE,          \
+                                std::string(#call) + ": " + hipGetErrorString(e_));    \
+    } while (0)
+
+namespace frep {
+
+uint32_t maxs_for(uint32_t slot_count);
+frep_status fail(frep_ctx* c, frep_status s, const std::string& msg);
+TapeObj* get_tape(frep_ctx* ctx, frep_tape_t h);
+TreeObj* get_tree(frep_ctx* ctx, frep_tree_t h);
+frep_status check_grid(frep_ctx* ctx, const frep_grid* g);
+void free_tree_buffers(TreeObj* t);
+uint64_t host_alloc(const TreeObj* t, uint32_t L);
+uint32_t host_nodes(const TreeObj* t, uint32_t L);
+uint64_t host_pool_top(const TreeObj* t);
+// Device bytes held by a tree's buffers (records, tape pool, constants).
+uint64_t tree_device_bytes(const TreeObj* t);
+
+// New tree object (no level buffers yet); caller holds ctx->mu.
+frep_status new_tree(frep_ctx* ctx, frep_tree_t* out);
+
+frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid* grid, Job* j,
+                        hipStream_t st, const CapHint* hint);
+// Device bytes prepare_job + ensure_job would give a fresh tree.
+uint64_t estimate_job_bytes(const TapeObj* tp, const frep_grid* grid, const CapHint* hint);
+frep_status ensure_job(frep_ctx* ctx, Job* j);
+frep_status regrow_job(frep_ctx* ctx, Job* j, uint32_t attempt);
+LevelArgs level_args(const TreeObj* t, uint32_t L, uint32_t mm_stride);
+
+BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx);
+void fill_batch(const BatchPlan& P, const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
+                uint8_t* hs);
+frep_status ensure_stage(frep_ctx* ctx, BatchStage* S, const BatchPlan& P);
+void free_stage(BatchStage* S);
+// Enqueues a filled plan on st: argument copy, launches (persistent grids
+// capped at grid[i]), counter gather and readback into S->h_cnt.
+frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const BatchPlan& P,
+                          const std::vector<uint32_t>& grid, hipEvent_t* ev);
+
+// Kernel arguments of a render of tree t (outputs are device pointers).
+RenderArgs render_args(const TreeObj* t, const frep_camera* cam, uint8_t* rgba,
+                       unsigned long long* hit_ids, float* normals);
+frep_status check_camera(frep_ctx* ctx, const frep_camera* cam);
+
+}  // namespace frep

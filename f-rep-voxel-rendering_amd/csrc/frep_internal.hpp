@@ -99,13 +99,16 @@ struct RenderLevel {
     uint32_t ns;
     uint32_t dim;
     float cs;
+    uint32_t cap;             // records the level's buffer holds
 };
 
 struct RenderArgs {
     RenderLevel lv[kMaxLevels];
     uint32_t n_levels;
     const uint32_t* pool;
+    uint64_t pool_words;      // readable words of the tape pool
     const float* consts;
+    const uint32_t* counters; // the tree's counter block (overflow flags)
     float low[3];
     float world_size;
     float tape_time;

@@ -643,6 +643,10 @@ __device__ bool dda(const RenderArgs& a, const float o[3], const float d[3], Hit
                 return true;
             } else if (!last && (rec[4 + W + (idx >> 5)] & bit)) {
                 node = rec[4 + 2 * W + idx];
+                // a child id outside the next level's buffer (a tree whose
+                // build ran out of capacity) ends the ray instead of reading
+                // past the buffer
+                if (node >= a.lv[lvl + 1].cap) return false;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) np[c] = np[c] + (float)cell[c] * cs;
                 t_max = fmin_glm(tn[0], fmin_glm(tn[1], tn[2]));
@@ -683,6 +687,9 @@ __device__ __forceinline__ void normalize3(float v[3]) {
 
 template <int MAXS>
 __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
+    // a tree whose build ran out of capacity is incomplete: render nothing
+    // (the caller rebuilds it and renders again)
+    if (a.counters && load_counter(a.counters + CNT_OVERFLOW) != 0) return;
     const uint32_t lane = lane_id();
     const uint32_t tiles_x = (a.width + 7) / 8;
     const uint32_t tile = blockIdx.x;
@@ -724,7 +731,7 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
         // Waterfall over the distinct tapes hit in this wavefront: each round
         // evaluates one tape, wave-uniform, for the lanes that hit it.
         V4 g{0, 0, 0, 0};
-        bool need = hit;
+        bool need = hit && toff < a.pool_words;
         for (;;) {
             const uint64_t m = __ballot(need);
             if (m == 0) break;

@@ -26,6 +26,9 @@ EXPORTED = [
     "frep_voxelize", "frep_voxelize_batch", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
     "frep_camera_look", "frep_render", "frep_eval_points", "frep_eval_intervals",
     "frep_eval_gradients", "frep_math",
+    "frep_server_create", "frep_server_destroy", "frep_server_last_error", "frep_server_scene",
+    "frep_server_submit", "frep_server_wait", "frep_server_poll", "frep_server_stats_get",
+    "frep_server_reset_stats",
 ]
 
 
@@ -64,6 +67,32 @@ class Timing(C.Structure):
                 ("init_ms", C.c_float), ("render_ms", C.c_float), ("batch_ms", C.c_float)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("device_budget_bytes", C.c_uint64), ("max_batch", C.c_uint32),
+                ("max_inflight", C.c_uint32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("pad", C.c_uint32), ("camera", Camera),
+                ("rgba", C.POINTER(C.c_uint8)), ("hit_ids", C.POINTER(C.c_uint64)),
+                ("normals", C.POINTER(C.c_float))]
+
+
+class Result(C.Structure):
+    _fields_ = [("status", C.c_int), ("voxelized", C.c_uint32), ("batch_size", C.c_uint32),
+                ("latency_ms", C.c_float)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("batches", C.c_uint64), ("voxelizations", C.c_uint64),
+                ("evictions", C.c_uint64), ("resident_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("resident_scenes", C.c_uint32),
+                ("queued", C.c_uint32), ("p50_ms", C.c_float), ("p99_ms", C.c_float),
+                ("mean_ms", C.c_float), ("max_ms", C.c_float), ("mean_batch", C.c_float),
+                ("elapsed_s", C.c_float)]
+
+
 _lib = None
 
 
@@ -100,6 +129,15 @@ def load():
                                           f32p, u8p]),
         "frep_eval_gradients": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint32, C.c_float, f32p]),
         "frep_math": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint32, f32p]),
+        "frep_server_create": (C.c_int, [vp, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
+        "frep_server_destroy": (C.c_int, [vp]),
+        "frep_server_last_error": (C.c_char_p, [vp]),
+        "frep_server_scene": (C.c_int, [vp, C.c_uint32, C.POINTER(Grid), u32p]),
+        "frep_server_submit": (C.c_int, [vp, C.POINTER(Request), u64p]),
+        "frep_server_wait": (C.c_int, [vp, C.c_uint64, C.POINTER(Result)]),
+        "frep_server_poll": (C.c_int, [vp, C.c_uint64, C.POINTER(Result), C.POINTER(C.c_int)]),
+        "frep_server_stats_get": (C.c_int, [vp, C.POINTER(ServerStats)]),
+        "frep_server_reset_stats": (C.c_int, [vp]),
     }
     for name, (res, args) in proto.items():
         fn = getattr(L, name)
@@ -321,3 +359,77 @@ def camera_look(cam: Camera, initial_up, fov_deg) -> Camera:
     if st != FREP_OK:
         raise FrepError(st, "frep_camera_look failed")
     return cam
+
+
+class Server:
+    """Request server on a context (frep_server_*): scenes stay resident,
+    requests that arrive together are batched, batches overlap on the GPU."""
+
+    def __init__(self, ctx: Context, budget_bytes=0, max_batch=0, max_inflight=0):
+        self.ctx = ctx
+        self.L = ctx.L
+        cfg = ServerConfig(budget_bytes, max_batch, max_inflight)
+        h = C.c_void_p()
+        st = self.L.frep_server_create(ctx.h, C.byref(cfg), C.byref(h))
+        if st != FREP_OK:
+            raise FrepError(st, ctx.L.frep_last_error(ctx.h).decode())
+        self.h = h
+        self._out = {}
+
+    def _check(self, st):
+        if st != FREP_OK:
+            raise FrepError(st, self.L.frep_server_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.frep_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def scene(self, tape: int, grid: dict) -> int:
+        g = make_grid(grid)
+        out = C.c_uint32()
+        self._check(self.L.frep_server_scene(self.h, tape, C.byref(g), C.byref(out)))
+        return out.value
+
+    def submit(self, scene: int, cam: Camera, want=("rgba",)) -> int:
+        spp = max(1, cam.spp)
+        npx = cam.width * cam.height
+        rgba = np.zeros((cam.height, cam.width, 4), dtype=np.uint8) if "rgba" in want else None
+        hits = np.zeros(npx * spp, dtype=np.uint64) if "hits" in want else None
+        nrm = np.zeros((npx * spp, 3), dtype=np.float32) if "normals" in want else None
+        req = Request()
+        req.scene = scene
+        req.camera = cam
+        req.rgba = _p(rgba, C.c_uint8) if rgba is not None else None
+        req.hit_ids = _p(hits, C.c_uint64) if hits is not None else None
+        req.normals = _p(nrm, C.c_float) if nrm is not None else None
+        t = C.c_uint64()
+        self._check(self.L.frep_server_submit(self.h, C.byref(req), C.byref(t)))
+        self._out[t.value] = (rgba, hits, nrm)
+        return t.value
+
+    def wait(self, ticket: int):
+        """Returns (Result, rgba, hits, normals) of a submitted request."""
+        r = Result()
+        self._check(self.L.frep_server_wait(self.h, ticket, C.byref(r)))
+        rgba, hits, nrm = self._out.pop(ticket)
+        return r, rgba, hits, nrm
+
+    def poll(self, ticket: int):
+        r, done = Result(), C.c_int()
+        self._check(self.L.frep_server_poll(self.h, ticket, C.byref(r), C.byref(done)))
+        if not done.value:
+            return None
+        rgba, hits, nrm = self._out.pop(ticket)
+        return r, rgba, hits, nrm
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        self._check(self.L.frep_server_stats_get(self.h, C.byref(s)))
+        return s
+
+    def reset_stats(self):
+        self._check(self.L.frep_server_reset_stats(self.h))

@@ -218,3 +218,38 @@ def camera_basis(pos, forward, initial_up, fov_deg, width, height, ortho=False,
         sw = (float(swx), float(swx * F(F(height) / F(width))))
     return {"pos": [f32(v) for v in pos], "forward": f, "right": [float(v) for v in r],
             "up": [float(v) for v in up], "screen_world": list(sw)}
+
+
+def orbit_basis(azimuth: float, elevation: float, radius: float = 3.0, width: int = 960,
+                height: int = 540, fov_deg: float = 45.0):
+    """Camera on a sphere of `radius` around the origin looking at it
+    (angles in radians, y up)."""
+    ce, se = math.cos(elevation), math.sin(elevation)
+    pos = (radius * ce * math.cos(azimuth), radius * se, radius * ce * math.sin(azimuth))
+    fwd = tuple(-c / radius for c in pos)
+    return camera_basis(pos, fwd, (0.0, 1.0, 0.0), fov_deg, width, height)
+
+
+def served_requests(n: int, seed: int = 7, rate: float = 100.0, p_rerender: float = 0.5,
+                    width: int = 960, height: int = 540):
+    """The served workload: Poisson arrivals at `rate` req/s (exponential
+    gaps).  Each request renders a random orbit view (radius 3, azimuth
+    U[0, 2pi), elevation U[-60, 60] degrees) of a scene; with probability
+    p_rerender the scene is one requested before (a resident tree), else a new
+    24-primitive random CSG (as random_csg, seed 10000 + index) at 1024^3.
+    Returns a list of (arrival_s, scene_index, azimuth, elevation, seed);
+    scene_index k names shape random_csg(24, 10000 + k)."""
+    rng = SplitMix64(seed)
+    out, t, n_scenes = [], 0.0, 0
+    for i in range(n):
+        u = max(rng.uniform(0.0, 1.0), 1e-7)
+        t += -math.log(u) / rate
+        if n_scenes and rng.uniform(0.0, 1.0) < p_rerender:
+            k = rng.randint(0, n_scenes - 1)
+        else:
+            k = n_scenes
+            n_scenes += 1
+        az = rng.uniform(0.0, 2.0 * math.pi)
+        el = rng.uniform(-math.pi / 3.0, math.pi / 3.0)
+        out.append((t, k, az, el, 1000 + i))
+    return out

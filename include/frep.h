@@ -172,6 +172,67 @@ frep_status frep_eval_gradients(frep_ctx* ctx, frep_tape_t tape, const float* xy
 /* fn: 0 sin, 1 cos, 2 exp -- the device implementations used by the tapes. */
 frep_status frep_math(frep_ctx* ctx, uint32_t fn, const float* x, uint32_t n, float* out);
 
+/* ---- serving: request loop with continuous batching ----
+ * The reference has no serving layer (its application voxelizes and renders
+ * one scene per frame, vulkan/src/application.cpp:60-93); these entry
+ * points have no counterpart there.  A server owns a host thread and its own
+ * streams on the context's device.  A scene is a (tape, grid) pair whose
+ * tree the server builds on first use and keeps resident; every request
+ * renders one scene with its own camera.  Requests that arrive while others
+ * run are batched (their trees built by the batched level kernels) and
+ * batches overlap on the device.  Scenes are built only while the resident
+ * trees fit the device budget; idle trees are evicted least recently used
+ * first and rebuilt on demand.  Each request's outputs are what
+ * frep_voxelize + frep_render give for its scene and camera.  The tapes of
+ * registered scenes must outlive the server. */
+typedef struct frep_server frep_server;
+typedef uint32_t frep_scene_t;
+
+typedef struct frep_server_config {
+    uint64_t device_budget_bytes;   /* trees + tape pools; 0 = 64 GiB */
+    uint32_t max_batch;             /* requests per batch, 1..64; 0 = 64 */
+    uint32_t max_inflight;          /* batches on the device at once, 1..4; 0 = 2 */
+} frep_server_config;
+
+typedef struct frep_request {
+    frep_scene_t scene;
+    uint32_t pad;
+    frep_camera camera;
+    uint8_t* rgba;                  /* host outputs as in frep_render (any may be */
+    uint64_t* hit_ids;              /* NULL); written before the request completes */
+    float* normals;
+} frep_request;
+
+typedef struct frep_result {
+    frep_status status;
+    uint32_t voxelized;             /* 1: the request's batch (re)built the scene's tree */
+    uint32_t batch_size;            /* requests in its batch */
+    float latency_ms;               /* submit -> outputs written */
+} frep_result;
+
+typedef struct frep_server_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t batches, voxelizations, evictions;
+    uint64_t resident_bytes, budget_bytes;
+    uint32_t resident_scenes, queued;
+    float p50_ms, p99_ms, mean_ms, max_ms;   /* latency of completed requests */
+    float mean_batch;                        /* requests per batch */
+    float elapsed_s;                         /* first submit -> last completion */
+} frep_server_stats;
+
+frep_status frep_server_create(frep_ctx* ctx, const frep_server_config* cfg, frep_server** out);
+frep_status frep_server_destroy(frep_server* srv);
+const char* frep_server_last_error(const frep_server* srv);
+frep_status frep_server_scene(frep_server* srv, frep_tape_t tape, const frep_grid* grid,
+                              frep_scene_t* out);
+frep_status frep_server_submit(frep_server* srv, const frep_request* req, uint64_t* ticket);
+/* Blocks until the request completes; the ticket is released. */
+frep_status frep_server_wait(frep_server* srv, uint64_t ticket, frep_result* out);
+/* *done = 1 (and the ticket released, *out filled) if the request completed. */
+frep_status frep_server_poll(frep_server* srv, uint64_t ticket, frep_result* out, int* done);
+frep_status frep_server_stats_get(frep_server* srv, frep_server_stats* out);
+frep_status frep_server_reset_stats(frep_server* srv);
+
 #ifdef __cplusplus
 }
 #endif

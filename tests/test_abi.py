@@ -36,6 +36,10 @@ def test_struct_layouts_match_header_sizes():
     assert C.sizeof(lib.Camera) == 4 * (3 * 4 + 2 + 6 + 4 + 1 + 16 + 16 + 4)
     assert C.sizeof(lib.TreeInfo) == 4 * (1 + 8 * 4) + 4 + 16 + 8
     assert C.sizeof(lib.Timing) == 4 * (1 + 8 + 3)
+    assert C.sizeof(lib.ServerConfig) == 16
+    assert C.sizeof(lib.Request) == 8 + C.sizeof(lib.Camera) + 4 + 3 * 8
+    assert C.sizeof(lib.Result) == 16
+    assert C.sizeof(lib.ServerStats) == 8 * 8 + 4 * 2 + 4 * 6
 
 
 def test_camera_look_host_helper():

@@ -298,6 +298,82 @@ def test_voxelize_batch_rejects_duplicate_tree(ctx):
     ctx.tree_free(tr)
 
 
+def _plain_outputs(ctx, tape, grid, cam):
+    th = ctx.upload(tape)
+    tr = ctx.voxelize(th, grid)
+    out = ctx.render(tr, cam)
+    ctx.tree_free(tr)
+    return out
+
+
+def _served_case(n_req=36, size=(96, 64)):
+    scenes = [
+        (TAPES["spheres8"](), shapes.make_grid([8, 8, 4])),
+        (TAPES["csg24"](), shapes.make_grid([16, 8, 8])),
+        (TAPES["trig"](), shapes.make_grid([8, 8, 4], tape_time=0.5)),
+        (shapes.random_csg(24, 3), shapes.make_grid([16, 8, 8])),
+        (shapes.random_csg(24, 4), shapes.make_grid([8, 8, 8])),
+        (shapes.spheres_union(30, 21), shapes.make_grid([8, 4, 4])),
+    ]
+    rng = np.random.default_rng(5)
+    reqs = []
+    for i in range(n_req):
+        k = int(rng.integers(0, len(scenes)))
+        b = shapes.orbit_basis(float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(-1.0, 1.0)),
+                               3.0, size[0], size[1])
+        cam = lib.make_camera(b, size[0], size[1], spp=1 + (i % 2), jitter=True, seed=100 + i)
+        reqs.append((k, cam))
+    return scenes, reqs
+
+
+def _check_served(ctx, srv, scenes, reqs, handles=None):
+    if handles is None:
+        handles = [srv.scene(ctx.upload(t), g) for t, g in scenes]
+    tickets = [srv.submit(handles[k], cam, want=("rgba", "hits", "normals")) for k, cam in reqs]
+    results = [srv.wait(t) for t in tickets]
+    for (k, cam), (res, rgba, hits, nrm) in zip(reqs, results):
+        assert res.status == lib.FREP_OK
+        prgba, phits, pnrm = _plain_outputs(ctx, scenes[k][0], scenes[k][1], cam)
+        assert np.array_equal(hits, phits)
+        assert same_bits(nrm, pnrm)
+        assert np.array_equal(rgba, prgba)
+    return results, handles
+
+
+def test_server_results_equal_plain_path(ctx):
+    """Requests submitted together are batched (trees built by the batched
+    kernels, renders on the server's streams); every output equals what
+    frep_voxelize + frep_render give for the same scene and camera."""
+    scenes, reqs = _served_case()
+    srv = lib.Server(ctx, max_batch=16, max_inflight=2)
+    try:
+        results, handles = _check_served(ctx, srv, scenes, reqs)
+        st = srv.stats()
+        assert st.completed == len(reqs) and st.failed == 0
+        assert max(r[0].batch_size for r in results) > 1
+        assert st.voxelizations == len(scenes)       # each scene built once, then resident
+        assert st.p99_ms >= st.p50_ms > 0
+        # a second round renders resident trees only
+        srv.reset_stats()
+        _check_served(ctx, srv, scenes, reqs[:12], handles)
+        assert srv.stats().voxelizations == 0
+    finally:
+        srv.close()
+
+
+def test_server_evicts_under_a_small_budget(ctx):
+    """A budget that holds about two trees: scenes are evicted and rebuilt,
+    and every output is still the plain path's."""
+    scenes, reqs = _served_case(n_req=24, size=(64, 48))
+    srv = lib.Server(ctx, budget_bytes=64 << 20, max_batch=4, max_inflight=2)
+    try:
+        _check_served(ctx, srv, scenes, reqs)
+        st = srv.stats()
+        assert st.failed == 0 and st.evictions > 0
+    finally:
+        srv.close()
+
+
 def test_asm_and_cpp_interpreters_agree(oracle):
     """The hand-scheduled point interpreter (8 points per lane) and the C++
     interpreter give identical trees and point values (FREP_NO_ASM=1 forces

@@ -15,7 +15,9 @@ restated in oracle/) timed on a bounded sample on this host.
 from __future__ import annotations
 
 import argparse
+from concurrent.futures import ThreadPoolExecutor
 import json
+import math
 import os
 import sys
 import time
@@ -98,6 +100,105 @@ def bench_batched(ctx, shapes, reps=5, warm=6):
     return out
 
 
+def bench_served(ctx, lib, shapes, rates=(50, 100, 200, 500), seconds=2.0, size=(960, 540),
+                 burst=512):
+    """Served workload: Poisson arrivals of voxelize+render requests (new
+    24-primitive random CSG scenes at 1024^3, or with p = 0.5 a re-render of
+    a resident one, 960x540 orbit views).  For each arrival rate the stream
+    is replayed in real time through frep_server (continuous batching), and
+    the same stream through the plain path (frep_voxelize + frep_render per
+    request, one at a time); the plain path's latencies follow from its
+    measured service times and the same arrival times (FIFO).  The closed
+    loop submits `burst` requests at once to each path."""
+    grid = shapes.make_grid([16, 8, 8])
+    tapes = {}
+
+    def tape(k):
+        if k not in tapes:
+            tapes[k] = ctx.upload(shapes.random_csg(24, 10000 + k))
+        return tapes[k]
+
+    def camera(az, el, seed):
+        b = shapes.orbit_basis(az, el, 3.0, size[0], size[1])
+        return lib.make_camera(b, size[0], size[1], seed=seed)
+
+    def pct(v, q):
+        v = sorted(v)
+        return v[min(len(v) - 1, max(0, int(math.ceil(q * len(v))) - 1))]
+
+    def served_run(reqs, realtime):
+        srv = lib.Server(ctx, budget_bytes=64 << 30)
+        handles = {}
+        for _, k, _, _, _ in reqs:
+            if k not in handles:
+                handles[k] = srv.scene(tape(k), grid)
+        cams = [camera(az, el, sd) for _, _, az, el, sd in reqs]
+        # clients: this thread submits on the arrival schedule, a pool of
+        # threads collects results (each wait copies one image out)
+        pool = ThreadPoolExecutor(max_workers=8)
+        futures = []
+        t0 = time.perf_counter()
+        for (arr, k, _, _, _), cam in zip(reqs, cams):
+            if realtime:
+                d = t0 + arr - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+            futures.append(pool.submit(srv.wait, srv.submit(handles[k], cam)))
+        for f in futures:
+            f.result()
+        wall = time.perf_counter() - t0
+        pool.shutdown()
+        st = srv.stats()
+        res = {"req_per_s": round(st.completed / wall, 2), "p50_ms": round(st.p50_ms, 3),
+               "p99_ms": round(st.p99_ms, 3), "mean_batch": round(st.mean_batch, 2),
+               "builds": int(st.voxelizations), "failed": int(st.failed)}
+        srv.close()
+        return res
+
+    def plain_run(reqs):
+        trees = {}
+        service = []
+        for _, k, az, el, sd in reqs:
+            cam = camera(az, el, sd)
+            t0 = time.perf_counter()
+            if k not in trees:
+                trees[k] = ctx.voxelize(tape(k), grid)
+            ctx.render(trees[k], cam, want=("rgba",))
+            service.append(time.perf_counter() - t0)
+        for tr in trees.values():
+            ctx.tree_free(tr)
+        finish, lat = 0.0, []
+        for (arr, _, _, _, _), sv in zip(reqs, service):
+            finish = max(arr, finish) + sv
+            lat.append((finish - arr) * 1e3)
+        return {"req_per_s": round(len(reqs) / sum(service), 2), "p50_ms": round(pct(lat, 0.5), 3),
+                "p99_ms": round(pct(lat, 0.99), 3)}
+
+    # warm both paths (kernels, allocator, capacity hints)
+    warm = shapes.served_requests(64, seed=3, rate=1e6)
+    served_run(warm, False)
+    plain_run(warm[:16])
+    per_rate = []
+    for rate in rates:
+        reqs = shapes.served_requests(int(rate * seconds), seed=7, rate=rate)
+        per_rate.append({"rate": rate, "requests": len(reqs), "served": served_run(reqs, True),
+                         "plain": plain_run(reqs)})
+    reqs = shapes.served_requests(burst, seed=11, rate=1e6)
+    served = served_run(reqs, False)
+    plain = plain_run(reqs)
+    for th in tapes.values():
+        ctx.tape_free(th)
+    return {
+        "workload": "voxelize (24-primitive random CSG, 1024^3, branching 16/8/8) + render "
+                    "960x540 orbit view per request; p = 0.5 re-render of a resident scene",
+        "per_rate": per_rate,
+        "closed_loop": {"requests": burst, "served_req_per_s": served["req_per_s"],
+                        "plain_req_per_s": plain["req_per_s"],
+                        "speedup": round(served["req_per_s"] / plain["req_per_s"], 3),
+                        "served_p99_ms": served["p99_ms"], "mean_batch": served["mean_batch"]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +208,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-render", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -198,6 +300,8 @@ def main():
                          "frames_per_s_incl_copy": round(1.0 / rwall, 2)}
     if not args.no_batched:
         out["batched"] = bench_batched(ctx, shapes)
+    if not args.no_served:
+        out["served"] = bench_served(ctx, lib, shapes)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(tape, grid)
     ctx.close()

@@ -92,36 +92,42 @@ frep_status check_grid(frep_ctx* ctx, const frep_grid* g) {
     return FREP_OK;
 }
 
-void free_tree_buffers(TreeObj* t) {
+// Tree buffers come from the context's device pool; the callers have
+// synchronized with every launch that used them.
+void free_tree_buffers(frep_ctx* ctx, TreeObj* t) {
     for (uint32_t L = 0; L < kMaxLevels; ++L) {
-        if (t->d_nodes[L]) (void)hipFree(t->d_nodes[L]);
+        ctx->dpool.free(t->d_nodes[L]);
         t->d_nodes[L] = nullptr;
         t->node_cap[L] = 0;
     }
-    if (t->d_pool) (void)hipFree(t->d_pool);
+    ctx->dpool.free(t->d_pool);
     t->d_pool = nullptr;
     t->pool_cap = 0;
-    if (t->d_consts) (void)hipFree(t->d_consts);
+    ctx->dpool.free(t->d_consts);
     t->d_consts = nullptr;
     t->consts_cap = 0;
 }
 
 frep_status ensure_level(frep_ctx* ctx, TreeObj* t, uint32_t L, uint32_t cap) {
     if (t->d_nodes[L] && t->node_cap[L] >= cap) return FREP_OK;
-    if (t->d_nodes[L]) HIP_TRY(hipFree(t->d_nodes[L]));
+    ctx->dpool.free(t->d_nodes[L]);
     t->d_nodes[L] = nullptr;
     t->node_cap[L] = 0;
-    HIP_TRY(hipMalloc(&t->d_nodes[L], (size_t)cap * t->ns[L] * sizeof(uint32_t)));
+    void* p = nullptr;
+    HIP_TRY(ctx->dpool.alloc((size_t)cap * t->ns[L] * sizeof(uint32_t), &p));
+    t->d_nodes[L] = static_cast<uint32_t*>(p);
     t->node_cap[L] = cap;
     return FREP_OK;
 }
 
 frep_status ensure_pool(frep_ctx* ctx, TreeObj* t, uint64_t words) {
     if (t->d_pool && t->pool_cap >= words) return FREP_OK;
-    if (t->d_pool) HIP_TRY(hipFree(t->d_pool));
+    ctx->dpool.free(t->d_pool);
     t->d_pool = nullptr;
     t->pool_cap = 0;
-    HIP_TRY(hipMalloc(&t->d_pool, (size_t)(words + kPoolPad) * sizeof(uint32_t)));
+    void* p = nullptr;
+    HIP_TRY(ctx->dpool.alloc((size_t)(words + kPoolPad) * sizeof(uint32_t), &p));
+    t->d_pool = static_cast<uint32_t*>(p);
     t->pool_cap = words;
     return FREP_OK;
 }
@@ -160,8 +166,12 @@ uint64_t tree_device_bytes(const TreeObj* t) {
 
 frep_status new_tree(frep_ctx* ctx, frep_tree_t* out) {
     std::unique_ptr<TreeObj> t(new TreeObj());
-    HIP_TRY(hipMalloc(&t->d_counters, CNT_WORDS * 4));
-    HIP_TRY(hipHostMalloc(&t->h_counters, CNT_WORDS * 4, hipHostMallocDefault));
+    void* d = nullptr;
+    void* h = nullptr;
+    HIP_TRY(ctx->dpool.alloc(CNT_WORDS * 4, &d));
+    HIP_TRY(ctx->hpool.alloc(CNT_WORDS * 4, &h));
+    t->d_counters = static_cast<uint32_t*>(d);
+    t->h_counters = static_cast<uint32_t*>(h);
     std::memset(t->h_counters, 0, CNT_WORDS * 4);
     ctx->trees.push_back(std::move(t));
     *out = (frep_tree_t)ctx->trees.size();
@@ -208,9 +218,9 @@ void frep_shutdown(frep_ctx* ctx) {
     }
     for (auto& t : ctx->trees) {
         if (!t) continue;
-        free_tree_buffers(t.get());
-        if (t->d_counters) (void)hipFree(t->d_counters);
-        if (t->h_counters) (void)hipHostFree(t->h_counters);
+        free_tree_buffers(ctx, t.get());
+        ctx->dpool.free(t->d_counters);
+        ctx->hpool.free(t->h_counters);
     }
     if (ctx->d_out) (void)hipFree(ctx->d_out);
     free_stage(&ctx->stage);
@@ -304,9 +314,9 @@ frep_status frep_tree_free(frep_ctx* ctx, frep_tree_t h) {
     TreeObj* t = get_tree(ctx, h);
     if (!t) return fail(ctx, FREP_INVALID_ARG, "unknown tree handle");
     HIP_TRY(hipStreamSynchronize(ctx->stream));
-    free_tree_buffers(t);
-    if (t->d_counters) HIP_TRY(hipFree(t->d_counters));
-    if (t->h_counters) HIP_TRY(hipHostFree(t->h_counters));
+    free_tree_buffers(ctx, t);
+    ctx->dpool.free(t->d_counters);
+    ctx->hpool.free(t->h_counters);
     ctx->trees[h - 1].reset();
     return FREP_OK;
 }
@@ -354,7 +364,7 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
     if (!same_shape) {
         // record sizes change with the dims: drop the level buffers
         for (uint32_t L = 0; L < kMaxLevels; ++L) {
-            if (t->d_nodes[L]) HIP_TRY(hipFree(t->d_nodes[L]));
+            ctx->dpool.free(t->d_nodes[L]);
             t->d_nodes[L] = nullptr;
             t->node_cap[L] = 0;
         }
@@ -374,9 +384,11 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
     initial_caps(tp, grid, t, hint, j->want, &j->pool_want);
     const uint32_t nc = (uint32_t)tp->consts.size();
     if (!t->d_consts || t->consts_cap < nc + kPoolPad) {
-        if (t->d_consts) HIP_TRY(hipFree(t->d_consts));
+        ctx->dpool.free(t->d_consts);
         t->d_consts = nullptr;
-        HIP_TRY(hipMalloc(&t->d_consts, (size_t)(nc + kPoolPad) * 4));
+        void* p = nullptr;
+        HIP_TRY(ctx->dpool.alloc((size_t)(nc + kPoolPad) * 4, &p));
+        t->d_consts = static_cast<float*>(p);
         t->consts_cap = nc + kPoolPad;
     }
     if (nc)

@@ -47,6 +47,38 @@ struct TreeObj {
     uint32_t retries = 0;
 };
 
+// Caching sub-allocator (frep_pool.hip): device or pinned host blocks carved
+// from large slabs and recycled by size class.  Thread-safe.  A block may be
+// freed only after the work using it has completed.
+class BlockPool {
+  public:
+    explicit BlockPool(bool pinned);
+    ~BlockPool();
+    BlockPool(const BlockPool&) = delete;
+    BlockPool& operator=(const BlockPool&) = delete;
+    hipError_t alloc(size_t bytes, void** out);
+    void free(void* p);
+    void release_all();
+    size_t reserved_bytes();   // slabs + blocks outside the slabs
+
+  private:
+    static constexpr uint32_t kDirectClass = 0xFFFFFFFFu;
+    struct Slab {
+        char* base;
+        size_t size, used;
+    };
+    struct Live {
+        uint32_t cls;
+        size_t size;
+    };
+    bool pinned_;
+    std::mutex mu_;
+    std::vector<Slab> slabs_;
+    std::unordered_map<void*, Live> live_;
+    std::vector<std::vector<void*>> free_;
+    size_t bytes_ = 0;
+};
+
 // Autotuner state of one batch signature.
 struct TuneRecord {
     uint32_t trial = 0;                // candidates timed so far
@@ -122,6 +154,9 @@ struct frep_ctx {
     size_t out_bytes = 0;
     // staging of the synchronous batched voxelization
     frep::BatchStage stage;
+    // tree buffers (device) and small pinned mirrors / request outputs
+    frep::BlockPool dpool{false};
+    frep::BlockPool hpool{true};
     // autotuner records and graphs of batches, keyed by their argument tables
     bool autotune = true;              // FREP_NO_AUTOTUNE=1 turns it off
     std::unordered_map<uint64_t, frep::TuneRecord> tune;
@@ -146,7 +181,7 @@ frep_status fail(frep_ctx* c, frep_status s, const std::string& msg);
 TapeObj* get_tape(frep_ctx* ctx, frep_tape_t h);
 TreeObj* get_tree(frep_ctx* ctx, frep_tree_t h);
 frep_status check_grid(frep_ctx* ctx, const frep_grid* g);
-void free_tree_buffers(TreeObj* t);
+void free_tree_buffers(frep_ctx* ctx, TreeObj* t);
 uint64_t host_alloc(const TreeObj* t, uint32_t L);
 uint32_t host_nodes(const TreeObj* t, uint32_t L);
 uint64_t host_pool_top(const TreeObj* t);

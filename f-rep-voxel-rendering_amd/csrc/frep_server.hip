@@ -60,18 +60,44 @@ struct Scene {
     uint64_t last_use = 0;
 };
 
+// Output layout of one request: rgba, hit ids, normals at 256-byte aligned
+// offsets of one block (the same layout in the slot's device buffer, at
+// `dbase`, and in the request's pinned block).
+struct OutLayout {
+    size_t o_rgba = 0, o_hit = 0, o_nrm = 0, total = 0;
+    size_t b_rgba = 0, b_hit = 0, b_nrm = 0;
+};
+
 struct Ticket {
     frep_request req{};
     Clock::time_point submitted;
     bool done = false;
     frep_result result{};
+    OutLayout out;
+    void* h_block = nullptr;             // pinned outputs, copied out by the waiter
 };
 
+// A request as its batch carries it (copied from the ticket and the scene
+// at admission, so the launch needs no lock on the server's tables).
 struct SlotReq {
     uint64_t ticket;
     frep_scene_t scene;
-    size_t off_rgba, off_hit, off_nrm;   // offsets in the slot's output buffers
-    size_t b_rgba, b_hit, b_nrm;
+    frep_tree_t tree;
+    frep_camera cam;
+    size_t dbase;                        // offset in the slot's device buffer
+    OutLayout out;
+    void* h_block;                       // pinned outputs of the request
+};
+
+// A tree the batch builds.
+struct SlotJob {
+    frep_scene_t scene;
+    frep_tape_t tape;
+    frep_tree_t tree;
+    frep_grid grid;
+    bool has_hint;
+    CapHint hint;
+    uint64_t est;                        // budget charged while building
 };
 
 struct Slot {
@@ -79,12 +105,11 @@ struct Slot {
     hipEvent_t done = nullptr;
     bool busy = false;
     BatchStage stage;
-    std::vector<Job> jobs;               // trees this batch builds
-    std::vector<frep_scene_t> job_scene;
-    std::vector<uint64_t> job_est;       // budget charged while building
+    std::vector<SlotJob> build;          // trees this batch builds
+    std::vector<Job> jobs;               // their voxelization jobs (same order)
     std::vector<SlotReq> reqs;
+    size_t out_need = 0;                 // device output bytes of the batch
     void* d_out = nullptr;
-    void* h_out = nullptr;               // pinned
     size_t out_bytes = 0;
     uint64_t seq = 0;                    // launch order
 };
@@ -147,7 +172,7 @@ bool evict_one(frep_server* srv) {
         std::lock_guard<std::mutex> lk(ctx->mu);
         TreeObj* t = get_tree(ctx, victim->tree);
         if (t) {
-            free_tree_buffers(t);
+            free_tree_buffers(ctx, t);
             t->built = false;
         }
     }
@@ -179,87 +204,96 @@ void complete(frep_server* srv, uint64_t ticket, frep_status st, uint32_t voxeli
     srv->last_done = now;
 }
 
-// Output bytes of a request and their offsets (256-byte aligned).
-size_t layout_outputs(const frep_request& r, size_t base, SlotReq* sr) {
+OutLayout layout_outputs(const frep_request& r) {
     const frep_camera& c = r.camera;
     const uint32_t spp = c.spp ? c.spp : 1;
     const size_t px = (size_t)c.width * c.height;
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    sr->b_rgba = r.rgba ? px * 4 : 0;
-    sr->b_hit = r.hit_ids ? px * spp * 8 : 0;
-    sr->b_nrm = r.normals ? px * spp * 12 : 0;
-    sr->off_rgba = al(base);
-    sr->off_hit = al(sr->off_rgba + sr->b_rgba);
-    sr->off_nrm = al(sr->off_hit + sr->b_hit);
-    return al(sr->off_nrm + sr->b_nrm);
+    OutLayout o;
+    o.b_rgba = r.rgba ? px * 4 : 0;
+    o.b_hit = r.hit_ids ? px * spp * 8 : 0;
+    o.b_nrm = r.normals ? px * spp * 12 : 0;
+    o.o_rgba = 0;
+    o.o_hit = al(o.b_rgba);
+    o.o_nrm = al(o.o_hit + o.b_hit);
+    o.total = al(o.o_nrm + o.b_nrm);
+    return o;
 }
 
-// Enqueues the renders of the slot's requests (for those whose scene is in
-// `only`, or all when only == 0) and their copies to the pinned outputs.
+// Enqueues the renders of the slot's requests (only those of scene `only`
+// when it is not 0) and the copies of their outputs to the pinned blocks.
 frep_status enqueue_renders(frep_server* srv, Slot& S, frep_scene_t only) {
     frep_ctx* ctx = srv->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    char* d = static_cast<char*>(S.d_out);
-    char* h = static_cast<char*>(S.h_out);
     for (const SlotReq& r : S.reqs) {
         if (only && r.scene != only) continue;
-        const Ticket& tk = srv->tickets[r.ticket];
-        const Scene& sc = srv->scenes[r.scene - 1];
-        const TreeObj* t = get_tree(ctx, sc.tree);
+        const TreeObj* t = get_tree(ctx, r.tree);
+        char* d = static_cast<char*>(S.d_out) + r.dbase;
+        char* h = static_cast<char*>(r.h_block);
+        const OutLayout& o = r.out;
         const RenderArgs a = render_args(
-            t, &tk.req.camera, r.b_rgba ? (uint8_t*)(d + r.off_rgba) : nullptr,
-            r.b_hit ? (unsigned long long*)(d + r.off_hit) : nullptr,
-            r.b_nrm ? (float*)(d + r.off_nrm) : nullptr);
+            t, &r.cam, o.b_rgba ? (uint8_t*)(d + o.o_rgba) : nullptr,
+            o.b_hit ? (unsigned long long*)(d + o.o_hit) : nullptr,
+            o.b_nrm ? (float*)(d + o.o_nrm) : nullptr);
         HIP_TRY(launch_render(S.stream, a, t->maxs));
-        if (r.b_rgba)
-            HIP_TRY(hipMemcpyAsync(h + r.off_rgba, d + r.off_rgba, r.b_rgba, hipMemcpyDeviceToHost,
+        if (o.b_rgba)
+            HIP_TRY(hipMemcpyAsync(h + o.o_rgba, d + o.o_rgba, o.b_rgba, hipMemcpyDeviceToHost,
                                    S.stream));
-        if (r.b_hit)
-            HIP_TRY(hipMemcpyAsync(h + r.off_hit, d + r.off_hit, r.b_hit, hipMemcpyDeviceToHost,
+        if (o.b_hit)
+            HIP_TRY(hipMemcpyAsync(h + o.o_hit, d + o.o_hit, o.b_hit, hipMemcpyDeviceToHost,
                                    S.stream));
-        if (r.b_nrm)
-            HIP_TRY(hipMemcpyAsync(h + r.off_nrm, d + r.off_nrm, r.b_nrm, hipMemcpyDeviceToHost,
+        if (o.b_nrm)
+            HIP_TRY(hipMemcpyAsync(h + o.o_nrm, d + o.o_nrm, o.b_nrm, hipMemcpyDeviceToHost,
                                    S.stream));
     }
     return FREP_OK;
 }
 
-// Forms a batch from the queue and enqueues it on slot S.  Caller holds
-// srv->mu.  Returns FREP_OK with S.busy == false when nothing was admitted.
-frep_status launch_batch(frep_server* srv, Slot& S) {
+// Admission: moves requests from the queue into slot S (FIFO, skipping those
+// whose scene a batch in flight is building, and those whose scene does not
+// fit the budget yet).  Caller holds srv->mu.  Returns the number admitted.
+size_t admit(frep_server* srv, Slot& S) {
     frep_ctx* ctx = srv->ctx;
+    S.build.clear();
     S.jobs.clear();
-    S.job_scene.clear();
-    S.job_est.clear();
     S.reqs.clear();
-    std::vector<frep_scene_t> building;   // scenes this batch builds
-    size_t out = 0;
+    S.out_need = 0;
+    auto ours = [&](frep_scene_t sid) {
+        for (const SlotJob& j : S.build)
+            if (j.scene == sid) return true;
+        return false;
+    };
     for (auto it = srv->queue.begin(); it != srv->queue.end();) {
         if (S.reqs.size() >= srv->cfg.max_batch) break;
         const uint64_t tid = *it;
         Ticket& tk = srv->tickets[tid];
         const frep_scene_t sid = tk.req.scene;
         Scene& sc = srv->scenes[sid - 1];
-        const bool ours = std::find(building.begin(), building.end(), sid) != building.end();
-        if (sc.building && !ours) {   // being built by a batch in flight
+        const bool mine = ours(sid);
+        if (sc.building && !mine) {   // being built by a batch in flight
             ++it;
             continue;
         }
-        if (!sc.resident && !ours) {
-            uint64_t est;
+        if (!sc.resident && !mine) {
+            SlotJob j{};
+            j.scene = sid;
+            j.tape = sc.tape;
+            j.grid = sc.grid;
+            auto h = srv->hints.find(dims_key(sc.grid));
+            j.has_hint = h != srv->hints.end();
+            if (j.has_hint) j.hint = h->second;
             {
                 std::lock_guard<std::mutex> lk(ctx->mu);
-                const TapeObj* tp = get_tape(ctx, sc.tape);
-                auto h = srv->hints.find(dims_key(sc.grid));
-                est = estimate_job_bytes(tp, &sc.grid, h == srv->hints.end() ? nullptr : &h->second);
+                j.est = estimate_job_bytes(get_tape(ctx, sc.tape), &sc.grid,
+                                           j.has_hint ? &j.hint : nullptr);
             }
-            while (srv->resident_bytes + est > srv->cfg.device_budget_bytes && evict_one(srv)) {
+            while (srv->resident_bytes + j.est > srv->cfg.device_budget_bytes && evict_one(srv)) {
             }
-            if (srv->resident_bytes + est > srv->cfg.device_budget_bytes) {
+            if (srv->resident_bytes + j.est > srv->cfg.device_budget_bytes) {
                 // does not fit now; if nothing is in flight it never will
                 bool any_busy = false;
                 for (uint32_t k = 0; k < srv->n_slots; ++k) any_busy = any_busy || srv->slots[k].busy;
-                if (!any_busy && S.reqs.empty() && building.empty()) {
+                if (!any_busy && S.reqs.empty()) {
                     complete(srv, tid, FREP_OUT_OF_MEMORY, 0, 0);
                     srv->err = "scene does not fit the device budget";
                     it = srv->queue.erase(it);
@@ -269,49 +303,79 @@ frep_status launch_batch(frep_server* srv, Slot& S) {
                 ++it;
                 continue;
             }
-            srv->resident_bytes += est;
+            if (!sc.tree) {
+                std::lock_guard<std::mutex> lk(ctx->mu);
+                if (new_tree(ctx, &sc.tree) != FREP_OK) {
+                    complete(srv, tid, FREP_OUT_OF_MEMORY, 0, 0);
+                    it = srv->queue.erase(it);
+                    srv->done_cv.notify_all();
+                    continue;
+                }
+            }
+            j.tree = sc.tree;
+            srv->resident_bytes += j.est;
             sc.building = true;
-            building.push_back(sid);
-            S.job_scene.push_back(sid);
-            S.job_est.push_back(est);
+            S.build.push_back(j);
         }
         sc.pins += 1;
         sc.last_use = ++srv->use_clock;
         SlotReq r{};
         r.ticket = tid;
         r.scene = sid;
-        out = layout_outputs(tk.req, out, &r);
+        r.tree = sc.tree;
+        r.cam = tk.req.camera;
+        r.out = layout_outputs(tk.req);
+        r.dbase = S.out_need;
+        r.h_block = nullptr;
+        if (r.out.total && ctx->hpool.alloc(r.out.total, &r.h_block) != hipSuccess) {
+            // cannot take outputs: fail this request
+            sc.pins -= 1;
+            complete(srv, tid, FREP_OUT_OF_MEMORY, 0, 0);
+            it = srv->queue.erase(it);
+            srv->done_cv.notify_all();
+            continue;
+        }
+        tk.out = r.out;
+        tk.h_block = r.h_block;
+        S.out_need += r.out.total;
         S.reqs.push_back(r);
         it = srv->queue.erase(it);
     }
-    if (S.reqs.empty()) return FREP_OK;
-    // device and pinned output buffers of the batch
-    if (S.out_bytes < out) {
+    if (S.reqs.empty()) {
+        // scenes reserved for builds with no request left cannot happen: every
+        // build is pushed together with its first request
+        return 0;
+    }
+    S.busy = true;          // reserved; the work is enqueued without srv->mu
+    S.seq = ++srv->launch_seq;
+    ++srv->batches;
+    srv->batched_reqs += S.reqs.size();
+    srv->voxelizations += S.build.size();
+    return S.reqs.size();
+}
+
+// Enqueues the admitted batch of slot S: one batched voxelization of the
+// trees to build, the renders, and the completion event.  Runs without
+// srv->mu (it touches only the slot and ctx, under ctx->mu).
+frep_status enqueue_slot(frep_server* srv, Slot& S) {
+    frep_ctx* ctx = srv->ctx;
+    if (S.out_bytes < S.out_need) {
         if (S.d_out) HIP_TRY(hipFree(S.d_out));
-        if (S.h_out) HIP_TRY(hipHostFree(S.h_out));
         S.d_out = nullptr;
-        S.h_out = nullptr;
         S.out_bytes = 0;
-        const size_t cap = out + out / 2 + 4096;
+        const size_t cap = S.out_need + S.out_need / 2 + 4096;
         HIP_TRY(hipMalloc(&S.d_out, cap));
-        HIP_TRY(hipHostMalloc(&S.h_out, cap, hipHostMallocDefault));
         S.out_bytes = cap;
     }
-    // trees to build: one batched voxelization
-    if (!S.job_scene.empty()) {
+    if (!S.build.empty()) {
         std::lock_guard<std::mutex> lk(ctx->mu);
-        S.jobs.resize(S.job_scene.size());
+        S.jobs.assign(S.build.size(), Job{});
         std::vector<uint32_t> idx(S.jobs.size());
         for (size_t k = 0; k < S.jobs.size(); ++k) {
-            Scene& sc = srv->scenes[S.job_scene[k] - 1];
-            if (!sc.tree) {
-                const frep_status s = new_tree(ctx, &sc.tree);
-                if (s != FREP_OK) return s;
-            }
-            auto h = srv->hints.find(dims_key(sc.grid));
+            const SlotJob& j = S.build[k];
             const frep_status s =
-                prepare_job(ctx, get_tape(ctx, sc.tape), get_tree(ctx, sc.tree), &sc.grid, &S.jobs[k],
-                            S.stream, h == srv->hints.end() ? nullptr : &h->second);
+                prepare_job(ctx, get_tape(ctx, j.tape), get_tree(ctx, j.tree), &j.grid, &S.jobs[k],
+                            S.stream, j.has_hint ? &j.hint : nullptr);
             if (s != FREP_OK) return s;
             const frep_status e = ensure_job(ctx, &S.jobs[k]);
             if (e != FREP_OK) return e;
@@ -330,11 +394,6 @@ frep_status launch_batch(frep_server* srv, Slot& S) {
         if (s != FREP_OK) return s;
     }
     HIP_TRY(hipEventRecord(S.done, S.stream));
-    S.busy = true;
-    S.seq = ++srv->launch_seq;
-    ++srv->batches;
-    srv->batched_reqs += S.reqs.size();
-    srv->voxelizations += S.jobs.size();
     return FREP_OK;
 }
 
@@ -372,10 +431,16 @@ frep_status rebuild(frep_server* srv, Slot& S, size_t k) {
         S.jobs[k] = one[0];
         if (st != FREP_OK) return st;
     }
-    const frep_status s = enqueue_renders(srv, S, S.job_scene[k]);
+    const frep_status s = enqueue_renders(srv, S, S.build[k].scene);
     if (s != FREP_OK) return s;
     HIP_TRY(hipStreamSynchronize(S.stream));
     return FREP_OK;
+}
+
+bool builds(const Slot& S, frep_scene_t sid) {
+    for (const SlotJob& j : S.build)
+        if (j.scene == sid) return true;
+    return false;
 }
 
 // Completes the batch on slot S (its event has fired).  Caller holds srv->mu.
@@ -383,17 +448,18 @@ void finalize(frep_server* srv, Slot& S) {
     frep_ctx* ctx = srv->ctx;
     std::vector<frep_scene_t> failed_scenes;
     for (size_t k = 0; k < S.jobs.size(); ++k) {
-        Scene& sc = srv->scenes[S.job_scene[k] - 1];
+        const SlotJob& j = S.build[k];
+        Scene& sc = srv->scenes[j.scene - 1];
         TreeObj* t = S.jobs[k].t;
         std::memcpy(t->h_counters, S.stage.h_cnt + k * CNT_WORDS, CNT_WORDS * 4);
         frep_status st = FREP_OK;
         if (t->h_counters[CNT_OVERFLOW] != 0) st = rebuild(srv, S, k);
-        srv->resident_bytes -= std::min(srv->resident_bytes, S.job_est[k]);
+        srv->resident_bytes -= std::min(srv->resident_bytes, j.est);
         sc.building = false;
         if (st != FREP_OK) {
-            failed_scenes.push_back(S.job_scene[k]);
+            failed_scenes.push_back(j.scene);
             std::lock_guard<std::mutex> lk(ctx->mu);
-            free_tree_buffers(t);
+            free_tree_buffers(ctx, t);
             t->built = false;
             continue;
         }
@@ -410,34 +476,29 @@ void finalize(frep_server* srv, Slot& S) {
             h.nodes[L] = std::max(h.nodes[L], host_nodes(t, L));
         h.pool_words = std::max<uint64_t>(h.pool_words, host_pool_top(t));
     }
-    const char* h = static_cast<const char*>(S.h_out);
     const uint32_t bs = (uint32_t)S.reqs.size();
     for (const SlotReq& r : S.reqs) {
         Scene& sc = srv->scenes[r.scene - 1];
         if (sc.pins) --sc.pins;
         const bool bad = std::find(failed_scenes.begin(), failed_scenes.end(), r.scene) !=
                          failed_scenes.end();
-        const bool built = std::find(S.job_scene.begin(), S.job_scene.end(), r.scene) !=
-                           S.job_scene.end();
-        Ticket& tk = srv->tickets[r.ticket];
-        if (!bad) {
-            if (r.b_rgba) std::memcpy(tk.req.rgba, h + r.off_rgba, r.b_rgba);
-            if (r.b_hit) std::memcpy(tk.req.hit_ids, h + r.off_hit, r.b_hit);
-            if (r.b_nrm) std::memcpy(tk.req.normals, h + r.off_nrm, r.b_nrm);
-        }
-        complete(srv, r.ticket, bad ? FREP_OUT_OF_MEMORY : FREP_OK, built ? 1u : 0u, bs);
+        // outputs stay in the request's pinned block until its waiter copies them
+        complete(srv, r.ticket, bad ? FREP_OUT_OF_MEMORY : FREP_OK, builds(S, r.scene) ? 1u : 0u,
+                 bs);
     }
     S.busy = false;
     S.reqs.clear();
+    S.build.clear();
+    S.jobs.clear();
     srv->done_cv.notify_all();
 }
 
-// A device error inside the loop fails the slot's requests.
+// An error while enqueueing fails the slot's requests.  Caller holds srv->mu.
 void fail_slot(frep_server* srv, Slot& S, frep_status st) {
-    for (size_t k = 0; k < S.job_scene.size(); ++k) {
-        Scene& sc = srv->scenes[S.job_scene[k] - 1];
+    for (const SlotJob& j : S.build) {
+        Scene& sc = srv->scenes[j.scene - 1];
         sc.building = false;
-        srv->resident_bytes -= std::min(srv->resident_bytes, S.job_est[k]);
+        srv->resident_bytes -= std::min(srv->resident_bytes, j.est);
     }
     for (const SlotReq& r : S.reqs) {
         Scene& sc = srv->scenes[r.scene - 1];
@@ -446,8 +507,7 @@ void fail_slot(frep_server* srv, Slot& S, frep_status st) {
     }
     S.reqs.clear();
     S.jobs.clear();
-    S.job_scene.clear();
-    S.job_est.clear();
+    S.build.clear();
     S.busy = false;
     srv->done_cv.notify_all();
 }
@@ -470,12 +530,16 @@ void server_loop(frep_server* srv) {
         if (srv->stop && srv->queue.empty() && !any_busy) break;
         if (free_slot && !srv->queue.empty()) {
             const size_t before = srv->queue.size();
-            const frep_status s = launch_batch(srv, *free_slot);
-            if (s != FREP_OK) {
-                fail_slot(srv, *free_slot, s);
+            if (admit(srv, *free_slot)) {
+                // enqueue without the server lock, so submitters never wait on
+                // HIP calls; only this thread touches the slots
+                lk.unlock();
+                const frep_status s = enqueue_slot(srv, *free_slot);
+                lk.lock();
+                if (s != FREP_OK) fail_slot(srv, *free_slot, s);
                 continue;
             }
-            if (free_slot->busy || srv->queue.size() != before) continue;
+            if (srv->queue.size() != before) continue;
         }
         if (!free_slot && !srv->queue.empty()) {
             // every slot busy: wait for the oldest batch
@@ -541,7 +605,6 @@ frep_status frep_server_destroy(frep_server* srv) {
         if (S.stream) (void)hipStreamSynchronize(S.stream);
         free_stage(&S.stage);
         if (S.d_out) (void)hipFree(S.d_out);
-        if (S.h_out) (void)hipHostFree(S.h_out);
         if (S.done) (void)hipEventDestroy(S.done);
         if (S.stream) (void)hipStreamDestroy(S.stream);
     }
@@ -550,9 +613,9 @@ frep_status frep_server_destroy(frep_server* srv) {
         for (Scene& sc : srv->scenes) {
             TreeObj* t = get_tree(ctx, sc.tree);
             if (!t) continue;
-            free_tree_buffers(t);
-            if (t->d_counters) (void)hipFree(t->d_counters);
-            if (t->h_counters) (void)hipHostFree(t->h_counters);
+            free_tree_buffers(ctx, t);
+            ctx->dpool.free(t->d_counters);
+            ctx->hpool.free(t->h_counters);
             ctx->trees[sc.tree - 1].reset();
         }
     }
@@ -613,28 +676,50 @@ frep_status frep_server_submit(frep_server* srv, const frep_request* req, uint64
     return FREP_OK;
 }
 
+// Copies a completed request's outputs from its pinned block to the caller's
+// buffers (on the caller's thread) and releases the block.
+static void deliver(frep_server* srv, const Ticket& tk) {
+    const char* h = static_cast<const char*>(tk.h_block);
+    if (h && tk.result.status == FREP_OK) {
+        const OutLayout& o = tk.out;
+        if (o.b_rgba) std::memcpy(tk.req.rgba, h + o.o_rgba, o.b_rgba);
+        if (o.b_hit) std::memcpy(tk.req.hit_ids, h + o.o_hit, o.b_hit);
+        if (o.b_nrm) std::memcpy(tk.req.normals, h + o.o_nrm, o.b_nrm);
+    }
+    srv->ctx->hpool.free(tk.h_block);
+}
+
 frep_status frep_server_poll(frep_server* srv, uint64_t ticket, frep_result* out, int* done) {
     if (!srv || !done) return FREP_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(srv->mu);
-    auto it = srv->tickets.find(ticket);
-    if (it == srv->tickets.end()) return srv_fail(srv, FREP_INVALID_ARG, "unknown ticket");
-    *done = it->second.done ? 1 : 0;
-    if (it->second.done) {
-        if (out) *out = it->second.result;
+    Ticket tk;
+    {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        auto it = srv->tickets.find(ticket);
+        if (it == srv->tickets.end()) return srv_fail(srv, FREP_INVALID_ARG, "unknown ticket");
+        *done = it->second.done ? 1 : 0;
+        if (!it->second.done) return FREP_OK;
+        tk = it->second;
         srv->tickets.erase(it);
     }
+    deliver(srv, tk);
+    if (out) *out = tk.result;
     return FREP_OK;
 }
 
 frep_status frep_server_wait(frep_server* srv, uint64_t ticket, frep_result* out) {
     if (!srv) return FREP_INVALID_ARG;
-    std::unique_lock<std::mutex> lk(srv->mu);
-    auto it = srv->tickets.find(ticket);
-    if (it == srv->tickets.end()) return srv_fail(srv, FREP_INVALID_ARG, "unknown ticket");
-    srv->done_cv.wait(lk, [&] { return srv->tickets[ticket].done; });
-    it = srv->tickets.find(ticket);
-    if (out) *out = it->second.result;
-    srv->tickets.erase(it);
+    Ticket tk;
+    {
+        std::unique_lock<std::mutex> lk(srv->mu);
+        auto it = srv->tickets.find(ticket);
+        if (it == srv->tickets.end()) return srv_fail(srv, FREP_INVALID_ARG, "unknown ticket");
+        srv->done_cv.wait(lk, [&] { return srv->tickets[ticket].done; });
+        it = srv->tickets.find(ticket);
+        tk = it->second;
+        srv->tickets.erase(it);
+    }
+    deliver(srv, tk);
+    if (out) *out = tk.result;
     return FREP_OK;
 }
 

@@ -27,6 +27,28 @@ sys.path.insert(0, os.path.join(ROOT, "f-rep-voxel-rendering_amd"))
 
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "voxelize_csg24_1024_hbm.md")
+WORKLOAD_TEXT = {
+    "csg24_1024": "random CSG of 24 primitives (seed 2)",
+    "spheres8_256": "union of 8 spheres (seed 1)",
+    "gyroid_4096": "gyroid lattice (k = 20) thickened, clipped by a box, smooth-unioned with 32 spheres",
+}
+
+
+def profiled_traffic(level):
+    """HBM bytes per launch of a level kernel of the headline workload, from
+    the committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (scaled by the
+    copy-kernel calibration); None if there is no such summary."""
+    try:
+        text = open(TRAFFIC_PROFILE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        if line.startswith("<!-- traffic:"):
+            table = json.loads(line[len("<!-- traffic:"):].rstrip("-> ").strip())
+            v = table.get(f"level{level}")
+            return None if v is None else int(v)
+    return None
 
 
 def cpu_baseline(tape, grid, budget_s=12.0):
@@ -216,7 +238,6 @@ def main():
         sys.exit(2)
 
     from frep import lib, shapes
-    from frep import tree as ftree
 
     tape, grid = shapes.workload(args.workload)
     ctx = lib.Context(0)
@@ -246,8 +267,9 @@ def main():
     voxels = float(F) ** 3
     value = voxels / (ms_per_step * 1e-3) / 1e9
 
-    levels = ftree.parse(ctx.export(tree))
-    work = ftree.level_flops(levels)
+    info = ctx.tree_info(tree)
+    work = [{"flops": int(info.work_flops[L]), "op_evals": int(info.op_evals[L]),
+             "kind": "point" if L == nl - 1 else "interval"} for L in range(nl)]
     dom = max(range(nl), key=lambda L: level_ms[L])
     achieved = work[dom]["flops"] / (level_ms[dom] * 1e-3) / 1e12
     out = {
@@ -264,7 +286,7 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: random CSG of 24 primitives (seed 2), "
+            "workload": f"{args.workload}: {WORKLOAD_TEXT[args.workload]}, "
                         f"{tape.n}-op tape, {F}^3 grid, branching {grid['dims']}",
             "tape_ops": tape.n, "slots": tape.slot_count,
             "nodes_per_level": [int(info.node_count[L]) for L in range(nl)],
@@ -278,14 +300,19 @@ def main():
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / FP32_PEAK_TFLOPS, 5),
-            "traffic": None,
+            "traffic": profiled_traffic(dom) if args.workload == "csg24_1024" else None,
+            "traffic_source": "profiles/voxelize_csg24_1024_hbm.md (rocprofv3 FETCH_SIZE, "
+                              "WRITE_SIZE per dispatch x copy-kernel calibration)",
             "algorithmic_flops_per_launch": work[dom]["flops"],
         },
     }
     if not args.no_render:
-        basis = shapes.camera_basis((1.8, 1.2, 2.4), (-0.55, -0.35, -0.75), (0, 1, 0), 45.0,
-                                    1920, 1080)
-        cam = lib.make_camera(basis, 1920, 1080)
+        fr = shapes.frame(args.workload)
+        w, h = fr["width"], fr["height"]
+        basis = shapes.camera_basis(fr["pos"], fr["fwd"], fr["up"], 45.0, w, h,
+                                    ortho=fr["ortho"], extent=fr.get("extent", (2.0, 2.0)))
+        cam = lib.make_camera(basis, w, h, spp=fr["spp"], ortho=fr["ortho"],
+                              jitter=fr["jitter"], seed=1234)
         ctx.render(tree, cam, want=("rgba",))
         ctx.set_timing(True)
         t0 = time.perf_counter()
@@ -296,7 +323,10 @@ def main():
             rms += ctx.timing().render_ms
         rwall = (time.perf_counter() - t0) / reps
         ctx.set_timing(False)
-        out["render"] = {"frame": "1920x1080 perspective, 1 spp", "kernel_ms": round(rms / reps, 4),
+        kind = "orthographic" if fr["ortho"] else "perspective"
+        jit = ", jittered" if fr["jitter"] else ""
+        out["render"] = {"frame": f"{w}x{h} {kind}, {fr['spp']} spp{jit}",
+                         "kernel_ms": round(rms / reps, 4),
                          "frames_per_s_incl_copy": round(1.0 / rwall, 2)}
     if not args.no_batched:
         out["batched"] = bench_batched(ctx, shapes)

@@ -911,27 +911,29 @@ frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t h, frep_tree_info* out
     out->n_levels = t->grid.n_levels;
     HIP_TRY(hipSetDevice(ctx->device));
     // tape-length stats (what the reference's stats buffer records,
-    // vulkan/shaders/voxelizer/main.comp:146-147): read back node headers
-    // and the tapes' size words
+    // vulkan/shaders/voxelizer/main.comp:146-147) and the work of each
+    // level, reduced on the device
     const uint64_t top = host_pool_top(t);
-    std::vector<uint32_t> pool(top);
-    if (top) HIP_TRY(hipMemcpy(pool.data(), t->d_pool, top * 4, hipMemcpyDeviceToHost));
-    for (uint32_t L = 0; L < t->grid.n_levels; ++L) {
-        const uint32_t n = host_nodes(t, L);
+    const uint32_t nl = t->grid.n_levels;
+    void* acc = nullptr;
+    HIP_TRY(ctx->dpool.alloc(kMaxLevels * 3 * 8, &acc));
+    unsigned long long* d_acc = static_cast<unsigned long long*>(acc);
+    HIP_TRY(hipMemsetAsync(d_acc, 0, kMaxLevels * 3 * 8, ctx->stream));
+    for (uint32_t L = 0; L < nl; ++L)
+        HIP_TRY(launch_tree_stats(ctx->stream, t->d_nodes[L], t->ns[L], host_nodes(t, L), t->d_pool,
+                                  t->pool_cap + kPoolPad, L + 1 < nl, d_acc + 3 * L));
+    unsigned long long h_acc[kMaxLevels * 3] = {};
+    HIP_TRY(hipMemcpyAsync(h_acc, d_acc, sizeof(h_acc), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->dpool.free(acc);
+    for (uint32_t L = 0; L < nl; ++L) {
+        const uint64_t cells = (uint64_t)t->grid.dims[L] * t->grid.dims[L] * t->grid.dims[L];
         out->dims[L] = t->grid.dims[L];
-        out->node_count[L] = n;
-        std::vector<uint32_t> rec((size_t)n * t->ns[L]);
-        if (n) HIP_TRY(hipMemcpy(rec.data(), t->d_nodes[L], rec.size() * 4, hipMemcpyDeviceToHost));
-        uint64_t sum = 0;
-        uint32_t mx = 0;
-        for (uint32_t j = 0; j < n; ++j) {
-            const uint32_t off = rec[(size_t)j * t->ns[L]];
-            const uint32_t len = off < top ? pool[off] : 0;
-            sum += len;
-            mx = std::max(mx, len);
-        }
-        out->tape_len_sum[L] = (uint32_t)std::min<uint64_t>(sum, 0xFFFFFFFFull);
-        out->tape_len_max[L] = mx;
+        out->node_count[L] = host_nodes(t, L);
+        out->tape_len_sum[L] = h_acc[3 * L];
+        out->tape_len_max[L] = (uint32_t)h_acc[3 * L + 1];
+        out->op_evals[L] = h_acc[3 * L] * cells;
+        out->work_flops[L] = h_acc[3 * L + 2] * cells;
     }
     out->tape_pool_words = top;
     out->device_bytes = tree_device_bytes(t);

@@ -148,6 +148,11 @@ hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t
 hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32_t n,
                                   uint32_t* out);
 uint32_t point_k_used(uint32_t maxs, uint32_t dim);
+// Adds {sum of tape lengths, max length, weighted op count} of one level's
+// nodes to out[0..2] (interval weights if `interval`, else point weights).
+hipError_t launch_tree_stats(hipStream_t st, const uint32_t* nodes, uint32_t ns, uint32_t n_nodes,
+                             const uint32_t* pool, uint64_t pool_words, bool interval,
+                             unsigned long long* out);
 hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs);
 hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
                               const float* xyz, uint32_t n, float t, float* out, uint32_t maxs,

@@ -10,6 +10,7 @@
 #include "frep_device.hpp"
 #include "frep_internal.hpp"
 
+#include <algorithm>
 #include <map>
 #include <utility>
 
@@ -1003,6 +1004,60 @@ hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
     FREP_DISPATCH_MAXS(maxs, FREP_POINT_CALL);
 #undef FREP_POINT_CALL
 #undef FREP_POINT_LAUNCH
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Work statistics of one level of a built tree: one wavefront per node reads
+// the node's tape (lanes stride over its words) and adds its length and the
+// weighted cost of its instructions to out[0..2] = {sum of lengths, max
+// length, sum of weights}.  Weights are FP32 operations per evaluation:
+// interval rules for levels above the last, point rules for the last.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_tree_stats(const uint32_t* nodes, uint32_t ns,
+                                                    uint32_t n_nodes, const uint32_t* pool,
+                                                    uint64_t pool_words, uint32_t interval,
+                                                    unsigned long long* out) {
+    const uint32_t lane = lane_id();
+    const uint32_t nwaves = gridDim.x * (blockDim.x / 64);
+    for (uint32_t node = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6); node < n_nodes;
+         node += nwaves) {
+        const uint32_t toff = nodes[(size_t)node * ns];
+        if ((uint64_t)toff + 1 > pool_words) continue;
+        uint32_t n = pool[toff];
+        if ((uint64_t)toff + 1 + n > pool_words) n = (uint32_t)(pool_words - toff - 1);
+        unsigned long long w = 0;
+        for (uint32_t i = lane; i < n; i += 64) {
+            const uint32_t op = pool[(size_t)toff + 1 + i] & 0xFFu;
+            uint32_t c = 0;
+            if (interval) {
+                c = (op == OP_SIN || op == OP_COS || op == OP_DIV) ? 12u
+                    : op == OP_MUL                                   ? 10u
+                    : (op == OP_MIN || op == OP_MAX || op == OP_SQRT) ? 4u
+                    : (op == OP_ADD || op == OP_SUB || op == OP_NEG || op == OP_EXP) ? 2u
+                                                                                     : 0u;
+            } else {
+                c = (op == OP_CONST || op == OP_COPY || op > OP_COPY) ? 0u : 1u;
+            }
+            w += c;
+        }
+#pragma unroll
+        for (int d = 32; d >= 1; d >>= 1) w += (unsigned long long)__shfl_xor((long long)w, d);
+        if (lane == 0) {
+            atomicAdd(out + 0, (unsigned long long)n);
+            atomicMax(out + 1, (unsigned long long)n);
+            atomicAdd(out + 2, w);
+        }
+    }
+}
+
+hipError_t launch_tree_stats(hipStream_t st, const uint32_t* nodes, uint32_t ns, uint32_t n_nodes,
+                             const uint32_t* pool, uint64_t pool_words, bool interval,
+                             unsigned long long* out) {
+    if (n_nodes == 0) return hipSuccess;
+    const uint32_t blocks = std::min<uint32_t>((n_nodes + 3) / 4, 65536);
+    k_tree_stats<<<blocks, 256, 0, st>>>(nodes, ns, n_nodes, pool, pool_words, interval ? 1u : 0u,
+                                         out);
     return hipGetLastError();
 }
 

@@ -56,10 +56,12 @@ class Camera(C.Structure):
 class TreeInfo(C.Structure):
     _fields_ = [("n_levels", C.c_uint32), ("dims", C.c_uint32 * MAX_LEVELS),
                 ("node_count", C.c_uint32 * MAX_LEVELS),
-                ("tape_len_sum", C.c_uint32 * MAX_LEVELS),
                 ("tape_len_max", C.c_uint32 * MAX_LEVELS),
-                ("tape_pool_words", C.c_uint64), ("device_bytes", C.c_uint64),
-                ("slot_count", C.c_uint32), ("retries", C.c_uint32)]
+                ("slot_count", C.c_uint32), ("retries", C.c_uint32), ("pad", C.c_uint32),
+                ("tape_len_sum", C.c_uint64 * MAX_LEVELS),
+                ("op_evals", C.c_uint64 * MAX_LEVELS),
+                ("work_flops", C.c_uint64 * MAX_LEVELS),
+                ("tape_pool_words", C.c_uint64), ("device_bytes", C.c_uint64)]
 
 
 class Timing(C.Structure):

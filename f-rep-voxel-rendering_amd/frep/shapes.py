@@ -184,15 +184,32 @@ def batch_mix(n_shapes: int = 64, seed_base: int = 1000, min_prims: int = 4,
     return out
 
 
-# Named workloads (shape factory, grid).
+# Named workloads: shape factory, grid, and the frame rendered of it.
 WORKLOADS = {
-    "spheres8_256": (lambda: spheres_union(8, 1), make_grid([8, 8, 4])),
-    "csg24_1024": (lambda: random_csg(24, 2), make_grid([16, 8, 8])),
+    # 8 spheres, 256^3 (8/8/4), one 512x512 orthographic frame looking down -z
+    "spheres8_256": (lambda: spheres_union(8, 1), make_grid([8, 8, 4]),
+                     dict(pos=(0.0, 0.0, 3.0), fwd=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0),
+                          width=512, height=512, ortho=True, extent=(2.0, 2.0), spp=1,
+                          jitter=False)),
+    # 24-primitive random CSG, 1024^3 (16/8/8), 1920x1080 perspective
+    "csg24_1024": (lambda: random_csg(24, 2), make_grid([16, 8, 8]),
+                   dict(pos=(1.8, 1.2, 2.4), fwd=(-0.55, -0.35, -0.75), up=(0.0, 1.0, 0.0),
+                        width=1920, height=1080, ortho=False, spp=1, jitter=False)),
+    # gyroid lattice blended with 32 spheres, 4096^3 (16/16/16), 1920x1080 at
+    # 4 jittered samples per pixel
+    "gyroid_4096": (lambda: gyroid_blend(), make_grid([16, 16, 16]),
+                    dict(pos=(1.8, 1.2, 2.4), fwd=(-0.55, -0.35, -0.75), up=(0.0, 1.0, 0.0),
+                         width=1920, height=1080, ortho=False, spp=4, jitter=True)),
 }
 
 
+def frame(name: str) -> dict:
+    """The frame rendered of a named workload (camera and sampling)."""
+    return dict(WORKLOADS[name][2])
+
+
 def workload(name: str):
-    fn, grid = WORKLOADS[name]
+    fn, grid, _ = WORKLOADS[name]
     return fn(), dict(grid)
 
 

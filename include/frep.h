@@ -91,12 +91,21 @@ typedef struct frep_tree_info {
     uint32_t n_levels;
     uint32_t dims[FREP_MAX_LEVELS];
     uint32_t node_count[FREP_MAX_LEVELS];
-    uint32_t tape_len_sum[FREP_MAX_LEVELS];  /* sum of node tape lengths per level */
     uint32_t tape_len_max[FREP_MAX_LEVELS];
-    uint64_t tape_pool_words;                /* words of the tape pool in use */
-    uint64_t device_bytes;                   /* device memory held by the tree */
     uint32_t slot_count;
     uint32_t retries;                        /* pool regrowths of the last voxelize */
+    uint32_t pad;
+    /* sum of node tape lengths per level (the reference's ShaderStats) */
+    uint64_t tape_len_sum[FREP_MAX_LEVELS];
+    /* work of the level: sum over its nodes of tape length x dim^3 cells,
+     * and the same weighted per op as FP32 operations -- above the last
+     * level interval rules (ADD/SUB/NEG 2, MUL 10, DIV 12, MIN/MAX 4,
+     * SQRT 4, EXP 2, SIN/COS 12), at the last level point rules (1 per op);
+     * CONST and COPY count 0 */
+    uint64_t op_evals[FREP_MAX_LEVELS];
+    uint64_t work_flops[FREP_MAX_LEVELS];
+    uint64_t tape_pool_words;                /* words of the tape pool in use */
+    uint64_t device_bytes;                   /* device memory held by the tree */
 } frep_tree_info;
 
 /* Per-launch GPU durations of the last frep_voxelize / frep_render, measured

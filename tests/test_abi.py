@@ -34,7 +34,7 @@ def test_struct_layouts_match_header_sizes():
     # sizes implied by the header's field lists (all 4-byte fields, one u64 pair)
     assert C.sizeof(lib.Grid) == 4 * (1 + 8 + 3 + 1 + 1)
     assert C.sizeof(lib.Camera) == 4 * (3 * 4 + 2 + 6 + 4 + 1 + 16 + 16 + 4)
-    assert C.sizeof(lib.TreeInfo) == 4 * (1 + 8 * 4) + 4 + 16 + 8
+    assert C.sizeof(lib.TreeInfo) == 4 * (1 + 8 * 3 + 3) + 8 * (8 * 3 + 2)
     assert C.sizeof(lib.Timing) == 4 * (1 + 8 + 3)
     assert C.sizeof(lib.ServerConfig) == 16
     assert C.sizeof(lib.Request) == 8 + C.sizeof(lib.Camera) + 4 + 3 * 8

@@ -135,6 +135,25 @@ def test_voxelize_canonical_bit_exact(ctx, oracle, name, dims):
     ctx.tree_free(tree)
 
 
+@pytest.mark.parametrize("name,dims", [("csg24", [16, 8, 8]), ("trig", [8, 8, 4])])
+def test_tree_info_work_matches_exported_tapes(ctx, name, dims):
+    """The device-reduced work statistics of frep_tree_info equal the same
+    sums taken over the exported tree's tapes on the host."""
+    from frep import tree as ftree
+    tape = TAPES[name]()
+    tr = ctx.voxelize(ctx.upload(tape), shapes.make_grid(dims, tape_time=0.5))
+    info = ctx.tree_info(tr)
+    levels = ftree.parse(ctx.export(tr))
+    work = ftree.level_flops(levels)
+    for L, lv in enumerate(levels):
+        lens = [int(t[0]) for t in lv.tapes]
+        assert info.tape_len_sum[L] == sum(lens)
+        assert info.tape_len_max[L] == max(lens)
+        assert info.op_evals[L] == work[L]["op_evals"]
+        assert info.work_flops[L] == work[L]["flops"]
+    ctx.tree_free(tr)
+
+
 def test_voxelize_deterministic_and_reusable(ctx, oracle):
     tape = shapes.random_csg(24, 2)
     grid = shapes.make_grid([16, 8, 8])
@@ -401,3 +420,50 @@ def test_asm_and_cpp_interpreters_agree(oracle):
     finally:
         fast.close()
         plain.close()
+
+
+def test_cpp_engine_classes_match_the_c_abi(ctx, tmp_path):
+    """The C++ engine classes (include/frep_engine.hpp), driven like the
+    reference's Application::Draw by cpp/engine_demo, give the same tree,
+    image and hit ids as the C ABI called directly."""
+    import os
+    import struct
+    import subprocess
+    exe = os.path.join(os.path.dirname(lib.LIB_PATH), "engine_demo")
+    if not os.path.exists(exe):
+        pytest.fail("engine_demo is not built (run __graft_entry__.build())")
+    tape = TAPES["csg24"]()
+    dims, w, h = [16, 8, 8], 160, 120
+    path = tmp_path / "tape.bin"
+    with open(path, "wb") as f:
+        words = np.asarray(tape.words, dtype=np.uint32)
+        consts = np.asarray(tape.consts, dtype=np.float32)
+        f.write(struct.pack("<I", len(words)) + words.tobytes())
+        f.write(struct.pack("<I", len(consts)) + consts.tobytes())
+        f.write(struct.pack("<I", tape.slot_count))
+    prefix = str(tmp_path / "out")
+    r = subprocess.run([exe, str(path), ",".join(map(str, dims)), str(w), str(h), prefix],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got_tree = np.fromfile(prefix + ".tree", dtype=np.uint32)
+    got_rgba = np.fromfile(prefix + ".rgba", dtype=np.uint8).reshape(h, w, 4)
+    got_hits = np.fromfile(prefix + ".hits", dtype=np.uint64)
+    th = ctx.upload(tape)
+    tree = ctx.voxelize(th, shapes.make_grid(dims))
+    cam = lib.Camera()
+    cam.pos[:] = [0.0, 0.0, 3.0]
+    cam.forward[:] = [0.0, 0.0, -1.0]
+    cam.width, cam.height, cam.spp = w, h, 1
+    cam.background[:] = [0.0, 0.0, 0.0, 1.0]
+    cam.ambient[:] = [0.2, 0.2, 0.2, 1.0]
+    lights = lib.default_lights()
+    cam.light_count = len(lights)
+    for l, (d, col) in enumerate(lights):
+        cam.light_dir[l][:] = d
+        cam.light_color[l][:] = col
+    lib.camera_look(cam, [0.0, 1.0, 0.0], 45.0)
+    rgba, hits, _ = ctx.render(tree, cam, want=("rgba", "hits"))
+    assert np.array_equal(got_tree, ctx.export(tree))
+    assert np.array_equal(got_hits, hits)
+    assert np.array_equal(got_rgba, rgba)
+    ctx.tree_free(tree)

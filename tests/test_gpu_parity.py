@@ -325,7 +325,7 @@ def _plain_outputs(ctx, tape, grid, cam):
     return out
 
 
-def _served_case(n_req=36, size=(96, 64)):
+def _served_case(n_req=36, size=(96, 64), cyclic=False):
     scenes = [
         (TAPES["spheres8"](), shapes.make_grid([8, 8, 4])),
         (TAPES["csg24"](), shapes.make_grid([16, 8, 8])),
@@ -337,7 +337,7 @@ def _served_case(n_req=36, size=(96, 64)):
     rng = np.random.default_rng(5)
     reqs = []
     for i in range(n_req):
-        k = int(rng.integers(0, len(scenes)))
+        k = i % len(scenes) if cyclic else int(rng.integers(0, len(scenes)))
         b = shapes.orbit_basis(float(rng.uniform(0, 2 * np.pi)), float(rng.uniform(-1.0, 1.0)),
                                3.0, size[0], size[1])
         cam = lib.make_camera(b, size[0], size[1], spp=1 + (i % 2), jitter=True, seed=100 + i)
@@ -381,10 +381,11 @@ def test_server_results_equal_plain_path(ctx):
 
 
 def test_server_evicts_under_a_small_budget(ctx):
-    """A budget that holds about two trees: scenes are evicted and rebuilt,
-    and every output is still the plain path's."""
-    scenes, reqs = _served_case(n_req=24, size=(64, 48))
-    srv = lib.Server(ctx, budget_bytes=64 << 20, max_batch=4, max_inflight=2)
+    """A budget that holds one 1024^3 CSG tree (about 33 MB) but never two:
+    with the scenes requested in turn, trees are evicted and rebuilt, and
+    every output is still the plain path's."""
+    scenes, reqs = _served_case(n_req=24, size=(64, 48), cyclic=True)
+    srv = lib.Server(ctx, budget_bytes=56 << 20, max_batch=4, max_inflight=2)
     try:
         _check_served(ctx, srv, scenes, reqs)
         st = srv.stats()

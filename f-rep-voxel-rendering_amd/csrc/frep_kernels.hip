@@ -660,23 +660,54 @@ __device__ bool dda(const RenderArgs& a, const float o[3], const float d[3], Hit
     return false;
 }
 
+// Forward-mode gradient (vulkan/shaders/tape/gradient.comp:4-88) of the tape
+// at tape_words (wave-uniform) at each lane's point.  The slot file is four
+// per-component arrays that the compiler keeps in VGPRs (indexed by the
+// uniform slot number), and the tape words arrive through VGPR windows.
+// Called with every lane active; n_cap bounds the words read.
 template <int MAXS>
-__device__ __forceinline__ V4 eval_gradient(ctape_t tape, cconst_t consts, float x, float y,
-                                            float z, float t) {
-    V4 s[MAXS];
-    s[0] = V4{1, 0, 0, x};
-    s[1] = V4{0, 1, 0, y};
-    s[2] = V4{0, 0, 1, z};
-    s[3] = V4{0, 0, 0, t};
-    const uint32_t n = tape[0];
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t w = tape[1 + i];
-        const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-        const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
-        const float c = (op == OP_CONST) ? consts[w >> 16] : 0.0f;
-        s[o] = gradient_op(op, s[ia], s[ib], c);
+__device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t n_cap,
+                                            const float* consts, float x, float y, float z,
+                                            float t, uint32_t lane) {
+    uint32_t n = as_const(tape_words)[0];
+    if ((uint64_t)n + 1 > n_cap) n = n_cap ? (uint32_t)(n_cap - 1) : 0u;
+    const uint32_t* tw = tape_words + 1;
+    float gx[MAXS], gy[MAXS], gz[MAXS], gw[MAXS];
+    gx[0] = 1.0f; gy[0] = 0.0f; gz[0] = 0.0f; gw[0] = x;
+    gx[1] = 0.0f; gy[1] = 1.0f; gz[1] = 0.0f; gw[1] = y;
+    gx[2] = 0.0f; gy[2] = 0.0f; gz[2] = 1.0f; gw[2] = z;
+    gx[3] = 0.0f; gy[3] = 0.0f; gz[3] = 0.0f; gw[3] = t;
+    uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
+    uint32_t fw2 = window_load(tw, 128, n, lane);
+    float fc0 = window_consts(fw0, consts), fc1 = window_consts(fw1, consts);
+    for (uint32_t s = 0; s < n; s += 64) {
+        const uint32_t m = n - s < 64 ? n - s : 64;
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t w = rdlane(fw0, j);
+            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
+            const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+            const V4 A{gx[ia], gy[ia], gz[ia], gw[ia]};
+            FREP_FENCE();
+            V4 B = A;
+            float c = 0.0f;
+            if (op >= OP_ADD && op <= OP_MAX) {
+                B = V4{gx[ib], gy[ib], gz[ib], gw[ib]};
+                FREP_FENCE();
+            } else if (op == OP_CONST) {
+                c = rdlanef(fc0, j);
+            }
+            const V4 r = gradient_op(op, A, B, c);
+            FREP_FENCE();
+            gx[o] = r.x;
+            gy[o] = r.y;
+            gz[o] = r.z;
+            gw[o] = r.w;
+        }
+        fw0 = fw1; fc0 = fc1;
+        fw1 = fw2; fc1 = window_consts(fw2, consts);
+        fw2 = window_load(tw, (int32_t)s + 192, n, lane);
     }
-    return s[0];
+    return V4{gx[0], gy[0], gz[0], gw[0]};
 }
 
 __device__ __forceinline__ void normalize3(float v[3]) {
@@ -699,7 +730,6 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
     const bool valid = px < a.width && py < a.height;
     const uint32_t spp = a.spp ? a.spp : 1;
     float acc[4] = {0, 0, 0, 0};
-    cconst_t consts = as_const(a.consts);
     for (uint32_t smp = 0; smp < spp; ++smp) {
         float ox = 0.0f, oy = 0.0f;
         if (a.jitter) {
@@ -733,17 +763,20 @@ __global__ __launch_bounds__(64) void k_render(RenderArgs a) {
         // evaluates one tape, wave-uniform, for the lanes that hit it.
         V4 g{0, 0, 0, 0};
         bool need = hit && toff < a.pool_words;
+        float p[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) p[c] = orig[c] + dir[c] * h.t;
         for (;;) {
             const uint64_t m = __ballot(need);
             if (m == 0) break;
             const uint32_t leader = (uint32_t)__builtin_ctzll(m);
             const uint32_t tu = uni((uint32_t)__shfl((int)toff, (int)leader));
+            // every lane runs the tape (windows are loaded by the whole
+            // wavefront); the lanes that hit this tape keep the result
+            const V4 gv = eval_gradient<MAXS>(a.pool + tu, a.pool_words - tu, a.consts, p[0], p[1],
+                                              p[2], a.tape_time, lane);
             if (need && toff == tu) {
-                float p[3];
-#pragma unroll
-                for (int c = 0; c < 3; ++c) p[c] = orig[c] + dir[c] * h.t;
-                g = eval_gradient<MAXS>(as_const(a.pool + tu), consts, p[0], p[1], p[2],
-                                        a.tape_time);
+                g = gv;
                 need = false;
             }
         }
@@ -877,8 +910,8 @@ __global__ __launch_bounds__(256) void k_eval_gradients(const uint32_t* tape, co
                                                         float* out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t j = i < n ? i : n - 1;
-    const V4 g = eval_gradient<MAXS>(as_const(tape), as_const(consts), xyz[3 * j],
-                                     xyz[3 * j + 1], xyz[3 * j + 2], t);
+    const V4 g = eval_gradient<MAXS>(tape, ~0ull, consts, xyz[3 * j], xyz[3 * j + 1],
+                                     xyz[3 * j + 2], t, lane_id());
     if (i < n) {
         out[4 * i + 0] = g.x; out[4 * i + 1] = g.y; out[4 * i + 2] = g.z; out[4 * i + 3] = g.w;
     }

@@ -135,6 +135,28 @@ def test_voxelize_canonical_bit_exact(ctx, oracle, name, dims):
     ctx.tree_free(tree)
 
 
+def test_wide_format_tape_vs_oracle(ctx, oracle):
+    """A tape with more than 256 constants (format 2: 16-bit constant index),
+    as in the batched mix: tree, densities and a render equal the oracle's."""
+    tape = shapes.spheres_union(120, 1017)
+    assert len(tape.consts) > 256
+    grid = shapes.make_grid([8, 4, 4])
+    th = ctx.upload(tape)
+    tree = ctx.voxelize(th, grid)
+    ref = oracle.Tree(tape, grid)
+    assert np.array_equal(ctx.export(tree), ref.canonical())
+    rng = np.random.default_rng(17)
+    p = rng.uniform(-1, 1, (1000, 3)).astype(np.float32)
+    want = np.array([oracle.eval_point(tape, *q, 0.0) for q in p], dtype=np.float32)
+    assert same_bits(ctx.eval_points(th, p), want)
+    basis = shapes.camera_basis((0, 0, 3), (0, 0, -1), (0, 1, 0), 45.0, 96, 96)
+    cam = lib.make_camera(basis, 96, 96)
+    _, hits, _ = ctx.render(tree, cam)
+    ohits, _, _ = ref.render(oracle.make_camera(basis, 96, 96))
+    assert np.array_equal(hits, ohits)
+    ctx.tree_free(tree)
+
+
 @pytest.mark.parametrize("name,dims", [("csg24", [16, 8, 8]), ("trig", [8, 8, 4])])
 def test_tree_info_work_matches_exported_tapes(ctx, name, dims):
     """The device-reduced work statistics of frep_tree_info equal the same

@@ -13,19 +13,29 @@
 // covers an operand for all 8 points.  Points arrive in slots 0..2 (x, y, z,
 // pinned operands) and the density leaves in slot 0.
 //
-// Per tape instruction: 9 SALU to decode (slot fields masked to MAXS-1),
-// one window to copy operand A into v[RA..RA+7], a balanced 4-test branch on
-// the op-code bits, and one DST window writing the 8 results back.  ADD reads
-// B straight through the index window; SUB/MUL/MIN/MAX copy B first.  The
-// next instruction word is requested before the current one executes.
+// Tape words: staged in VGPR windows of 64 words (lane l of W0 holds word
+// s + l), fetched with one v_readlane per instruction; the constant of every
+// CONST word of a window is gathered into C0 once per window, so the loop
+// never waits on a scalar load.  Two windows are kept ahead.
 //
-// Arithmetic is bit-identical to the C++ paths: IEEE add/sub/mul, GLM
-// min/max as compare + select (v_cmp_lt + v_cndmask), NEG as a sign flip,
-// and SQRT as the correctly rounded sequence LLVM emits for fsqrt.f32
-// (v_sqrt_f32 estimate, one-ulp candidates checked with fused residuals,
-// scaling for tiny inputs, +-0/+inf passed through).  Hazards the hardware
-// does not interlock (VALU-written VCC/SGPR lane masks read by v_cndmask,
-// v_sqrt results read by VALU) are covered with s_nop as LLVM does.
+// Per instruction: v_readlane + 6 SALU to decode (slot fields masked to
+// MAXS-1), a balanced branch on the op-code bits, then
+//   CONST          8 moves through the DST window;
+//   ADD/SUB/MUL    in place (SRC0+DST window on the output slot) when the
+//                  output slot is one of the operand slots, else operand A
+//                  copied, B read through the window, result moved back;
+//   NEG            in place when the output slot is the operand slot;
+//   COPY           nothing when the output slot is the operand slot;
+//   MIN/MAX/SQRT   operands copied, result moved back.
+//
+// Arithmetic is bit-identical to the C++ paths: IEEE add/sub/mul (a - b is
+// computed as v_sub or v_subrev of the same operands), GLM min/max as
+// compare + select (v_cmp_lt + v_cndmask), NEG as a sign flip, and SQRT as
+// the correctly rounded sequence LLVM emits for fsqrt.f32 (v_sqrt_f32
+// estimate, one-ulp candidates checked with fused residuals, scaling for
+// tiny inputs, +-0/+inf passed through).  Hazards the hardware does not
+// interlock (VALU-written SGPRs read by VALU, lane masks read by
+// v_cndmask, v_sqrt results) are covered with s_nop as LLVM does.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -44,112 +54,181 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 #define FREP_ASM_CLOBBER_40_63                                                                \
     "v40", "v41", "v42", "v43", "v44", "v45", "v46", "v47", "v48", "v49", "v50", "v51", "v52", \
         "v53", "v54", "v55", "v56", "v57", "v58", "v59", "v60", "v61", "v62", "v63"
-#define FREP_ASM_CLOBBER_64_99                                                                \
+#define FREP_ASM_CLOBBER_64_100                                                               \
     "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", "v76", \
         "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", "v88",    \
-        "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99"
-#define FREP_ASM_CLOBBER_100_163                                                               \
-    "v100", "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",    \
+        "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99", "v100"
+#define FREP_ASM_CLOBBER_101_164                                                               \
+    "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",            \
         "v111", "v112", "v113", "v114", "v115", "v116", "v117", "v118", "v119", "v120", "v121", \
         "v122", "v123", "v124", "v125", "v126", "v127", "v128", "v129", "v130", "v131", "v132", \
         "v133", "v134", "v135", "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143", \
         "v144", "v145", "v146", "v147", "v148", "v149", "v150", "v151", "v152", "v153", "v154", \
-        "v155", "v156", "v157", "v158", "v159", "v160", "v161", "v162", "v163"
+        "v155", "v156", "v157", "v158", "v159", "v160", "v161", "v162", "v163", "v164"
 
-// The assembly text.  Symbols: FS first slot register, MS slot-file size,
-// RA / RR / RT operand, result and sqrt temporaries.
+// Building blocks of the assembly text.  Symbols: FS first slot register,
+// MS slot-file size, LG log2(MS), RA / RB operand and result registers,
+// WB tape windows (W0 W1 W2 = WB+0..2, C0 C1 = WB+3..4).
+#define FREP_ASM_READ(R, S)                                                                   \
+    "  s_set_gpr_idx_on " S ", gpr_idx(SRC0)\n"                                               \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    v_mov_b32 v[" R " + \\k], v[FS + \\k]\n"                                             \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM_WRITE(R)                                                                     \
+    "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    v_mov_b32 v[FS + \\k], v[" R " + \\k]\n"                                             \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"
+// Window of tape words starting at word %[sidx] (+ lane), index clamped to
+// the last word, into v[W]; uses RA+0..1.
+#define FREP_ASM_LOADW(W)                                                                     \
+    "  v_mbcnt_lo_u32_b32 v[RA + 1], -1, 0\n"                                                 \
+    "  v_mbcnt_hi_u32_b32 v[RA + 1], -1, v[RA + 1]\n"                                         \
+    "  v_add_u32_e32 v[RA + 0], %[sidx], v[RA + 1]\n"                                         \
+    "  v_min_u32_e32 v[RA + 0], %[nm1], v[RA + 0]\n"                                          \
+    "  v_lshlrev_b32_e32 v[RA + 0], 2, v[RA + 0]\n"                                           \
+    "  global_load_dword v[" W "], v[RA + 0], %[tw]\n"
+// Constant of each CONST word of window v[W] (pool index 0 otherwise) into
+// v[C]; v[W] must have arrived.  Uses RA+2..3.
+#define FREP_ASM_GATHER(C, W)                                                                 \
+    "  v_and_b32_e32 v[RA + 2], 0xff, v[" W "]\n"                                             \
+    "  v_cmp_eq_u32_e32 vcc, 0, v[RA + 2]\n"                                                  \
+    "  v_lshrrev_b32_e32 v[RA + 3], 16, v[" W "]\n"                                           \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e32 v[RA + 3], 0, v[RA + 3], vcc\n"                                      \
+    "  v_lshlrev_b32_e32 v[RA + 3], 2, v[RA + 3]\n"                                           \
+    "  global_load_dword v[" C "], v[RA + 3], %[cst]\n"
+// In-place binary op on the output slot: v[slot] = v[slot] OP v[R].
+#define FREP_ASM_INPLACE(OPC, S, R)                                                           \
+    "  s_set_gpr_idx_on " S ", gpr_idx(SRC0,DST)\n"                                           \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    " OPC " v[FS + \\k], v[FS + \\k], v[" R " + \\k]\n"                                 \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"
+// General binary op: RB = v[b] OP RA, then moved to the output slot.
+#define FREP_ASM_GENERAL(OPC)                                                                 \
+    FREP_ASM_READ("RA", "%[a]")                                                               \
+    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0)\n"                                                \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    " OPC " v[RB + \\k], v[FS + \\k], v[RA + \\k]\n"                                    \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n" FREP_ASM_WRITE("RB")
+// Commutative op (ADD, MUL): in place on whichever operand slot is the
+// output slot.
+#define FREP_ASM_COMMUTATIVE(NAME, OPC)                                                       \
+    ".Lfa_" NAME "%=:\n"                                                                      \
+    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
+    "  s_cbranch_scc1 .Lfa_" NAME "_oa%=\n"                                                   \
+    "  s_cmp_eq_u32 %[o], %[b]\n"                                                             \
+    "  s_cbranch_scc1 .Lfa_" NAME "_ob%=\n"                                                   \
+    FREP_ASM_GENERAL(OPC)                                                                     \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    ".Lfa_" NAME "_oa%=:\n"                                                                   \
+    FREP_ASM_READ("RB", "%[b]")                                                               \
+    FREP_ASM_INPLACE(OPC, "%[a]", "RB")                                                       \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    ".Lfa_" NAME "_ob%=:\n"                                                                   \
+    FREP_ASM_READ("RA", "%[a]")                                                               \
+    FREP_ASM_INPLACE(OPC, "%[b]", "RA")                                                       \
+    "  s_branch .Lfa_next%=\n"
+
 #define FREP_ASM_POINT_BODY                                                                   \
     ".set FS, 16\n"                                                                           \
     ".set MS, %[maxs]\n"                                                                      \
+    ".set LG, %[lg]\n"                                                                        \
     ".set RA, FS + 8*MS\n"                                                                    \
-    ".set RR, RA + 8\n"                                                                       \
-    ".set RT, RR + 8\n"                                                                       \
+    ".set RB, RA + 8\n"                                                                       \
+    ".set WB, RB + 8\n"                                                                       \
     ".irp k, 0,1,2,3,4,5,6,7\n"                                                               \
     "  v_mov_b32 v[FS + 24 + \\k], %[t]\n"                                                    \
     ".endr\n"                                                                                 \
     "s_mov_b32 %[cls], 0x260\n"                                                               \
     "s_mov_b32 %[tiny], 0xf800000\n"                                                          \
-    "s_load_dword %[n], %[tape], 0x0\n"                                                       \
-    "s_mov_b32 %[i], 0\n"                                                                     \
-    "s_waitcnt lgkmcnt(0)\n"                                                                  \
+    "s_mov_b32 %[sgn], 0x80000000\n"                                                          \
     "s_cmp_eq_u32 %[n], 0\n"                                                                  \
-    "s_cbranch_scc1 .Lfa_done%=\n"                                                            \
-    "s_load_dword %[w], %[tape], 0x4\n"                                                       \
-    "s_waitcnt lgkmcnt(0)\n"                                                                  \
+    "s_cbranch_scc1 .Lfa_end%=\n"                                                             \
+    "s_sub_u32 %[nm1], %[n], 1\n"                                                             \
+    "s_mov_b32 %[s], 0\n"                                                                     \
+    "s_mov_b32 %[sidx], 0\n"                                                                  \
+    FREP_ASM_LOADW("WB + 0")                                                                  \
+    "s_mov_b32 %[sidx], 64\n"                                                                 \
+    FREP_ASM_LOADW("WB + 1")                                                                  \
+    "s_mov_b32 %[sidx], 128\n"                                                                \
+    FREP_ASM_LOADW("WB + 2")                                                                  \
+    "s_waitcnt vmcnt(0)\n"                                                                    \
+    FREP_ASM_GATHER("WB + 3", "WB + 0")                                                       \
+    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
+    "s_waitcnt vmcnt(0)\n"                                                                    \
+    ".Lfa_win%=:\n"                                                                           \
+    "  s_sub_u32 %[m], %[n], %[s]\n"                                                          \
+    "  s_min_u32 %[m], %[m], 64\n"                                                            \
+    "  s_mov_b32 %[j], 0\n"                                                                   \
     ".Lfa_loop%=:\n"                                                                          \
-    "  s_lshl_b32 %[off], %[i], 2\n"                                                          \
-    "  s_add_u32 %[off], %[off], 8\n"                                                         \
-    "  s_load_dword %[wn], %[tape], %[off]\n"                                                 \
-    "  s_lshr_b32 %[a], %[w], 16\n"                                                           \
-    "  s_and_b32 %[a], %[a], MS - 1\n"                                                        \
+    "  v_readlane_b32 %[w], v[WB + 0], %[j]\n"                                                \
+    "  s_bfe_u32 %[a], %[w], (LG << 16) | 16\n"                                               \
     "  s_lshl_b32 %[a], %[a], 3\n"                                                            \
-    "  s_lshr_b32 %[b], %[w], 24\n"                                                           \
-    "  s_and_b32 %[b], %[b], MS - 1\n"                                                        \
+    "  s_bfe_u32 %[b], %[w], (LG << 16) | 24\n"                                               \
     "  s_lshl_b32 %[b], %[b], 3\n"                                                            \
-    "  s_lshr_b32 %[o], %[w], 8\n"                                                            \
-    "  s_and_b32 %[o], %[o], MS - 1\n"                                                        \
+    "  s_bfe_u32 %[o], %[w], (LG << 16) | 8\n"                                                \
     "  s_lshl_b32 %[o], %[o], 3\n"                                                            \
-    "  s_set_gpr_idx_on %[a], gpr_idx(SRC0)\n"                                                \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[RA + \\k], v[FS + \\k]\n"                                                \
-    "  .endr\n"                                                                               \
-    "  s_set_gpr_idx_off\n"                                                                   \
     "  s_bitcmp1_b32 %[w], 3\n"                                                               \
     "  s_cbranch_scc1 .Lfa_hi%=\n"                                                            \
     "  s_bitcmp1_b32 %[w], 2\n"                                                               \
     "  s_cbranch_scc1 .Lfa_mid%=\n"                                                           \
     "  s_bitcmp1_b32 %[w], 1\n"                                                               \
     "  s_cbranch_scc1 .Lfa_add%=\n"                                                           \
-    /* op 0: CONST, pool index = bits 16..31 */                                               \
-    "  s_lshr_b32 %[c], %[w], 16\n"                                                           \
-    "  s_lshl_b32 %[c], %[c], 2\n"                                                            \
-    "  s_load_dword %[c], %[cst], %[c]\n"                                                     \
-    "  s_waitcnt lgkmcnt(0)\n"                                                                \
+    /* op 0: CONST -- the window's constant of this word, straight to the slot */             \
+    "  v_readlane_b32 %[c], v[WB + 3], %[j]\n"                                                \
+    "  s_nop 1\n"                                                                             \
+    "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
     "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[RR + \\k], %[c]\n"                                                       \
-    "  .endr\n"                                                                               \
-    "  s_branch .Lfa_write%=\n"                                                               \
-    /* op 3: ADD, B read through the index window */                                          \
-    ".Lfa_add%=:\n"                                                                           \
-    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0)\n"                                                \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_add_f32 v[RR + \\k], v[FS + \\k], v[RA + \\k]\n"                                   \
+    "    v_mov_b32 v[FS + \\k], %[c]\n"                                                       \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_branch .Lfa_write%=\n"                                                               \
-    /* ops 4 SUB, 5 MUL, 7 MIN: B copied to RR first */                                       \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    /* op 3: ADD */                                                                           \
+    FREP_ASM_COMMUTATIVE("add", "v_add_f32_e32")                                              \
+    /* ops 4 SUB, 5 MUL, 7 MIN */                                                             \
     ".Lfa_mid%=:\n"                                                                           \
-    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0)\n"                                                \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[RR + \\k], v[FS + \\k]\n"                                                \
-    "  .endr\n"                                                                               \
-    "  s_set_gpr_idx_off\n"                                                                   \
     "  s_bitcmp1_b32 %[w], 1\n"                                                               \
     "  s_cbranch_scc1 .Lfa_min%=\n"                                                           \
     "  s_bitcmp1_b32 %[w], 0\n"                                                               \
     "  s_cbranch_scc1 .Lfa_mul%=\n"                                                           \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_sub_f32 v[RR + \\k], v[RA + \\k], v[RR + \\k]\n"                                   \
-    "  .endr\n"                                                                               \
-    "  s_branch .Lfa_write%=\n"                                                               \
-    ".Lfa_mul%=:\n"                                                                           \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mul_f32 v[RR + \\k], v[RA + \\k], v[RR + \\k]\n"                                   \
-    "  .endr\n"                                                                               \
-    "  s_branch .Lfa_write%=\n"                                                               \
+    /* SUB: a - b as v_sub (A in the window) or v_subrev (B in the window) */                 \
+    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
+    "  s_cbranch_scc1 .Lfa_sub_oa%=\n"                                                        \
+    "  s_cmp_eq_u32 %[o], %[b]\n"                                                             \
+    "  s_cbranch_scc1 .Lfa_sub_ob%=\n"                                                        \
+    FREP_ASM_GENERAL("v_subrev_f32_e32")                                                      \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    ".Lfa_sub_oa%=:\n"                                                                        \
+    FREP_ASM_READ("RB", "%[b]")                                                               \
+    FREP_ASM_INPLACE("v_sub_f32_e32", "%[a]", "RB")                                           \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    ".Lfa_sub_ob%=:\n"                                                                        \
+    FREP_ASM_READ("RA", "%[a]")                                                               \
+    FREP_ASM_INPLACE("v_subrev_f32_e32", "%[b]", "RA")                                        \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    FREP_ASM_COMMUTATIVE("mul", "v_mul_f32_e32")                                              \
     /* min(A,B) = B < A ? B : A; four compares into four SGPR pairs, then selects */          \
     ".Lfa_min%=:\n"                                                                           \
+    FREP_ASM_READ("RA", "%[a]")                                                               \
+    FREP_ASM_READ("RB", "%[b]")                                                               \
     "  .irp q, 0,4\n"                                                                         \
-    "    v_cmp_lt_f32_e64 %[p0], v[RR + \\q + 0], v[RA + \\q + 0]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p1], v[RR + \\q + 1], v[RA + \\q + 1]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p2], v[RR + \\q + 2], v[RA + \\q + 2]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p3], v[RR + \\q + 3], v[RA + \\q + 3]\n"                          \
-    "    v_cndmask_b32_e64 v[RR + \\q + 0], v[RA + \\q + 0], v[RR + \\q + 0], %[p0]\n"        \
-    "    v_cndmask_b32_e64 v[RR + \\q + 1], v[RA + \\q + 1], v[RR + \\q + 1], %[p1]\n"        \
-    "    v_cndmask_b32_e64 v[RR + \\q + 2], v[RA + \\q + 2], v[RR + \\q + 2], %[p2]\n"        \
-    "    v_cndmask_b32_e64 v[RR + \\q + 3], v[RA + \\q + 3], v[RR + \\q + 3], %[p3]\n"        \
+    "    v_cmp_lt_f32_e64 %[p0], v[RB + \\q + 0], v[RA + \\q + 0]\n"                          \
+    "    v_cmp_lt_f32_e64 %[p1], v[RB + \\q + 1], v[RA + \\q + 1]\n"                          \
+    "    v_cmp_lt_f32_e64 %[p2], v[RB + \\q + 2], v[RA + \\q + 2]\n"                          \
+    "    v_cmp_lt_f32_e64 %[p3], v[RB + \\q + 3], v[RA + \\q + 3]\n"                          \
+    "    s_nop 1\n"                                                                           \
+    "    v_cndmask_b32_e64 v[RB + \\q + 0], v[RA + \\q + 0], v[RB + \\q + 0], %[p0]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 1], v[RA + \\q + 1], v[RB + \\q + 1], %[p1]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 2], v[RA + \\q + 2], v[RB + \\q + 2], %[p2]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 3], v[RA + \\q + 3], v[RB + \\q + 3], %[p3]\n"        \
     "  .endr\n"                                                                               \
-    "  s_branch .Lfa_write%=\n"                                                               \
+    FREP_ASM_WRITE("RB")                                                                      \
+    "  s_branch .Lfa_next%=\n"                                                                \
     /* ops 8 MAX, 10 NEG, 11 SQRT, 12 COPY */                                                 \
     ".Lfa_hi%=:\n"                                                                            \
     "  s_bitcmp1_b32 %[w], 2\n"                                                               \
@@ -157,101 +236,131 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  s_bitcmp1_b32 %[w], 1\n"                                                               \
     "  s_cbranch_scc1 .Lfa_negsqrt%=\n"                                                       \
     /* max(A,B) = A < B ? B : A */                                                            \
-    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0)\n"                                                \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[RR + \\k], v[FS + \\k]\n"                                                \
-    "  .endr\n"                                                                               \
-    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM_READ("RA", "%[a]")                                                               \
+    FREP_ASM_READ("RB", "%[b]")                                                               \
     "  .irp q, 0,4\n"                                                                         \
-    "    v_cmp_lt_f32_e64 %[p0], v[RA + \\q + 0], v[RR + \\q + 0]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p1], v[RA + \\q + 1], v[RR + \\q + 1]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p2], v[RA + \\q + 2], v[RR + \\q + 2]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p3], v[RA + \\q + 3], v[RR + \\q + 3]\n"                          \
-    "    v_cndmask_b32_e64 v[RR + \\q + 0], v[RA + \\q + 0], v[RR + \\q + 0], %[p0]\n"        \
-    "    v_cndmask_b32_e64 v[RR + \\q + 1], v[RA + \\q + 1], v[RR + \\q + 1], %[p1]\n"        \
-    "    v_cndmask_b32_e64 v[RR + \\q + 2], v[RA + \\q + 2], v[RR + \\q + 2], %[p2]\n"        \
-    "    v_cndmask_b32_e64 v[RR + \\q + 3], v[RA + \\q + 3], v[RR + \\q + 3], %[p3]\n"        \
+    "    v_cmp_lt_f32_e64 %[p0], v[RA + \\q + 0], v[RB + \\q + 0]\n"                          \
+    "    v_cmp_lt_f32_e64 %[p1], v[RA + \\q + 1], v[RB + \\q + 1]\n"                          \
+    "    v_cmp_lt_f32_e64 %[p2], v[RA + \\q + 2], v[RB + \\q + 2]\n"                          \
+    "    v_cmp_lt_f32_e64 %[p3], v[RA + \\q + 3], v[RB + \\q + 3]\n"                          \
+    "    s_nop 1\n"                                                                           \
+    "    v_cndmask_b32_e64 v[RB + \\q + 0], v[RA + \\q + 0], v[RB + \\q + 0], %[p0]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 1], v[RA + \\q + 1], v[RB + \\q + 1], %[p1]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 2], v[RA + \\q + 2], v[RB + \\q + 2], %[p2]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 3], v[RA + \\q + 3], v[RB + \\q + 3], %[p3]\n"        \
     "  .endr\n"                                                                               \
-    "  s_branch .Lfa_write%=\n"                                                               \
+    FREP_ASM_WRITE("RB")                                                                      \
+    "  s_branch .Lfa_next%=\n"                                                                \
     ".Lfa_negsqrt%=:\n"                                                                       \
     "  s_bitcmp1_b32 %[w], 0\n"                                                               \
     "  s_cbranch_scc1 .Lfa_sqrt%=\n"                                                          \
+    /* NEG: sign flip, in place when the output slot is the operand slot */                   \
+    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
+    "  s_cbranch_scc1 .Lfa_neg_oa%=\n"                                                        \
+    "  s_set_gpr_idx_on %[a], gpr_idx(SRC0)\n"                                                \
     "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_xor_b32 v[RR + \\k], 0x80000000, v[RA + \\k]\n"                                    \
-    "  .endr\n"                                                                               \
-    "  s_branch .Lfa_write%=\n"                                                               \
-    /* correctly rounded sqrt, one point at a time; RT..RT+3 temporaries */                   \
-    ".Lfa_sqrt%=:\n"                                                                          \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mul_f32_e32 v[RT + 0], 0x4f800000, v[RA + \\k]\n"                                  \
-    "    v_cmp_gt_f32_e32 vcc, %[tiny], v[RA + \\k]\n"                                        \
-    "    s_nop 1\n"                                                                           \
-    "    v_cndmask_b32_e32 v[RT + 0], v[RA + \\k], v[RT + 0], vcc\n"                          \
-    "    v_sqrt_f32_e32 v[RT + 1], v[RT + 0]\n"                                               \
-    "    s_nop 0\n"                                                                           \
-    "    v_add_u32_e32 v[RT + 2], -1, v[RT + 1]\n"                                            \
-    "    v_fma_f32 v[RR + \\k], -v[RT + 2], v[RT + 1], v[RT + 0]\n"                           \
-    "    v_add_u32_e32 v[RT + 3], 1, v[RT + 1]\n"                                             \
-    "    v_cmp_ge_f32_e64 %[p0], 0, v[RR + \\k]\n"                                            \
-    "    s_nop 1\n"                                                                           \
-    "    v_cndmask_b32_e64 v[RT + 2], v[RT + 1], v[RT + 2], %[p0]\n"                          \
-    "    v_fma_f32 v[RT + 1], -v[RT + 3], v[RT + 1], v[RT + 0]\n"                             \
-    "    v_cmp_lt_f32_e64 %[p0], 0, v[RT + 1]\n"                                              \
-    "    s_nop 1\n"                                                                           \
-    "    v_cndmask_b32_e64 v[RT + 2], v[RT + 2], v[RT + 3], %[p0]\n"                          \
-    "    v_mul_f32_e32 v[RT + 3], 0x37800000, v[RT + 2]\n"                                    \
-    "    v_cndmask_b32_e32 v[RT + 2], v[RT + 2], v[RT + 3], vcc\n"                            \
-    "    v_cmp_class_f32_e64 vcc, v[RT + 0], %[cls]\n"                                        \
-    "    s_nop 1\n"                                                                           \
-    "    v_cndmask_b32_e32 v[RR + \\k], v[RT + 2], v[RT + 0], vcc\n"                          \
-    "  .endr\n"                                                                               \
-    "  s_branch .Lfa_write%=\n"                                                               \
-    ".Lfa_copy%=:\n"                                                                          \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[RR + \\k], v[RA + \\k]\n"                                                \
-    "  .endr\n"                                                                               \
-    ".Lfa_write%=:\n"                                                                         \
-    "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[FS + \\k], v[RR + \\k]\n"                                                \
+    "    v_xor_b32_e64 v[RB + \\k], v[FS + \\k], %[sgn]\n"                                    \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_add_u32 %[i], %[i], 1\n"                                                             \
-    "  s_waitcnt lgkmcnt(0)\n"                                                                \
-    "  s_mov_b32 %[w], %[wn]\n"                                                               \
-    "  s_cmp_lt_u32 %[i], %[n]\n"                                                             \
+    FREP_ASM_WRITE("RB")                                                                      \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    ".Lfa_neg_oa%=:\n"                                                                        \
+    "  s_set_gpr_idx_on %[a], gpr_idx(SRC0,DST)\n"                                            \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    v_xor_b32_e64 v[FS + \\k], v[FS + \\k], %[sgn]\n"                                    \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"                                                                   \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    /* correctly rounded sqrt, one point at a time: RB+0..4 temporaries, the  \
+       result replaces the operand copy in RA */                                              \
+    ".Lfa_sqrt%=:\n"                                                                          \
+    FREP_ASM_READ("RA", "%[a]")                                                               \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    v_mul_f32_e32 v[RB + 0], 0x4f800000, v[RA + \\k]\n"                                  \
+    "    v_cmp_gt_f32_e32 vcc, %[tiny], v[RA + \\k]\n"                                        \
+    "    s_nop 1\n"                                                                           \
+    "    v_cndmask_b32_e32 v[RB + 0], v[RA + \\k], v[RB + 0], vcc\n"                          \
+    "    v_sqrt_f32_e32 v[RB + 1], v[RB + 0]\n"                                               \
+    "    s_nop 0\n"                                                                           \
+    "    v_add_u32_e32 v[RB + 2], -1, v[RB + 1]\n"                                            \
+    "    v_fma_f32 v[RB + 4], -v[RB + 2], v[RB + 1], v[RB + 0]\n"                             \
+    "    v_add_u32_e32 v[RB + 3], 1, v[RB + 1]\n"                                             \
+    "    v_cmp_ge_f32_e64 %[p0], 0, v[RB + 4]\n"                                              \
+    "    s_nop 1\n"                                                                           \
+    "    v_cndmask_b32_e64 v[RB + 2], v[RB + 1], v[RB + 2], %[p0]\n"                          \
+    "    v_fma_f32 v[RB + 1], -v[RB + 3], v[RB + 1], v[RB + 0]\n"                             \
+    "    v_cmp_lt_f32_e64 %[p0], 0, v[RB + 1]\n"                                              \
+    "    s_nop 1\n"                                                                           \
+    "    v_cndmask_b32_e64 v[RB + 2], v[RB + 2], v[RB + 3], %[p0]\n"                          \
+    "    v_mul_f32_e32 v[RB + 3], 0x37800000, v[RB + 2]\n"                                    \
+    "    v_cndmask_b32_e32 v[RB + 2], v[RB + 2], v[RB + 3], vcc\n"                            \
+    "    v_cmp_class_f32_e64 vcc, v[RB + 0], %[cls]\n"                                        \
+    "    s_nop 1\n"                                                                           \
+    "    v_cndmask_b32_e32 v[RA + \\k], v[RB + 2], v[RB + 0], vcc\n"                          \
+    "  .endr\n"                                                                               \
+    FREP_ASM_WRITE("RA")                                                                      \
+    "  s_branch .Lfa_next%=\n"                                                                \
+    /* COPY: nothing to do when the output slot is the operand slot */                        \
+    ".Lfa_copy%=:\n"                                                                          \
+    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
+    "  s_cbranch_scc1 .Lfa_next%=\n"                                                          \
+    FREP_ASM_READ("RA", "%[a]")                                                               \
+    FREP_ASM_WRITE("RA")                                                                      \
+    ".Lfa_next%=:\n"                                                                          \
+    "  s_add_u32 %[j], %[j], 1\n"                                                             \
+    "  s_cmp_lt_u32 %[j], %[m]\n"                                                             \
     "  s_cbranch_scc1 .Lfa_loop%=\n"                                                          \
-    ".Lfa_done%=:\n"
+    /* next window: the loads issued one window ago have had a window to arrive */            \
+    "  s_add_u32 %[s], %[s], 64\n"                                                            \
+    "  s_cmp_lt_u32 %[s], %[n]\n"                                                             \
+    "  s_cbranch_scc0 .Lfa_done%=\n"                                                          \
+    "  s_waitcnt vmcnt(0)\n"                                                                  \
+    "  v_mov_b32 v[WB + 0], v[WB + 1]\n"                                                      \
+    "  v_mov_b32 v[WB + 3], v[WB + 4]\n"                                                      \
+    "  v_mov_b32 v[WB + 1], v[WB + 2]\n"                                                      \
+    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
+    "  s_add_u32 %[sidx], %[s], 128\n"                                                        \
+    FREP_ASM_LOADW("WB + 2")                                                                  \
+    "  s_branch .Lfa_win%=\n"                                                                 \
+    ".Lfa_done%=:\n"                                                                          \
+    /* no load may still be writing these registers when the block ends */                    \
+    "s_waitcnt vmcnt(0)\n"                                                                    \
+    ".Lfa_end%=:\n"
 
 // x, y, z: the 8 points of this lane (one register each); returns their
-// densities.  tape: [n, instrs] with 4 readable words past the end; consts:
-// the shape's pool.  MAXS must be 8 or 16 (slot file <= 128 registers).
+// densities.  tape: [n, instrs]; consts: the shape's pool (index 0 readable
+// even when empty).  MAXS must be 8 or 16 (slot file <= 128 registers).
+// Every lane of the wavefront must be active.
 template <int MAXS>
 __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const float* consts,
                                                    float t, v8f x, v8f y, v8f z) {
     static_assert(MAXS == 8 || MAXS == 16, "assembly path covers 8 or 16 slots");
-    uint32_t n, w, wn, a, b, o, i, off, c, cls, tiny;
+    typedef const __attribute__((address_space(4))) uint32_t* cword_t;
+    const uint32_t n = ((cword_t)tape)[0];
+    const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
+    uint32_t w, a, b, o, c, j, m, s, sidx, nm1, cls, tiny, sgn;
     uint64_t p0, p1, p2, p3;
-    const uint64_t tp = (uint64_t)tape, cp = (uint64_t)consts;
     if constexpr (MAXS == 16) {
         asm volatile(FREP_ASM_POINT_BODY
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [n] "=&s"(n),
-                       [w] "=&s"(w), [wn] "=&s"(wn), [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o),
-                       [i] "=&s"(i), [off] "=&s"(off), [c] "=&s"(c), [cls] "=&s"(cls),
-                       [tiny] "=&s"(tiny), [p0] "=&s"(p0), [p1] "=&s"(p1), [p2] "=&s"(p2),
-                       [p3] "=&s"(p3)
-                     : [tape] "s"(tp), [cst] "s"(cp), [t] "v"(t), [maxs] "n"(MAXS)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_99, FREP_ASM_CLOBBER_100_163,
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [c] "=&s"(c), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(4)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_164,
                        "vcc", "memory");
     } else {
         asm volatile(FREP_ASM_POINT_BODY
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [n] "=&s"(n),
-                       [w] "=&s"(w), [wn] "=&s"(wn), [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o),
-                       [i] "=&s"(i), [off] "=&s"(off), [c] "=&s"(c), [cls] "=&s"(cls),
-                       [tiny] "=&s"(tiny), [p0] "=&s"(p0), [p1] "=&s"(p1), [p2] "=&s"(p2),
-                       [p3] "=&s"(p3)
-                     : [tape] "s"(tp), [cst] "s"(cp), [t] "v"(t), [maxs] "n"(MAXS)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_99, "vcc", "memory");
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [c] "=&s"(c), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(3)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, "vcc", "memory");
     }
     return x;
 }

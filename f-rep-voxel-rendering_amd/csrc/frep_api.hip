@@ -828,7 +828,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
                                            t->asm_ok));
             else
                 HIP_TRY(launch_level_interval(ctx->stream, a, t->maxs, ctx->cu_count,
-                                              ctx->level_grid));
+                                              ctx->level_grid, t->asm_ok));
             if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
         }
         HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4, hipMemcpyDeviceToHost,

@@ -132,8 +132,9 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
                             const uint32_t* root_tape, uint32_t root_off, uint32_t n_levels);
 // Level kernels run a persistent grid of min(max_grid, cu_count x resident
 // blocks per CU) blocks; max_grid bounds the per-wave scratch rows.
+// asm_ok: every op of the tape is in the assembly interpreters' set.
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t cu_count, uint32_t max_grid);
+                                 uint32_t cu_count, uint32_t max_grid, bool asm_ok);
 // asm_ok: every op of the tape is in the assembly interpreter's set.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                               uint32_t cu_count, uint32_t max_grid, bool asm_ok);

@@ -28,13 +28,17 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__device__ __forceinline__ uint32_t load_counter(const uint32_t* p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Counter words read by the kernels (level node counts, tape bases, the
+// overflow bits of earlier levels) are final before the reading launch
+// starts: earlier launches on the stream wrote them, and the launch
+// boundary makes those writes visible.  They are read with plain loads,
+// served by the L2: agent-scope atomic loads would go to the memory side,
+// to the same line as the allocation atomics of the running level, and
+// queue behind them (half the level-1 throughput on the 1024^3 random CSG).
+__device__ __forceinline__ uint32_t load_counter(const uint32_t* p) { return *p; }
 
 __device__ __forceinline__ uint64_t load_alloc(const uint32_t* counters, uint32_t level) {
-    return __hip_atomic_load((const unsigned long long*)(counters + CNT_ALLOC + 2 * level),
-                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return *(const unsigned long long*)(counters + CNT_ALLOC + 2 * level);
 }
 
 // Node count of a level (level 0 is the root alone), written by the
@@ -42,6 +46,46 @@ __device__ __forceinline__ uint64_t load_alloc(const uint32_t* counters, uint32_
 __device__ __forceinline__ uint32_t level_nodes(const uint32_t* counters, uint32_t level) {
     return level == 0 ? 1u : uni(alloc_nodes(load_alloc(counters, level)));
 }
+
+// Child reservation of one wavefront: `req` = (nodes << kAllocShift) | words
+// (0 when the chunk has no ambiguous cell) added to the packed counter of
+// the next level; every lane gets the counter's previous value.
+struct WaveReserve {
+    __device__ __forceinline__ uint64_t operator()(unsigned long long* ctr, uint64_t req,
+                                                   uint32_t lane) const {
+        unsigned long long old = 0;
+        if (req != 0 && lane == 0) old = atomicAdd(ctr, (unsigned long long)req);
+        return ((uint64_t)uni((uint32_t)__shfl((int)(uint32_t)(old >> 32), 0)) << 32) |
+               uni((uint32_t)__shfl((int)(uint32_t)old, 0));
+    }
+};
+
+// Child reservation of a whole block: the wavefronts post their requests in
+// LDS, one atomic reserves their sum and each wavefront's share starts after
+// those of the wavefronts before it, so ids stay dense with a quarter of the
+// same-address atomics.  Every wavefront of the block calls it once per
+// round (two barriers); `req`/`base` alternate between two LDS copies by
+// round parity, so a round's posts never overwrite values the previous
+// round still reads.
+struct BlockReserve {
+    unsigned long long* req;    // LDS, one entry per wavefront of the block
+    unsigned long long* base;   // LDS, the block's reserved start
+    uint32_t wib;               // wavefront index in the block
+    __device__ __forceinline__ uint64_t operator()(unsigned long long* ctr, uint64_t req_w,
+                                                   uint32_t lane) const {
+        if (lane == 0) req[wib] = req_w;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long t = 0;
+            for (uint32_t k = 0; k < kLevelBlock / 64; ++k) t += req[k];
+            *base = t != 0 ? atomicAdd(ctr, t) : 0ull;
+        }
+        __syncthreads();
+        unsigned long long old = *base;
+        for (uint32_t k = 0; k < wib; ++k) old += req[k];
+        return old;
+    }
+};
 
 // Write the 64-bit ballot as two mask words (lanes 0 and 1, one word each).
 __device__ __forceinline__ void store_mask64(uint32_t* dst, uint64_t m, uint32_t lane) {
@@ -98,9 +142,10 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 // second pass.  If nothing was removed the child reuses the parent's tape
 // (shorten.comp:188-191).
 // ---------------------------------------------------------------------------
-template <int MAXS>
+template <int MAXS, bool ASM, class Reserve>
 __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
-                                               uint64_t child_base, uint32_t lane, uint4* scr) {
+                                               uint64_t child_base, uint32_t lane, uint4* scr,
+                                               const Reserve& reserve) {
     constexpr int AW = (MAXS + 31) / 32;
     const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
@@ -118,12 +163,22 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     const uint32_t* tw = a.pool + toff + 1;             // instruction words
 
     // ---- forward pass: interval evaluation (interval.comp:121-160) ----
+    float ilo, ihi;
+    uint32_t mm = 0;
+    if constexpr (ASM) {
+        typedef __attribute__((address_space(3))) uint4 lds_row_t;
+        v4f lo = {lx, ly, lz, a.tape_time};
+        v4f hi = {lx + a.cs, ly + a.cs, lz + a.cs, a.tape_time};
+        mm = run_interval_tape_asm<MAXS>(tw, n, a.consts, (uint32_t)(uintptr_t)(lds_row_t*)scr,
+                                         lo, hi);
+        ilo = lo[0];
+        ihi = hi[0];
+    } else {
     float slo[MAXS], shi[MAXS];
     slo[0] = lx; shi[0] = lx + a.cs;
     slo[1] = ly; shi[1] = ly + a.cs;
     slo[2] = lz; shi[2] = lz + a.cs;
     slo[3] = a.tape_time; shi[3] = a.tape_time;
-    uint32_t mm = 0;
     // three windows in flight: the current one (with its constants), the
     // next (constants requested), the one after (words requested)
     uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
@@ -206,8 +261,10 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
         fw1 = fw2; fc1 = window_consts(fw2, a.consts);
         fw2 = window_load(tw, (int32_t)s + 192, n, lane);
     }
+    ilo = slo[0];
+    ihi = shi[0];
+    }
     // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
-    const float ilo = slo[0], ihi = shi[0];
     const bool full = ihi < 0.0f;
     const bool empty = ilo > 0.0f;
     const bool amb = !full && !empty;
@@ -217,18 +274,14 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
 
     uint32_t child_id = kNoChild;
+    // one packed reservation: namb child ids and namb*stride pool words
+    const uint32_t namb = (uint32_t)__popcll(Bamb);
+    const uint32_t stride = n + 1;
+    const uint64_t old =
+        reserve((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
+                ((uint64_t)namb << kAllocShift) | ((uint64_t)namb * stride), lane);
     if (Bamb != 0) {
-        const uint32_t namb = (uint32_t)__popcll(Bamb);
         const uint32_t rank = lane_rank(Bamb);
-        const uint32_t stride = n + 1;
-        // one packed atomic reserves namb child ids and namb*stride words
-        unsigned long long old = 0;
-        if (lane == 0)
-            old = atomicAdd((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
-                            ((unsigned long long)namb << kAllocShift) |
-                                ((unsigned long long)namb * stride));
-        old = ((unsigned long long)uni((uint32_t)__shfl((int)(uint32_t)(old >> 32), 0)) << 32) |
-              uni((uint32_t)__shfl((int)(uint32_t)old, 0));
         const uint32_t cbase = alloc_nodes(old);
         const uint64_t pbase = child_base + alloc_words(old);
         const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
@@ -322,7 +375,7 @@ __device__ __forceinline__ uint64_t child_tape_base(const uint32_t* counters, ui
     return b;
 }
 
-template <int MAXS>
+template <int MAXS, bool ASM>
 __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
     // min/max choice ballots of this wavefront's current chunk: mm_stride rows
     // of {FIRST lanes, SECOND lanes} (two 64-bit masks) per wavefront
@@ -330,8 +383,6 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
     if ((load_counter(a.counters + CNT_OVERFLOW) & ovf_before(a.level)) != 0) return;
     const uint32_t lane = lane_id();
     const uint32_t wib = uni(threadIdx.x >> 6);
-    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + wib;
-    const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
     uint32_t n_nodes = level_nodes(a.counters, a.level);
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
     uint64_t child_base = child_tape_base(a.counters, a.level);
@@ -339,9 +390,21 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
     const uint32_t cpn = a.dim * a.dim * a.dim / 64;
     const uint32_t total = n_nodes * cpn;
     uint4* scr = lds_choice + (size_t)wib * a.mm_stride;
-    for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
-        const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
-        interval_chunk<MAXS>(a, node, sub, child_base, lane, scr);
+    // rounds of kLevelBlock/64 consecutive chunks per block, one reservation
+    // atomic per round; a wavefront past the end still takes part
+    constexpr uint32_t WPB = kLevelBlock / 64;
+    __shared__ unsigned long long blk_req[2][WPB], blk_base[2];
+    unsigned long long* ctr = (unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1));
+    uint32_t par = 0;
+    for (uint32_t g = blockIdx.x; g * WPB < total; g += gridDim.x, par ^= 1u) {
+        const BlockReserve br{blk_req[par], &blk_base[par], wib};
+        const uint32_t chunk = g * WPB + wib;
+        if (chunk < total) {
+            const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
+            interval_chunk<MAXS, ASM>(a, node, sub, child_base, lane, scr, br);
+        } else {
+            br(ctr, 0, lane);
+        }
     }
 }
 
@@ -494,7 +557,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval_batch(BatchArgs 
         if (skip) continue;
         const uint32_t local = chunk - start;
         const uint32_t node = local / cpn, sub = local - node * cpn;
-        interval_chunk<MAXS>(a, node, sub, cb, lane, scr);
+        interval_chunk<MAXS, false>(a, node, sub, cb, lane, scr, WaveReserve{});
     }
 }
 
@@ -959,12 +1022,12 @@ uint32_t resident_grid(Kern kern, uint32_t cu_count, uint32_t max_grid, size_t l
 
 // The occupancy of an interval kernel depends on its dynamic LDS (choice
 // rows); the last answer is kept per instantiation and thread.
-template <int M>
+template <int M, bool ASM>
 uint32_t interval_grid(uint32_t cu_count, uint32_t max_grid, size_t lds) {
     thread_local size_t last_lds = ~(size_t)0;
     thread_local uint32_t last_cu = 0, last_max = 0, g = 0;
     if (lds != last_lds || cu_count != last_cu || max_grid != last_max) {
-        g = resident_grid(k_level_interval<M>, cu_count, max_grid, lds);
+        g = resident_grid(k_level_interval<M, ASM>, cu_count, max_grid, lds);
         last_lds = lds;
         last_cu = cu_count;
         last_max = max_grid;
@@ -987,12 +1050,33 @@ size_t interval_lds_bytes(uint32_t mm_stride) {
     return (size_t)(kLevelBlock / 64) * mm_stride * sizeof(uint4);
 }
 
+// The assembly interval interpreter (run_interval_tape_asm) is not
+// dispatched: its first run on the device, on the 1024^3 random CSG, ended in
+// a device fault whose cause has not been found (an instruction-level
+// emulation of its code matches the C++ rules on that tape), so every
+// interval level runs the C++ interpreter.
+constexpr bool kIntervalAsm = false;
+
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t cu_count, uint32_t max_grid) {
+                                 uint32_t cu_count, uint32_t max_grid, bool asm_ok) {
     const size_t lds = interval_lds_bytes(a.mm_stride);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
-    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<interval_grid<M>(cu_count, max_grid, lds),
-                                                    kLevelBlock, lds, st>>>(a)));
+    if constexpr (kIntervalAsm) {
+        if (asm_ok && maxs == 8) {
+            k_level_interval<8, true><<<interval_grid<8, true>(cu_count, max_grid, lds),
+                                        kLevelBlock, lds, st>>>(a);
+            return hipGetLastError();
+        }
+        if (asm_ok && maxs == 16) {
+            k_level_interval<16, true><<<interval_grid<16, true>(cu_count, max_grid, lds),
+                                         kLevelBlock, lds, st>>>(a);
+            return hipGetLastError();
+        }
+    }
+    (void)asm_ok;
+    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M, false><<<interval_grid<M, false>(
+                                                               cu_count, max_grid, lds),
+                                                           kLevelBlock, lds, st>>>(a)));
     return hipGetLastError();
 }
 

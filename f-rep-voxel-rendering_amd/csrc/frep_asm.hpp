@@ -19,8 +19,9 @@
 // never waits on a scalar load.  Two windows are kept ahead.
 //
 // Per instruction: v_readlane + 6 SALU to decode (slot fields masked to
-// MAXS-1), a balanced branch on the op-code bits, then
-//   CONST          8 moves through the DST window;
+// MAXS-1), a balanced branch on the op-code bits, then (every operand copy
+// and result move being 4 v_mov_b64)
+//   CONST          4 64-bit moves of {c, c} through the DST window;
 //   ADD/SUB/MUL    in place (SRC0+DST window on the output slot) when the
 //                  output slot is one of the operand slots, else operand A
 //                  copied, B read through the window, result moved back;
@@ -69,16 +70,19 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 // Building blocks of the assembly text.  Symbols: FS first slot register,
 // MS slot-file size, LG log2(MS), RA / RB operand and result registers,
 // WB tape windows (W0 W1 W2 = WB+0..2, C0 C1 = WB+3..4).
+// Operand and result moves go two registers at a time (v_mov_b64 issues at
+// the rate of v_mov_b32, and the index window offsets 64-bit operands like
+// 32-bit ones); the slot file, RA and RB start at even registers.
 #define FREP_ASM_READ(R, S)                                                                   \
     "  s_set_gpr_idx_on " S ", gpr_idx(SRC0)\n"                                               \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[" R " + \\k], v[FS + \\k]\n"                                             \
+    "  .irp k, 0,2,4,6\n"                                                                     \
+    "    v_mov_b64 v[" R " + \\k:" R " + \\k + 1], v[FS + \\k:FS + \\k + 1]\n"                 \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"
 #define FREP_ASM_WRITE(R)                                                                     \
     "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[FS + \\k], v[" R " + \\k]\n"                                             \
+    "  .irp k, 0,2,4,6\n"                                                                     \
+    "    v_mov_b64 v[FS + \\k:FS + \\k + 1], v[" R " + \\k:" R " + \\k + 1]\n"                 \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"
 // Window of tape words starting at word %[sidx] (+ lane), index clamped to
@@ -179,12 +183,14 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  s_cbranch_scc1 .Lfa_mid%=\n"                                                           \
     "  s_bitcmp1_b32 %[w], 1\n"                                                               \
     "  s_cbranch_scc1 .Lfa_add%=\n"                                                           \
-    /* op 0: CONST -- the window's constant of this word, straight to the slot */             \
-    "  v_readlane_b32 %[c], v[WB + 3], %[j]\n"                                                \
+    /* op 0: CONST -- the window's constant of this word as the pair {c, c} in  \
+       vcc, written to the slot two registers at a time */                                    \
+    "  v_readlane_b32 vcc_lo, v[WB + 3], %[j]\n"                                              \
+    "  s_mov_b32 vcc_hi, vcc_lo\n"                                                            \
     "  s_nop 1\n"                                                                             \
     "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
-    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    v_mov_b32 v[FS + \\k], %[c]\n"                                                       \
+    "  .irp k, 0,2,4,6\n"                                                                     \
+    "    v_mov_b64 v[FS + \\k:FS + \\k + 1], vcc\n"                                             \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"                                                                   \
     "  s_branch .Lfa_next%=\n"                                                                \
@@ -338,12 +344,12 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
     typedef const __attribute__((address_space(4))) uint32_t* cword_t;
     const uint32_t n = ((cword_t)tape)[0];
     const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
-    uint32_t w, a, b, o, c, j, m, s, sidx, nm1, cls, tiny, sgn;
+    uint32_t w, a, b, o, j, m, s, sidx, nm1, cls, tiny, sgn;
     uint64_t p0, p1, p2, p3;
     if constexpr (MAXS == 16) {
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [c] "=&s"(c), [j] "=&s"(j),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
                        [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
                        [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
                        [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
@@ -354,7 +360,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
     } else {
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [c] "=&s"(c), [j] "=&s"(j),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
                        [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
                        [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
                        [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)

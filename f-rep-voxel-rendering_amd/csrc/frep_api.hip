@@ -824,8 +824,8 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         for (uint32_t L = 0; L < nl; ++L) {
             const LevelArgs a = level_args(t, L, j.mm_stride);
             if (L + 1 == nl)
-                HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, ctx->cu_count, ctx->level_grid,
-                                           t->asm_ok));
+                HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, t->slot_count, ctx->cu_count,
+                                           ctx->level_grid, t->asm_ok));
             else
                 HIP_TRY(launch_level_interval(ctx->stream, a, t->maxs, ctx->cu_count,
                                               ctx->level_grid, t->asm_ok));

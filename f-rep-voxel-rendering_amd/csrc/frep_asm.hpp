@@ -59,6 +59,9 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", "v76", \
         "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", "v88",    \
         "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99", "v100"
+#define FREP_ASM_CLOBBER_101_116                                                               \
+    "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",            \
+        "v111", "v112", "v113", "v114", "v115", "v116"
 #define FREP_ASM_CLOBBER_101_164                                                               \
     "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",            \
         "v111", "v112", "v113", "v114", "v115", "v116", "v117", "v118", "v119", "v120", "v121", \
@@ -340,7 +343,7 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 template <int MAXS>
 __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const float* consts,
                                                    float t, v8f x, v8f y, v8f z) {
-    static_assert(MAXS == 8 || MAXS == 16, "assembly path covers 8 or 16 slots");
+    static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
     typedef const __attribute__((address_space(4))) uint32_t* cword_t;
     const uint32_t n = ((cword_t)tape)[0];
     const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
@@ -357,6 +360,17 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_164,
                        "vcc", "memory");
+    } else if constexpr (MAXS == 10) {
+        asm volatile(FREP_ASM_POINT_BODY
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(4)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
+                       FREP_ASM_CLOBBER_101_116, "vcc", "memory");
     } else {
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),

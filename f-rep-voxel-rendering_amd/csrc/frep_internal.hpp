@@ -136,8 +136,9 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                                  uint32_t cu_count, uint32_t max_grid, bool asm_ok);
 // asm_ok: every op of the tape is in the assembly interpreter's set.
+// slots: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                              uint32_t cu_count, uint32_t max_grid, bool asm_ok);
+                              uint32_t slots, uint32_t cu_count, uint32_t max_grid, bool asm_ok);
 // Batched forms: init of n trees, one level over many shapes, and a copy of
 // the n counter blocks into out (n * CNT_WORDS words) for one readback.
 // max_grid caps the persistent grid (the autotuner varies it).

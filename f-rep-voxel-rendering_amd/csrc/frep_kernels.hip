@@ -416,7 +416,8 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
 template <int MAXS, int K, bool ASM>
 __device__ __forceinline__ void point_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                             uint32_t lane) {
-    static_assert(!ASM || (K == 8 && (MAXS == 8 || MAXS == 16)), "assembly path shape");
+    static_assert(!ASM || (K == 8 && (MAXS == 8 || MAXS == 10 || MAXS == 16)),
+                  "assembly path shape");
     const uint32_t dim = a.dim;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
     ctape_t hdr = as_const(rec);
@@ -1095,12 +1096,18 @@ uint32_t point_k_used(uint32_t maxs, uint32_t dim) {
 }
 
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                              uint32_t cu_count, uint32_t max_grid, bool asm_ok) {
+                              uint32_t slots, uint32_t cu_count, uint32_t max_grid, bool asm_ok) {
     const uint32_t k = point_k_used(maxs, a.dim);
-    // hand-scheduled interpreter: 8 points per lane, slot file of 8 or 16
+    // hand-scheduled interpreter: 8 points per lane, slot file of 8, 10 or
+    // 16 (the 10-slot file keeps 4 wavefronts per SIMD where 16 allows 3)
     if (asm_ok && k == 8 && maxs == 8) {
         k_level_point<8, 8, true><<<point_grid<8, 8, true>(cu_count, max_grid), kLevelBlock, 0,
                                     st>>>(a);
+        return hipGetLastError();
+    }
+    if (asm_ok && k == 8 && maxs == 16 && slots <= 10) {
+        k_level_point<10, 8, true><<<point_grid<10, 8, true>(cu_count, max_grid), kLevelBlock, 0,
+                                     st>>>(a);
         return hipGetLastError();
     }
     if (asm_ok && k == 8 && maxs == 16) {

@@ -106,6 +106,10 @@ void free_tree_buffers(frep_ctx* ctx, TreeObj* t) {
     ctx->dpool.free(t->d_consts);
     t->d_consts = nullptr;
     t->consts_cap = 0;
+    t->consts_uid = 0;
+    if (t->vexec) (void)hipGraphExecDestroy(t->vexec);
+    t->vexec = nullptr;
+    t->vkey = 0;
 }
 
 frep_status ensure_level(frep_ctx* ctx, TreeObj* t, uint32_t L, uint32_t cap) {
@@ -201,6 +205,8 @@ frep_status frep_init(int device, frep_ctx** out) {
     c->no_asm = na && na[0] == '1';
     const char* nt = std::getenv("FREP_NO_AUTOTUNE");
     c->autotune = !(nt && nt[0] == '1');
+    const char* ng = std::getenv("FREP_NO_GRAPH");
+    c->graphs = !(ng && ng[0] == '1');
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     *out = c.release();
@@ -266,6 +272,7 @@ frep_status frep_tape_upload(frep_ctx* ctx, const uint32_t* words, uint32_t n_wo
     if (v) return fail(ctx, FREP_BAD_TAPE, "malformed tape (code " + std::to_string(v) + ")");
     HIP_TRY(hipSetDevice(ctx->device));
     std::unique_ptr<TapeObj> t(new TapeObj());
+    t->uid = ctx->next_uid++;
     const uint32_t n = words[0];
     t->words.assign(words, words + n + 1);
     t->consts.assign(consts, consts + n_consts);
@@ -386,14 +393,18 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
     if (!t->d_consts || t->consts_cap < nc + kPoolPad) {
         ctx->dpool.free(t->d_consts);
         t->d_consts = nullptr;
+        t->consts_uid = 0;
         void* p = nullptr;
         HIP_TRY(ctx->dpool.alloc((size_t)(nc + kPoolPad) * 4, &p));
         t->d_consts = static_cast<float*>(p);
         t->consts_cap = nc + kPoolPad;
     }
-    if (nc)
+    // the copy is enqueued once per (upload, buffer): it is complete before
+    // any later work on the tree, which is ordered after this call's work
+    if (nc && t->consts_uid != tp->uid)
         HIP_TRY(hipMemcpyAsync(t->d_consts, tp->d_consts, (size_t)nc * 4, hipMemcpyDeviceToDevice,
                                st));
+    t->consts_uid = tp->uid;
     // min/max choice rows live in the interval kernel's LDS
     j->mm_stride = std::max<uint32_t>(tp->n_mm, 1);
     if (nl > 1 && j->mm_stride > kMaxMinMax)
@@ -817,22 +828,66 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             const frep_status s = ensure_job(ctx, &j);
             if (s != FREP_OK) return s;
         }
-        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-        HIP_TRY(launch_init_tree(ctx->stream, t->d_counters, t->d_nodes[0], t->d_pool, tp->d_words,
-                                 kRootOff, nl));
-        if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
-        for (uint32_t L = 0; L < nl; ++L) {
-            const LevelArgs a = level_args(t, L, j.mm_stride);
-            if (L + 1 == nl)
-                HIP_TRY(launch_level_point(ctx->stream, a, t->maxs, t->slot_count, ctx->cu_count,
-                                           ctx->level_grid, t->asm_ok));
-            else
-                HIP_TRY(launch_level_interval(ctx->stream, a, t->maxs, ctx->cu_count,
-                                              ctx->level_grid, t->asm_ok));
-            if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
+        LevelArgs la[kMaxLevels];
+        for (uint32_t L = 0; L < nl; ++L) la[L] = level_args(t, L, j.mm_stride);
+        // init, the level launches and the counter readback; with timing on,
+        // an event before the init and after each launch
+        auto enqueue = [&](bool events) -> frep_status {
+            if (events) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
+            HIP_TRY(launch_init_tree(ctx->stream, t->d_counters, t->d_nodes[0], t->d_pool,
+                                     tp->d_words, kRootOff, nl));
+            if (events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
+            for (uint32_t L = 0; L < nl; ++L) {
+                if (L + 1 == nl)
+                    HIP_TRY(launch_level_point(ctx->stream, la[L], t->maxs, t->slot_count,
+                                               ctx->cu_count, ctx->level_grid, t->asm_ok));
+                else
+                    HIP_TRY(launch_level_interval(ctx->stream, la[L], t->maxs, ctx->cu_count,
+                                                  ctx->level_grid, t->asm_ok));
+                if (events) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
+            }
+            HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4,
+                                   hipMemcpyDeviceToHost, ctx->stream));
+            return FREP_OK;
+        };
+        // a repeated voxelization (same buffers, tape, grid and time) is
+        // replayed as one graph: its launches then follow each other without
+        // the per-launch gaps of separate submissions.  A call whose
+        // arguments differ from the previous one's is launched directly, so
+        // animated calls never pay for a capture.
+        uint64_t key = 0;
+        if (!ctx->timing && ctx->graphs) {
+            key = fnv1a(la, sizeof(LevelArgs) * nl);
+            const uintptr_t v[8] = {(uintptr_t)tp->d_words, (uintptr_t)t->h_counters, nl,
+                                    t->maxs, t->slot_count, t->asm_ok ? 1u : 0u,
+                                    ctx->cu_count, ctx->level_grid};
+            key = fnv1a(v, sizeof(v), key);
         }
-        HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4, hipMemcpyDeviceToHost,
-                               ctx->stream));
+        const bool replay = key != 0 && ((t->vexec && t->vkey == key) || t->vkey_seen == key);
+        if (!replay) {
+            const frep_status s = enqueue(ctx->timing);
+            if (s != FREP_OK) return s;
+            t->vkey_seen = key;
+        } else {
+            if (!t->vexec || t->vkey != key) {
+                if (t->vexec) (void)hipGraphExecDestroy(t->vexec);
+                t->vexec = nullptr;
+                hipGraph_t graph = nullptr;
+                HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                const frep_status s = enqueue(false);
+                const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+                if (s != FREP_OK) {
+                    if (graph) (void)hipGraphDestroy(graph);
+                    return s;
+                }
+                HIP_TRY(e);
+                const hipError_t ie = hipGraphInstantiate(&t->vexec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                HIP_TRY(ie);
+                t->vkey = key;
+            }
+            HIP_TRY(hipGraphLaunch(t->vexec, ctx->stream));
+        }
         HIP_TRY(hipStreamSynchronize(ctx->stream));
         if (t->h_counters[CNT_OVERFLOW] == 0) break;
         const frep_status s = regrow_job(ctx, &j, attempt);

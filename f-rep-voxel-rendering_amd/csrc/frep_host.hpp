@@ -27,6 +27,7 @@ struct TapeObj {
     bool asm_ok = false;               // all ops in the assembly interpreter's set
     uint32_t* d_words = nullptr;
     float* d_consts = nullptr;
+    uint64_t uid = 0;                  // unique per upload in its context
 };
 
 struct TreeObj {
@@ -45,6 +46,12 @@ struct TreeObj {
     uint32_t slot_count = 0, maxs = 0;
     bool asm_ok = false;
     uint32_t retries = 0;
+    uint64_t consts_uid = 0;           // upload whose constants d_consts holds (0: none)
+    // the plain voxelization's launches (init, levels, counter readback) as
+    // a graph, replayed while its key (every launch argument) is unchanged
+    hipGraphExec_t vexec = nullptr;
+    uint64_t vkey = 0;
+    uint64_t vkey_seen = 0;            // key of the last direct launch: captured when it repeats
 };
 
 // Caching sub-allocator (frep_pool.hip): device or pinned host blocks carved
@@ -147,6 +154,8 @@ struct frep_ctx {
     std::vector<std::unique_ptr<frep::TreeObj>> trees;
     bool timing = false;
     bool no_asm = false;               // FREP_NO_ASM=1: C++ interpreter only
+    bool graphs = true;                // FREP_NO_GRAPH=1: plain voxelization launched directly
+    uint64_t next_uid = 1;
     hipEvent_t ev[frep::kMaxLevels + 4] = {};
     frep_timing last{};
     // render output staging

@@ -135,6 +135,54 @@ def test_voxelize_canonical_bit_exact(ctx, oracle, name, dims):
     ctx.tree_free(tree)
 
 
+@pytest.mark.parametrize("n_prims,seed,slots", [(8, 3, 9), (20, 2, 11), (48, 5, 13)])
+def test_point_slot_files_vs_oracle(ctx, oracle, n_prims, seed, slots):
+    """The last level's assembly interpreter sizes its register file by the
+    tape's slot count (10 slots for 9..10, 16 for 11..16): trees equal the
+    oracle's on either side of the boundary."""
+    tape = shapes.random_csg(n_prims, seed)
+    assert tape.slot_count == slots
+    grid = shapes.make_grid([8, 8, 4])
+    tree = ctx.voxelize(ctx.upload(tape), grid)
+    assert np.array_equal(ctx.export(tree), oracle.Tree(tape, grid).canonical())
+    ctx.tree_free(tree)
+
+
+def test_repeated_voxelize_replays_equal_direct(oracle):
+    """Repeating a voxelization with unchanged arguments replays it from a
+    graph (captured on the first repeat); a call with new arguments (tape
+    time) runs directly.  Every call's tree equals the oracle's, and the
+    replays equal the direct launches of a context with graphs off."""
+    import os
+    tape = TAPES["trig"]()
+    os.environ["FREP_NO_GRAPH"] = "1"
+    try:
+        direct = lib.Context(0)
+    finally:
+        del os.environ["FREP_NO_GRAPH"]
+    fast = lib.Context(0)
+    try:
+        th = fast.upload(tape)
+        tree = fast.tree_create()
+        other = fast.tree_create()
+        dh = direct.upload(tape)
+        seq = [0.5, 0.5, 0.5, 0.25, 0.25, 0.5, 0.5]
+        for k, tt in enumerate(seq):
+            grid = shapes.make_grid([8, 8, 4], tape_time=tt)
+            fast.voxelize(th, grid, tree)
+            fast.voxelize(th, shapes.make_grid([16, 4], tape_time=tt), other)  # interleaved tree
+            got = fast.export(tree)
+            want = direct.export(direct.voxelize(dh, grid))
+            assert np.array_equal(got, want), k
+            if k in (0, 3):
+                assert np.array_equal(got, oracle.Tree(tape, grid).canonical()), k
+            info = fast.tree_info(tree)
+            assert list(info.node_count[:3]) == oracle.Tree(tape, grid).counts(), k
+    finally:
+        fast.close()
+        direct.close()
+
+
 def test_wide_format_tape_vs_oracle(ctx, oracle):
     """A tape with more than 256 constants (format 2: 16-bit constant index),
     as in the batched mix: tree, densities and a render equal the oracle's."""

@@ -209,6 +209,9 @@ frep_status frep_init(int device, frep_ctx** out) {
     c->graphs = !(ng && ng[0] == '1');
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
+    for (auto& st : c->lanes.st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreateWithFlags(&c->lanes.fork, hipEventDisableTiming));
+    for (auto& e : c->lanes.join) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     *out = c.release();
     return FREP_OK;
 }
@@ -234,6 +237,14 @@ void frep_shutdown(frep_ctx* ctx) {
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
     for (auto& e : ctx->bev) (void)hipEventDestroy(e);
     for (auto& e : ctx->ev)
+        if (e) (void)hipEventDestroy(e);
+    for (auto& st : ctx->lanes.st)
+        if (st) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+        }
+    if (ctx->lanes.fork) (void)hipEventDestroy(ctx->lanes.fork);
+    for (auto& e : ctx->lanes.join)
         if (e) (void)hipEventDestroy(e);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -506,14 +517,32 @@ constexpr uint32_t kTuneBlocksPerCu[] = {8, 4, 2};   // candidates, 4 wavefronts
 constexpr uint32_t kTuneTrials = sizeof(kTuneBlocksPerCu) / sizeof(kTuneBlocksPerCu[0]);
 constexpr size_t kTuneMaxEntries = 64;
 
-BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx) {
+BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
+                     uint32_t lanes) {
     BatchPlan P;
     P.m = (uint32_t)idx.size();
-    const uint32_t groups = (P.m + kBatchMax - 1) / kBatchMax;
-    for (uint32_t g = 0; g < groups; ++g) {
-        BatchGroup G;
-        G.first = g * kBatchMax;
-        G.count = std::min<uint32_t>(kBatchMax, P.m - G.first);
+    // group boundaries: lane cuts (long tapes first), then chunks of kBatchMax
+    std::vector<std::pair<uint32_t, uint32_t>> spans;    // (end, lane)
+    if (lanes > 1 && P.m >= 8) {
+        const uint32_t b1 = P.m / 8, b2 = P.m * 3 / 8;
+        spans = {{b1, 0u}, {b2, 1u}, {P.m, 2u}};
+    } else {
+        spans = {{P.m, 0u}};
+    }
+    std::vector<BatchGroup> bounds;
+    uint32_t begin = 0;
+    for (const auto& sp : spans) {
+        for (uint32_t f = begin; f < sp.first; f += kBatchMax) {
+            BatchGroup G;
+            G.first = f;
+            G.count = std::min<uint32_t>(kBatchMax, sp.first - f);
+            G.lane = sp.second;
+            bounds.push_back(G);
+        }
+        begin = sp.first;
+    }
+    for (uint32_t g = 0; g < bounds.size(); ++g) {
+        BatchGroup G = bounds[g];
         G.init_off = P.bytes;
         P.bytes = align16(P.bytes + G.count * sizeof(InitArgs));
         uint32_t max_nl = 0;
@@ -628,26 +657,45 @@ void free_stage(BatchStage* S) {
 // With ev != nullptr an event is recorded before the first level launch and
 // after each launch.
 frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const BatchPlan& P,
-                          const std::vector<uint32_t>& grid, hipEvent_t* ev) {
+                          const std::vector<uint32_t>& grid, hipEvent_t* ev,
+                          const LaneSet* lanes) {
     uint8_t* ds = static_cast<uint8_t*>(S->d_stage);
     HIP_TRY(hipMemcpyAsync(ds, S->h_stage, P.bytes, hipMemcpyHostToDevice, st));
+    bool used[kBatchLanes] = {};
+    for (const BatchGroup& G : P.groups) used[G.lane] = true;
+    if (!lanes) std::fill(used + 1, used + kBatchLanes, false);
+    // fork: the side lanes start after the argument copy
+    const bool fork = used[1] || used[2];
+    if (fork) {
+        HIP_TRY(hipEventRecord(lanes->fork, st));
+        for (uint32_t k = 1; k < kBatchLanes; ++k)
+            if (used[k]) HIP_TRY(hipStreamWaitEvent(lanes->st[k - 1], lanes->fork, 0));
+    }
     size_t li = 0;
     for (uint32_t g = 0; g < P.groups.size(); ++g) {
         const BatchGroup& G = P.groups[g];
+        const hipStream_t gs = (lanes && G.lane > 0) ? lanes->st[G.lane - 1] : st;
         const InitArgs* d_init = reinterpret_cast<const InitArgs*>(ds + G.init_off);
-        HIP_TRY(launch_init_batch(st, d_init, G.count));
-        if (ev && g == 0) HIP_TRY(hipEventRecord(ev[0], st));
+        HIP_TRY(launch_init_batch(gs, d_init, G.count));
+        if (ev && g == 0) HIP_TRY(hipEventRecord(ev[0], gs));
         for (; li < P.launches.size() && P.launches[li].group == g; ++li) {
             const BatchLaunch& la = P.launches[li];
             BatchArgs b{reinterpret_cast<const LevelArgs*>(ds + la.off), la.count, la.mm};
             if (la.point)
-                HIP_TRY(launch_level_point_batch(st, b, la.maxs, ctx->cu_count, grid[li]));
+                HIP_TRY(launch_level_point_batch(gs, b, la.maxs, ctx->cu_count, grid[li]));
             else
-                HIP_TRY(launch_level_interval_batch(st, b, la.maxs, ctx->cu_count, grid[li]));
-            if (ev) HIP_TRY(hipEventRecord(ev[1 + li], st));
+                HIP_TRY(launch_level_interval_batch(gs, b, la.maxs, ctx->cu_count, grid[li]));
+            if (ev) HIP_TRY(hipEventRecord(ev[1 + li], gs));
         }
-        HIP_TRY(launch_gather_counters(st, d_init, G.count, S->d_cnt + (size_t)G.first * CNT_WORDS));
+        HIP_TRY(launch_gather_counters(gs, d_init, G.count, S->d_cnt + (size_t)G.first * CNT_WORDS));
     }
+    // join: the readback waits for every lane
+    if (fork)
+        for (uint32_t k = 1; k < kBatchLanes; ++k)
+            if (used[k]) {
+                HIP_TRY(hipEventRecord(lanes->join[k - 1], lanes->st[k - 1]));
+                HIP_TRY(hipStreamWaitEvent(st, lanes->join[k - 1], 0));
+            }
     HIP_TRY(hipMemcpyAsync(S->h_cnt, S->d_cnt, (size_t)P.m * CNT_WORDS * 4, hipMemcpyDeviceToHost,
                            st));
     return FREP_OK;
@@ -665,9 +713,16 @@ frep_status ensure_events(frep_ctx* ctx, size_t n) {
 // Runs the batched voxelization of jobs[idx[0..m)] and waits; the counter
 // block of every tree is left in its host mirror.  tune: this is a full
 // batch whose work may be tuned and replayed from a graph.
-frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
+frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<uint32_t>& idx_in,
                       bool tune) {
-    const BatchPlan P = plan_batch(jobs, idx);
+    // longest tapes first: the plan's lanes then separate the few shapes
+    // whose long level-0/1 chains set the latency from the many short ones,
+    // which run beside them instead of after them
+    std::vector<uint32_t> idx(idx_in);
+    std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
+        return jobs[x].tp->words[0] > jobs[y].tp->words[0];
+    });
+    const BatchPlan P = plan_batch(jobs, idx, kBatchLanes);
     if (P.m == 0) return FREP_OK;
     BatchStage* S = &ctx->stage;
     if (S->stage_bytes < P.bytes || S->cnt_words < (size_t)P.m * CNT_WORDS) {
@@ -702,8 +757,12 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
     } else if (rec) {
         grid = rec->grid;
     }
+    // a plan with side lanes is launched directly: a graph replays forked
+    // branches one after another, losing the overlap the lanes are for
+    bool laned = false;
+    for (const BatchGroup& G : P.groups) laned = laned || G.lane > 0;
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-    if (rec && !timed_trial) {
+    if (rec && !timed_trial && !laned) {
         if (!rec->exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
@@ -720,7 +779,8 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
         }
         HIP_TRY(hipGraphLaunch(rec->exec, ctx->stream));
     } else {
-        const frep_status s = enqueue_batch(ctx, S, ctx->stream, P, grid, ev);
+        const frep_status s =
+            enqueue_batch(ctx, S, ctx->stream, P, grid, ev, ev ? nullptr : &ctx->lanes);
         if (s != FREP_OK) return s;
     }
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));

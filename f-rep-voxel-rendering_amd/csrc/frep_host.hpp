@@ -133,6 +133,16 @@ struct BatchLaunch {
 struct BatchGroup {
     uint32_t first, count;
     size_t init_off;                   // byte offset of its InitArgs table
+    uint32_t lane = 0;                 // stream lane (0: the enqueueing stream)
+};
+// Streams and events that let the lanes of a plan run side by side: lane
+// k > 0 runs on st[k - 1], forked from and joined back into the enqueueing
+// stream.
+constexpr uint32_t kBatchLanes = 3;
+struct LaneSet {
+    hipStream_t st[kBatchLanes - 1] = {};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kBatchLanes - 1] = {};
 };
 struct BatchPlan {
     uint32_t m = 0;
@@ -170,6 +180,7 @@ struct frep_ctx {
     bool autotune = true;              // FREP_NO_AUTOTUNE=1 turns it off
     std::unordered_map<uint64_t, frep::TuneRecord> tune;
     std::vector<hipEvent_t> bev;       // per-launch events of tuning trials
+    frep::LaneSet lanes;               // side streams of the synchronous batched path
 };
 
 // Returns from the enclosing function (which has a frep_ctx* ctx) with the
@@ -208,15 +219,22 @@ frep_status ensure_job(frep_ctx* ctx, Job* j);
 frep_status regrow_job(frep_ctx* ctx, Job* j, uint32_t attempt);
 LevelArgs level_args(const TreeObj* t, uint32_t L, uint32_t mm_stride);
 
-BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx);
+// lanes > 1: idx must list the jobs by decreasing tape length; the first
+// eighth of them, the next quarter and the rest form separate groups on
+// lanes 0, 1, 2 (a job count below 8 keeps one lane).
+BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
+                     uint32_t lanes = 1);
 void fill_batch(const BatchPlan& P, const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
                 uint8_t* hs);
 frep_status ensure_stage(frep_ctx* ctx, BatchStage* S, const BatchPlan& P);
 void free_stage(BatchStage* S);
 // Enqueues a filled plan on st: argument copy, launches (persistent grids
 // capped at grid[i]), counter gather and readback into S->h_cnt.
+// With lanes, the groups of lane k > 0 run on lanes->st[k - 1] beside those
+// of lane 0 (ev must then be null).
 frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const BatchPlan& P,
-                          const std::vector<uint32_t>& grid, hipEvent_t* ev);
+                          const std::vector<uint32_t>& grid, hipEvent_t* ev,
+                          const LaneSet* lanes = nullptr);
 
 // Kernel arguments of a render of tree t (outputs are device pointers).
 RenderArgs render_args(const TreeObj* t, const frep_camera* cam, uint8_t* rgba,

@@ -100,9 +100,16 @@ struct SlotJob {
     uint64_t est;                        // budget charged while building
 };
 
+// Renders of a slot run on side streams beside each other (one request per
+// stream in turn), forked from and joined back into the slot's stream.
+constexpr uint32_t kRenderLanes = 4;
+
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    hipStream_t rstream[kRenderLanes] = {};
+    hipEvent_t rfork = nullptr;
+    hipEvent_t rjoin[kRenderLanes] = {};
     bool busy = false;
     BatchStage stage;
     std::vector<SlotJob> build;          // trees this batch builds
@@ -225,8 +232,17 @@ OutLayout layout_outputs(const frep_request& r) {
 frep_status enqueue_renders(frep_server* srv, Slot& S, frep_scene_t only) {
     frep_ctx* ctx = srv->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
+    // fork: every render lane starts after the slot stream's work so far
+    // (the batch that builds the trees)
+    HIP_TRY(hipEventRecord(S.rfork, S.stream));
+    bool used[kRenderLanes] = {};
+    uint32_t next = 0;
     for (const SlotReq& r : S.reqs) {
         if (only && r.scene != only) continue;
+        const uint32_t ln = next++ % kRenderLanes;
+        const hipStream_t rs = S.rstream[ln];
+        if (!used[ln]) HIP_TRY(hipStreamWaitEvent(rs, S.rfork, 0));
+        used[ln] = true;
         const TreeObj* t = get_tree(ctx, r.tree);
         char* d = static_cast<char*>(S.d_out) + r.dbase;
         char* h = static_cast<char*>(r.h_block);
@@ -235,17 +251,20 @@ frep_status enqueue_renders(frep_server* srv, Slot& S, frep_scene_t only) {
             t, &r.cam, o.b_rgba ? (uint8_t*)(d + o.o_rgba) : nullptr,
             o.b_hit ? (unsigned long long*)(d + o.o_hit) : nullptr,
             o.b_nrm ? (float*)(d + o.o_nrm) : nullptr);
-        HIP_TRY(launch_render(S.stream, a, t->maxs));
+        HIP_TRY(launch_render(rs, a, t->maxs));
         if (o.b_rgba)
-            HIP_TRY(hipMemcpyAsync(h + o.o_rgba, d + o.o_rgba, o.b_rgba, hipMemcpyDeviceToHost,
-                                   S.stream));
+            HIP_TRY(hipMemcpyAsync(h + o.o_rgba, d + o.o_rgba, o.b_rgba, hipMemcpyDeviceToHost, rs));
         if (o.b_hit)
-            HIP_TRY(hipMemcpyAsync(h + o.o_hit, d + o.o_hit, o.b_hit, hipMemcpyDeviceToHost,
-                                   S.stream));
+            HIP_TRY(hipMemcpyAsync(h + o.o_hit, d + o.o_hit, o.b_hit, hipMemcpyDeviceToHost, rs));
         if (o.b_nrm)
-            HIP_TRY(hipMemcpyAsync(h + o.o_nrm, d + o.o_nrm, o.b_nrm, hipMemcpyDeviceToHost,
-                                   S.stream));
+            HIP_TRY(hipMemcpyAsync(h + o.o_nrm, d + o.o_nrm, o.b_nrm, hipMemcpyDeviceToHost, rs));
     }
+    // join: the slot stream (and its completion event) waits for every lane
+    for (uint32_t k = 0; k < kRenderLanes; ++k)
+        if (used[k]) {
+            HIP_TRY(hipEventRecord(S.rjoin[k], S.rstream[k]));
+            HIP_TRY(hipStreamWaitEvent(S.stream, S.rjoin[k], 0));
+        }
     return FREP_OK;
 }
 
@@ -583,6 +602,10 @@ frep_status frep_server_create(frep_ctx* ctx, const frep_server_config* cfg, fre
     for (uint32_t k = 0; k < srv->n_slots; ++k) {
         HIP_TRY(hipStreamCreateWithFlags(&srv->slots[k].stream, hipStreamNonBlocking));
         HIP_TRY(hipEventCreateWithFlags(&srv->slots[k].done, hipEventDisableTiming));
+        for (auto& st : srv->slots[k].rstream)
+            HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreateWithFlags(&srv->slots[k].rfork, hipEventDisableTiming));
+        for (auto& e : srv->slots[k].rjoin) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     }
     frep_server* p = srv.get();
     srv->th = std::thread(server_loop, p);
@@ -606,6 +629,14 @@ frep_status frep_server_destroy(frep_server* srv) {
         free_stage(&S.stage);
         if (S.d_out) (void)hipFree(S.d_out);
         if (S.done) (void)hipEventDestroy(S.done);
+        for (auto& st : S.rstream)
+            if (st) {
+                (void)hipStreamSynchronize(st);
+                (void)hipStreamDestroy(st);
+            }
+        if (S.rfork) (void)hipEventDestroy(S.rfork);
+        for (auto& e : S.rjoin)
+            if (e) (void)hipEventDestroy(e);
         if (S.stream) (void)hipStreamDestroy(S.stream);
     }
     {

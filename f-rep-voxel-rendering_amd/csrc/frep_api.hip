@@ -33,6 +33,63 @@ uint32_t maxs_for(uint32_t slot_count) {
     return m;
 }
 
+bool build_root_schedule(const uint32_t* w, std::vector<uint32_t>* out, TapeObj* t) {
+    const uint32_t n = w[0];
+    if (n == 0 || n > kRootMaxInstr) return false;
+    int32_t cur[256];
+    for (uint32_t s = 0; s < 256; ++s) cur[s] = s < 4 ? (int32_t)s : -1;
+    std::vector<uint32_t> va(n, 0), vb(n, 0), depth(4 + n, 0);
+    std::vector<uint32_t> consts;
+    uint32_t max_depth = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t x = w[1 + i];
+        const uint32_t op = x & 0xFFu, o = (x >> 8) & 0xFFu, a = (x >> 16) & 0xFFu, b = x >> 24;
+        if (op == OP_CONST) {
+            consts.push_back(i | ((a | (b << 8)) << 16));
+        } else {
+            const bool binary = op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV ||
+                                op == OP_MIN || op == OP_MAX;
+            if (cur[a] < 0 || (binary && cur[b] < 0)) return false;
+            va[i] = (uint32_t)cur[a];
+            vb[i] = binary ? (uint32_t)cur[b] : va[i];
+            depth[4 + i] = 1 + std::max(depth[va[i]], depth[vb[i]]);
+            max_depth = std::max(max_depth, depth[4 + i]);
+        }
+        cur[o] = (int32_t)(4 + i);
+    }
+    // passes: instructions of depth d in tape order, at most 64 per pass
+    std::vector<std::vector<uint32_t>> by_depth(max_depth + 1);
+    for (uint32_t i = 0; i < n; ++i)
+        if (depth[4 + i] > 0) by_depth[depth[4 + i]].push_back(i);
+    std::vector<uint32_t> offs{0}, ent;
+    for (uint32_t d = 1; d <= max_depth; ++d)
+        for (size_t k = 0; k < by_depth[d].size(); k += 64) {
+            const size_t e = std::min(by_depth[d].size(), k + 64);
+            for (size_t q = k; q < e; ++q) {
+                const uint32_t i = by_depth[d][q];
+                ent.push_back((w[1 + i] & 0xFFu) | (i << 8));
+                ent.push_back(va[i] | (vb[i] << 16));
+            }
+            offs.push_back(offs.back() + (uint32_t)(e - k));
+        }
+    const uint32_t n_pass = (uint32_t)offs.size() - 1;
+    if (2 * n_pass > n) return false;
+    const uint32_t nv = 4 + n;
+    const uint32_t cell_bytes = (8 * nv + nv + n + 15u) & ~15u;
+    if (root_cells_per_block(cell_bytes) == 0) return false;
+    out->assign(offs.begin(), offs.end());
+    t->root_c0 = (uint32_t)out->size();
+    out->insert(out->end(), consts.begin(), consts.end());
+    if (out->size() & 1u) out->push_back(0);
+    t->root_e0 = (uint32_t)out->size();
+    out->insert(out->end(), ent.begin(), ent.end());
+    t->root_pass = n_pass;
+    t->root_const = (uint32_t)consts.size();
+    t->root_res = (uint32_t)cur[0];
+    t->root_cell_bytes = cell_bytes;
+    return true;
+}
+
 frep_status fail(frep_ctx* c, frep_status s, const std::string& msg) {
     if (c) c->err = msg;
     return s;
@@ -207,6 +264,9 @@ frep_status frep_init(int device, frep_ctx** out) {
     c->autotune = !(nt && nt[0] == '1');
     const char* ng = std::getenv("FREP_NO_GRAPH");
     c->graphs = !(ng && ng[0] == '1');
+    const char* ro = std::getenv("FREP_ROOT_OPS");
+    c->root_ops = !(ro && ro[0] == '0');
+    c->root_ops_required = ro && ro[0] == '2';
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     for (auto& st : c->lanes.st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -224,6 +284,7 @@ void frep_shutdown(frep_ctx* ctx) {
         if (!t) continue;
         if (t->d_words) (void)hipFree(t->d_words);
         if (t->d_consts) (void)hipFree(t->d_consts);
+        if (t->d_sched) (void)hipFree(t->d_sched);
     }
     for (auto& t : ctx->trees) {
         if (!t) continue;
@@ -302,6 +363,11 @@ frep_status frep_tape_upload(frep_ctx* ctx, const uint32_t* words, uint32_t n_wo
     HIP_TRY(hipMalloc(&t->d_consts, (size_t)(n_consts + kPoolPad) * 4));
     if (n_consts)
         HIP_TRY(hipMemcpy(t->d_consts, consts, (size_t)n_consts * 4, hipMemcpyHostToDevice));
+    std::vector<uint32_t> sched;
+    if (build_root_schedule(words, &sched, t.get())) {
+        HIP_TRY(hipMalloc(&t->d_sched, sched.size() * 4));
+        HIP_TRY(hipMemcpy(t->d_sched, sched.data(), sched.size() * 4, hipMemcpyHostToDevice));
+    }
     ctx->tapes.push_back(std::move(t));
     *out = (frep_tape_t)ctx->tapes.size();
     return FREP_OK;
@@ -315,6 +381,7 @@ frep_status frep_tape_free(frep_ctx* ctx, frep_tape_t h) {
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (t->d_words) HIP_TRY(hipFree(t->d_words));
     if (t->d_consts) HIP_TRY(hipFree(t->d_consts));
+    if (t->d_sched) HIP_TRY(hipFree(t->d_sched));
     ctx->tapes[h - 1].reset();
     return FREP_OK;
 }
@@ -890,6 +957,24 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         }
         LevelArgs la[kMaxLevels];
         for (uint32_t L = 0; L < nl; ++L) la[L] = level_args(t, L, j.mm_stride);
+        // level 0 in passes over the tape's instructions when the tape has
+        // a schedule (k_root_interval)
+        const bool use_root = ctx->root_ops && tp->d_sched && nl > 1;
+        if (ctx->root_ops_required && nl > 1 && !use_root)
+            return fail(ctx, FREP_UNSUPPORTED, "tape has no level-0 schedule (FREP_ROOT_OPS=2)");
+        RootArgs ra{};
+        if (use_root) {
+            ra.a = la[0];
+            ra.sched = tp->d_sched;
+            ra.tape = tp->d_words;
+            ra.n_pass = tp->root_pass;
+            ra.n_const = tp->root_const;
+            ra.c0 = tp->root_c0;
+            ra.e0 = tp->root_e0;
+            ra.nv = 4 + tp->words[0];
+            ra.res = tp->root_res;
+            ra.cell_bytes = tp->root_cell_bytes;
+        }
         // init, the level launches and the counter readback; with timing on,
         // an event before the init and after each launch
         auto enqueue = [&](bool events) -> frep_status {
@@ -901,6 +986,8 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
                 if (L + 1 == nl)
                     HIP_TRY(launch_level_point(ctx->stream, la[L], t->maxs, t->slot_count,
                                                ctx->cu_count, ctx->level_grid, t->asm_ok));
+                else if (L == 0 && use_root)
+                    HIP_TRY(launch_root_interval(ctx->stream, ra));
                 else
                     HIP_TRY(launch_level_interval(ctx->stream, la[L], t->maxs, ctx->cu_count,
                                                   ctx->level_grid, t->asm_ok));
@@ -918,9 +1005,9 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         uint64_t key = 0;
         if (!ctx->timing && ctx->graphs) {
             key = fnv1a(la, sizeof(LevelArgs) * nl);
-            const uintptr_t v[8] = {(uintptr_t)tp->d_words, (uintptr_t)t->h_counters, nl,
+            const uintptr_t v[9] = {(uintptr_t)tp->d_words, (uintptr_t)t->h_counters, nl,
                                     t->maxs, t->slot_count, t->asm_ok ? 1u : 0u,
-                                    ctx->cu_count, ctx->level_grid};
+                                    ctx->cu_count, ctx->level_grid, use_root ? 1u : 0u};
             key = fnv1a(v, sizeof(v), key);
         }
         const bool replay = key != 0 && ((t->vexec && t->vkey == key) || t->vkey_seen == key);

@@ -28,6 +28,11 @@ struct TapeObj {
     uint32_t* d_words = nullptr;
     float* d_consts = nullptr;
     uint64_t uid = 0;                  // unique per upload in its context
+    // level-0 schedule of k_root_interval (null when the tape does not
+    // qualify; RootArgs describes the layout)
+    uint32_t* d_sched = nullptr;
+    uint32_t root_pass = 0, root_const = 0, root_c0 = 0, root_e0 = 0, root_res = 0;
+    uint32_t root_cell_bytes = 0;
 };
 
 struct TreeObj {
@@ -165,6 +170,8 @@ struct frep_ctx {
     bool timing = false;
     bool no_asm = false;               // FREP_NO_ASM=1: C++ interpreter only
     bool graphs = true;                // FREP_NO_GRAPH=1: plain voxelization launched directly
+    bool root_ops = true;              // FREP_ROOT_OPS=0: level 0 by the per-cell kernel
+    bool root_ops_required = false;    // FREP_ROOT_OPS=2: a tape without a level-0 schedule is an error
     uint64_t next_uid = 1;
     hipEvent_t ev[frep::kMaxLevels + 4] = {};
     frep_timing last{};
@@ -197,6 +204,12 @@ struct frep_ctx {
 namespace frep {
 
 uint32_t maxs_for(uint32_t slot_count);
+// Level-0 schedule of k_root_interval for tape words w ([n, instrs]):
+// fills out and the root_* fields of t, or returns false when the tape reads
+// a slot above 3 before writing it, has more than kRootMaxInstr
+// instructions, would average fewer than two instructions per pass, or
+// needs more LDS per cell than 8 cells per block allow.
+bool build_root_schedule(const uint32_t* w, std::vector<uint32_t>* out, TapeObj* t);
 frep_status fail(frep_ctx* c, frep_status s, const std::string& msg);
 TapeObj* get_tape(frep_ctx* ctx, frep_tape_t h);
 TreeObj* get_tree(frep_ctx* ctx, frep_tree_t h);

@@ -75,6 +75,28 @@ struct LevelArgs {
     uint32_t use_asm;         // point level: assembly interpreter (pk == 8, slot file <= 16)
 };
 
+// Root level evaluated in passes over the tape's instructions
+// (k_root_interval).  sched holds, in u32 words:
+//   [0 .. n_pass]          entry offset of each pass, then the total
+//   [c0 .. c0 + n_const)   CONST instructions: i | const index << 16
+//   [e0 ..)                two words per instruction: op | i << 8, va | vb << 16
+// Values: 0..3 = x, y, z, t; 4 + i = result of instruction i.
+struct RootArgs {
+    LevelArgs a;              // level 0 of the tree
+    const uint32_t* sched;
+    const uint32_t* tape;     // [n, instrs] of the root tape
+    uint32_t n_pass;
+    uint32_t n_const;
+    uint32_t c0;
+    uint32_t e0;              // even (entries are read as 8-byte pairs)
+    uint32_t nv;              // 4 + n
+    uint32_t res;             // value in slot 0 at the end of the tape
+    uint32_t cell_bytes;      // LDS per cell: 8 nv (values) + nv (live) + n (choices), 16-aligned
+    uint32_t pad;
+};
+constexpr uint32_t kRootMaxInstr = 1024;       // schedule limit (16-bit value ids, LDS)
+constexpr size_t kRootMaxLds = 60 * 1024;      // dynamic LDS per block
+
 // One tree of a batch: what the init kernel needs.
 struct InitArgs {
     uint32_t* counters;
@@ -139,6 +161,10 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
 // slots: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                               uint32_t slots, uint32_t cu_count, uint32_t max_grid, bool asm_ok);
+// Cells per block of k_root_interval for a per-cell LDS size: 16 or 8, or
+// 0 when not even 8 cells fit.
+uint32_t root_cells_per_block(uint32_t cell_bytes);
+hipError_t launch_root_interval(hipStream_t st, const RootArgs& r);
 // Batched forms: init of n trees, one level over many shapes, and a copy of
 // the n counter blocks into out (n * CNT_WORDS words) for one readback.
 // max_grid caps the persistent grid (the autotuner varies it).

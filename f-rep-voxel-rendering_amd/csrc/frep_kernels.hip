@@ -409,6 +409,204 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Root level, parallel over the tape's instructions.  The root is one node
+// of dim^3 cells evaluated with the full tape; the per-cell kernel above
+// gives it dim^3/64 wavefronts, each walking the whole tape one instruction
+// at a time.  Here one wavefront owns one cell and its lanes evaluate
+// independent instructions together (the same interval rules,
+// interval.comp:36-160, on the same operand values).
+//
+// The host renames the tape's slots into values (0..3 = x, y, z, t; 4+i =
+// result of instruction i) and groups the non-CONST instructions into passes
+// by dependency depth, at most 64 per pass, so every operand of a pass was
+// written by an earlier pass; CONST values are written before the first.
+// Values, choices and liveness flags of a cell live in LDS.
+//
+// Shortening (shorten.comp:164-223) keeps instruction i iff slot out(i) is
+// active when the backward pass reaches i, which is iff a kept later
+// instruction reads, through its min/max choice, the value i wrote.  In the
+// renamed form that is reachability from the result value along chosen
+// operand edges: the passes in reverse order.  Kept instructions are then
+// written in tape order, 64 at a time, at the same place in the cell's
+// (n+1)-word region as the per-cell kernel writes them.
+//
+// A block of C wavefronts owns C consecutive mask bits (a 16- or 8-bit
+// slice of a mask word, stored as such) and reserves the child ids and tape
+// regions of its ambiguous cells with one atomic.
+// ---------------------------------------------------------------------------
+template <int C>
+__device__ __forceinline__ void store_mask_slice(uint32_t* words, uint32_t slice, uint32_t bits) {
+    if constexpr (C == 16) ((uint16_t*)words)[slice] = (uint16_t)bits;
+    else ((uint8_t*)words)[slice] = (uint8_t)bits;
+}
+
+template <int C>
+__global__ __launch_bounds__(64 * C) void k_root_interval(RootArgs r) {
+    static_assert(C == 8 || C == 16, "mask slices are bytes or half-words");
+    extern __shared__ __align__(16) unsigned char root_lds[];
+    __shared__ uint32_t blk_flags[C];
+    __shared__ uint32_t blk_amb;
+    __shared__ unsigned long long blk_old;
+    const LevelArgs& a = r.a;
+    const uint32_t lane = lane_id();
+    const uint32_t wib = uni(threadIdx.x >> 6);
+    const uint32_t idx = blockIdx.x * C + wib;           // mask bit of this cell
+    const uint32_t n = r.nv - 4u;
+    float2* V = (float2*)(root_lds + (size_t)wib * r.cell_bytes);
+    uint8_t* live = (uint8_t*)(V + r.nv);
+    uint8_t* chs = live + r.nv;
+    const uint32_t dim = a.dim, W = dim * dim * dim / 32;
+    uint32_t* rec = a.nodes;
+    ctape_t hdr = as_const(rec);
+    const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
+    uint32_t cx, cy, cz;
+    cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
+    const uint32_t gx = ncx * dim + cx, gy = ncy * dim + cy, gz = ncz * dim + cz;
+    // main.comp:113-116, 124-126: box [pos, pos + cell_size]
+    const float lx = a.low[0] + (float)gx * a.cs;
+    const float ly = a.low[1] + (float)gy * a.cs;
+    const float lz = a.low[2] + (float)gz * a.cs;
+
+    // ---- forward pass ----
+    for (uint32_t v = lane; v < r.nv; v += 64) live[v] = 0;
+    if (lane < 4) {
+        const float lo = lane == 0 ? lx : (lane == 1 ? ly : (lane == 2 ? lz : a.tape_time));
+        const float hi = lane == 3 ? a.tape_time : lo + a.cs;
+        V[lane] = make_float2(lo, hi);
+    }
+    for (uint32_t k = lane; k < r.n_const; k += 64) {
+        const uint32_t e = r.sched[r.c0 + k];            // i | const index << 16
+        const float c = a.consts[e >> 16];
+        V[4 + (e & 0xFFFFu)] = make_float2(c, c);
+    }
+    const uint2* ent = (const uint2*)(r.sched + r.e0);  // {op | i << 8, va | vb << 16}
+    for (uint32_t p = 0; p < r.n_pass; ++p) {
+        const uint32_t b0 = r.sched[p], cnt = r.sched[p + 1] - b0;
+        if (lane < cnt) {
+            const uint2 e = ent[b0 + lane];
+            const uint32_t op = e.x & 0xFFu, i = e.x >> 8;
+            const float2 av = V[e.y & 0xFFFFu], bv = V[e.y >> 16];
+            const Itv A{av.x, av.y}, B{bv.x, bv.y};
+            Itv res;
+            if (op == OP_ADD) {
+                res = Itv{A.lo + B.lo, A.hi + B.hi};
+            } else if (op == OP_SUB) {
+                res = Itv{A.lo - B.hi, A.hi - B.lo};
+            } else if (op == OP_MUL) {
+                res = imul(A, B);
+            } else if (op == OP_MIN || op == OP_MAX) {
+                res = op == OP_MIN ? Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)}
+                                   : Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
+                chs[i] = (uint8_t)mm_choice(op, A, B);
+            } else {
+                res = interval_op(op, A, B, 0.0f);
+            }
+            V[4 + i] = make_float2(res.lo, res.hi);
+        }
+    }
+    const float2 R = V[r.res];
+    // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
+    const bool full = R.y < 0.0f;
+    const bool empty = R.x > 0.0f;
+    const bool amb = !full && !empty;
+
+    // ---- block: mask slices and one reservation for the block's children ----
+    const uint32_t stride = n + 1;
+    if (lane == 0) blk_flags[wib] = (full ? 1u : 0u) | (amb ? 2u : 0u);
+    __syncthreads();
+    if (wib == 0) {
+        const uint32_t f = lane < C ? blk_flags[lane] : 0u;
+        const uint64_t bf = __ballot((f & 1u) != 0), ba = __ballot((f & 2u) != 0);
+        const uint64_t namb = (uint64_t)__popcll(ba);
+        if (lane == 0) {
+            store_mask_slice<C>(rec + 4, blockIdx.x, (uint32_t)bf);
+            store_mask_slice<C>(rec + 4 + W, blockIdx.x, (uint32_t)ba);
+            blk_amb = (uint32_t)ba;
+            unsigned long long* ctr = (unsigned long long*)(a.counters + CNT_ALLOC + 2);
+            blk_old = namb ? atomicAdd(ctr, (namb << kAllocShift) | (namb * stride)) : 0ull;
+        }
+    }
+    __syncthreads();
+
+    uint32_t child_id = kNoChild;
+    if (amb) {
+        const uint32_t bam = blk_amb;
+        const uint32_t namb = (uint32_t)__popc(bam);
+        const uint32_t rank = (uint32_t)__popc(bam & ((1u << wib) - 1u));
+        const uint64_t old = blk_old;
+        const uint32_t cbase = alloc_nodes(old);
+        const uint64_t pbase = child_tape_base(a.counters, 0) + alloc_words(old);
+        const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
+        const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
+        if (!(nodes_ok && pool_ok)) {
+            if (lane == 0)
+                atomicOr(a.counters + CNT_OVERFLOW,
+                         (nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL));
+        } else {
+            // ---- backward pass: values reachable from the result ----
+            if (lane == 0) live[r.res] = 1;
+            for (uint32_t p = r.n_pass; p-- > 0;) {
+                const uint32_t b0 = r.sched[p], cnt = r.sched[p + 1] - b0;
+                if (lane < cnt) {
+                    const uint2 e = ent[b0 + lane];
+                    const uint32_t op = e.x & 0xFFu, i = e.x >> 8;
+                    if (live[4 + i]) {
+                        uint32_t sel = MM_FIRST;
+                        if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
+                            sel = MM_BOTH;
+                        else if (op == OP_MIN || op == OP_MAX)
+                            sel = chs[i];
+                        if (sel & MM_FIRST) live[e.y & 0xFFFFu] = 1;
+                        if (sel & MM_SECOND) live[e.y >> 16] = 1;
+                    }
+                }
+            }
+            // ---- kept instructions in tape order (shorten.comp:61-88, 188-191) ----
+            uint32_t k = 0;
+            for (uint32_t s = 0; s < n; s += 64)
+                k += (uint32_t)__popcll(__ballot(s + lane < n && live[4 + s + lane] != 0));
+            const uint64_t mybase = pbase + (uint64_t)rank * stride;
+            uint32_t final_off = toff;
+            if (k < n) {
+                uint32_t* out = a.pool + mybase + (n - k);
+                if (lane == 0) out[0] = k;
+                uint32_t j = 1;
+                for (uint32_t s = 0; s < n; s += 64) {
+                    const uint32_t i = s + lane;
+                    const bool kp = i < n && live[4 + i] != 0;
+                    const uint64_t b = __ballot(kp);
+                    if (kp) {
+                        uint32_t wd = r.tape[1 + i];
+                        const uint32_t op = wd & 0xFFu;
+                        if (op == OP_MIN || op == OP_MAX) {
+                            const uint32_t c = chs[i];
+                            if (c == MM_FIRST) {
+                                wd = (wd & ~0xFFu) | OP_COPY;
+                            } else if (c == MM_SECOND) {
+                                const uint32_t o = (wd >> 8) & 0xFFu, ib = wd >> 24;
+                                wd = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
+                            }
+                        }
+                        out[j + lane_rank(b)] = wd;
+                    }
+                    j += (uint32_t)__popcll(b);
+                }
+                final_off = (uint32_t)(mybase + n - k);
+            }
+            child_id = cbase + rank;
+            if (lane == 0) {
+                uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
+                crec[0] = final_off;
+                crec[1] = gx;
+                crec[2] = gy;
+                crec[3] = gz;
+            }
+        }
+    }
+    if (lane == 0) rec[4 + 2 * W + idx] = child_id;
+}
+
+// ---------------------------------------------------------------------------
 // Last level: point evaluation at each voxel's low corner, leaf iff < 0
 // (vulkan/shaders/voxelizer/main.comp:74-83).  Each lane evaluates K voxels
 // of the same node (same tape) so instruction decode is shared K ways.
@@ -1078,6 +1276,25 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
     FREP_DISPATCH_MAXS(maxs, (k_level_interval<M, false><<<interval_grid<M, false>(
                                                                cu_count, max_grid, lds),
                                                            kLevelBlock, lds, st>>>(a)));
+    return hipGetLastError();
+}
+
+uint32_t root_cells_per_block(uint32_t cell_bytes) {
+    if ((size_t)cell_bytes * 16 <= kRootMaxLds) return 16;
+    if ((size_t)cell_bytes * 8 <= kRootMaxLds) return 8;
+    return 0;
+}
+
+hipError_t launch_root_interval(hipStream_t st, const RootArgs& r) {
+    const uint32_t c = root_cells_per_block(r.cell_bytes);
+    const uint32_t cells = r.a.dim * r.a.dim * r.a.dim;
+    if (c == 0 || cells % c != 0 || (r.e0 & 1u) || r.nv < 4 || r.res >= r.nv)
+        return hipErrorInvalidValue;
+    const size_t lds = (size_t)c * r.cell_bytes;
+    if (c == 16)
+        k_root_interval<16><<<cells / 16, 1024, lds, st>>>(r);
+    else
+        k_root_interval<8><<<cells / 8, 512, lds, st>>>(r);
     return hipGetLastError();
 }
 

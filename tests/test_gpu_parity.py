@@ -183,6 +183,50 @@ def test_repeated_voxelize_replays_equal_direct(oracle):
         direct.close()
 
 
+def _slot0_unwritten_tape():
+    """Slot 0 is never written, so the result is x itself: the root's
+    shortened tapes keep nothing ([0])."""
+    from frep.tape import Tape
+    words = np.array([2, 0 | (4 << 8), 3 | (5 << 8) | (1 << 16) | (4 << 24)], dtype=np.uint32)
+    return Tape(words=words, consts=np.array([0.25], dtype=np.float32), slot_count=6)
+
+
+ROOT_CASES = CASES + [("edge_x", [8, 4]), ("spheres8", [4, 8])]
+
+
+def test_root_pass_kernel_equals_per_cell_kernel(oracle):
+    """Level 0 evaluated in dependency passes over the tape's instructions
+    (one wavefront per root cell; FREP_ROOT_OPS=2 makes its use mandatory)
+    gives the same tree as the per-cell kernel (FREP_ROOT_OPS=0) and as the
+    oracle, across root sizes 64 to 4096 cells, a non power-of-two root,
+    every op type at the root (trig), and a result that is an input."""
+    import os
+    ctxs = {}
+    for mode in ("0", "2"):
+        os.environ["FREP_ROOT_OPS"] = mode
+        try:
+            ctxs[mode] = lib.Context(0)
+        finally:
+            del os.environ["FREP_ROOT_OPS"]
+    tapes = dict(TAPES, edge_x=_slot0_unwritten_tape)
+    try:
+        for name, dims in ROOT_CASES:
+            tape = tapes[name]()
+            grid = shapes.make_grid(dims, tape_time=0.5 if name == "trig" else 0.0)
+            got = {}
+            for mode, c in ctxs.items():
+                th = c.upload(tape)
+                tree = c.voxelize(th, grid)
+                got[mode] = c.export(tree)
+                c.tree_free(tree)
+            want = oracle.Tree(tape, grid).canonical()
+            assert np.array_equal(got["2"], want), (name, dims)
+            assert np.array_equal(got["0"], want), (name, dims)
+    finally:
+        for c in ctxs.values():
+            c.close()
+
+
 def test_wide_format_tape_vs_oracle(ctx, oracle):
     """A tape with more than 256 constants (format 2: 16-bit constant index),
     as in the batched mix: tree, densities and a render equal the oracle's."""

@@ -33,61 +33,190 @@ uint32_t maxs_for(uint32_t slot_count) {
     return m;
 }
 
-bool build_root_schedule(const uint32_t* w, std::vector<uint32_t>* out, TapeObj* t) {
+// Values: 0..3 = x, y, z, t; 4 + i = result of instruction i.  Instructions
+// the result does not depend on are left out.  Each instruction is placed
+// as late as its readers allow (level = longest chain to the result counted
+// back from the deepest), which keeps values short-lived; a level of more
+// than lpc instructions takes several passes.  Value slots are reused by
+// values first written after the previous value's last read; constants
+// come from a table instead of slots.
+bool build_root_schedule(const uint32_t* w, uint32_t lpc, size_t lds_per_block,
+                         std::vector<uint32_t>* out, RootSched* rs) {
     const uint32_t n = w[0];
     if (n == 0 || n > kRootMaxInstr) return false;
     int32_t cur[256];
     for (uint32_t s = 0; s < 256; ++s) cur[s] = s < 4 ? (int32_t)s : -1;
-    std::vector<uint32_t> va(n, 0), vb(n, 0), depth(4 + n, 0);
-    std::vector<uint32_t> consts;
-    uint32_t max_depth = 0;
+    std::vector<uint32_t> va(n, 0), vb(n, 0);
+    std::vector<uint8_t> is_const(n, 0);
     for (uint32_t i = 0; i < n; ++i) {
         const uint32_t x = w[1 + i];
         const uint32_t op = x & 0xFFu, o = (x >> 8) & 0xFFu, a = (x >> 16) & 0xFFu, b = x >> 24;
         if (op == OP_CONST) {
-            consts.push_back(i | ((a | (b << 8)) << 16));
+            is_const[i] = 1;
         } else {
             const bool binary = op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV ||
                                 op == OP_MIN || op == OP_MAX;
             if (cur[a] < 0 || (binary && cur[b] < 0)) return false;
             va[i] = (uint32_t)cur[a];
             vb[i] = binary ? (uint32_t)cur[b] : va[i];
-            depth[4 + i] = 1 + std::max(depth[va[i]], depth[vb[i]]);
-            max_depth = std::max(max_depth, depth[4 + i]);
         }
         cur[o] = (int32_t)(4 + i);
     }
-    // passes: instructions of depth d in tape order, at most 64 per pass
-    std::vector<std::vector<uint32_t>> by_depth(max_depth + 1);
-    for (uint32_t i = 0; i < n; ++i)
-        if (depth[4 + i] > 0) by_depth[depth[4 + i]].push_back(i);
-    std::vector<uint32_t> offs{0}, ent;
-    for (uint32_t d = 1; d <= max_depth; ++d)
-        for (size_t k = 0; k < by_depth[d].size(); k += 64) {
-            const size_t e = std::min(by_depth[d].size(), k + 64);
-            for (size_t q = k; q < e; ++q) {
-                const uint32_t i = by_depth[d][q];
-                ent.push_back((w[1 + i] & 0xFFu) | (i << 8));
-                ent.push_back(va[i] | (vb[i] << 16));
-            }
-            offs.push_back(offs.back() + (uint32_t)(e - k));
+    const uint32_t res = (uint32_t)cur[0];
+    auto is_op = [&](uint32_t v) { return v >= 4 && !is_const[v - 4]; };
+    // instructions the result depends on (operands precede their readers)
+    std::vector<uint8_t> need(n, 0);
+    if (res >= 4) need[res - 4] = 1;
+    for (uint32_t i = n; i-- > 0;)
+        if (need[i] && !is_const[i]) {
+            if (va[i] >= 4) need[va[i] - 4] = 1;
+            if (vb[i] >= 4) need[vb[i] - 4] = 1;
         }
-    const uint32_t n_pass = (uint32_t)offs.size() - 1;
-    if (2 * n_pass > n) return false;
-    const uint32_t nv = 4 + n;
-    const uint32_t cell_bytes = (8 * nv + nv + n + 15u) & ~15u;
-    if (root_cells_per_block(cell_bytes) == 0) return false;
-    out->assign(offs.begin(), offs.end());
-    t->root_c0 = (uint32_t)out->size();
+    std::vector<uint32_t> depth(n, 0), height(n, 0);
+    uint32_t D = 0;
+    for (uint32_t i = 0; i < n; ++i)
+        if (need[i] && !is_const[i]) {
+            const uint32_t da = is_op(va[i]) ? depth[va[i] - 4] : 0u;
+            const uint32_t db = is_op(vb[i]) ? depth[vb[i] - 4] : 0u;
+            depth[i] = 1 + std::max(da, db);
+            D = std::max(D, depth[i]);
+        }
+    for (uint32_t i = n; i-- > 0;)
+        if (need[i] && !is_const[i])
+            for (const uint32_t v : {va[i], vb[i]})
+                if (is_op(v)) height[v - 4] = std::max(height[v - 4], height[i] + 1);
+    std::vector<std::vector<uint32_t>> by_level(D + 1);
+    for (uint32_t i = 0; i < n; ++i)
+        if (need[i] && !is_const[i]) by_level[D - height[i]].push_back(i);
+    std::vector<std::vector<uint32_t>> passes;
+    for (uint32_t L = 1; L <= D; ++L)
+        for (size_t k = 0; k < by_level[L].size(); k += lpc)
+            passes.emplace_back(by_level[L].begin() + k,
+                                by_level[L].begin() + std::min(by_level[L].size(), k + lpc));
+    if (2 * passes.size() > n) return false;
+    while (passes.size() % 4) passes.emplace_back();   // empty passes: the kernel runs 4 at a time
+    const uint32_t n_pass = (uint32_t)passes.size();
+    // last pass reading each value; the result is read after the last pass
+    std::vector<int32_t> last(n, -1);
+    for (uint32_t p = 0; p < n_pass; ++p)
+        for (const uint32_t i : passes[p])
+            for (const uint32_t v : {va[i], vb[i]})
+                if (is_op(v)) last[v - 4] = std::max(last[v - 4], (int32_t)p);
+    if (is_op(res)) last[res - 4] = (int32_t)n_pass;
+    std::vector<uint32_t> slot(n, 0), free_slots;
+    std::vector<std::pair<int32_t, uint32_t>> active, still;   // (last read pass, slot)
+    uint32_t n_slot = 4;
+    for (uint32_t p = 0; p < n_pass; ++p) {
+        still.clear();
+        for (const auto& ls : active) {
+            if (ls.first < (int32_t)p) free_slots.push_back(ls.second);
+            else still.push_back(ls);
+        }
+        active.swap(still);
+        for (const uint32_t i : passes[p]) {
+            uint32_t sl;
+            if (!free_slots.empty()) {
+                sl = free_slots.back();
+                free_slots.pop_back();
+            } else {
+                sl = n_slot++;
+            }
+            slot[i] = sl;
+            active.emplace_back(std::max(last[i], (int32_t)p), sl);
+        }
+    }
+    if (n_slot > kRootMaxSlots + 1) return false;
+    std::vector<uint32_t> kidx(n, 0), consts;
+    for (uint32_t i = 0; i < n; ++i)
+        if (need[i] && is_const[i]) {
+            kidx[i] = (uint32_t)consts.size();
+            consts.push_back(i | (((w[1 + i] >> 16) & 0xFFFFu) << 16));
+        }
+    if (consts.size() >= kRootConst) return false;
+    auto enc = [&](uint32_t v) -> uint32_t {
+        if (v < 4) return v;
+        return is_const[v - 4] ? (kRootConst | kidx[v - 4]) : slot[v - 4];
+    };
+    const uint32_t nk = (uint32_t)consts.size();
+    const uint32_t table_bytes = (10 * nk + 15u) & ~15u;
+    const uint32_t cell_bytes = (8 * n_slot + ((n + 3u) & ~3u) + n_slot + 15u) & ~15u;
+    const uint32_t cpb = root_cells_per_block(table_bytes, cell_bytes, lds_per_block);
+    if (cpb == 0) return false;
+    out->clear();
+    rs->c0 = 0;
     out->insert(out->end(), consts.begin(), consts.end());
+    rs->m0 = (uint32_t)out->size();
+    for (const auto& ps : passes) {
+        uint32_t m = 0;
+        for (const uint32_t i : ps) m |= 1u << (w[1 + i] & 0xFu);
+        out->push_back(m);
+    }
     if (out->size() & 1u) out->push_back(0);
-    t->root_e0 = (uint32_t)out->size();
-    out->insert(out->end(), ent.begin(), ent.end());
-    t->root_pass = n_pass;
-    t->root_const = (uint32_t)consts.size();
-    t->root_res = (uint32_t)cur[0];
-    t->root_cell_bytes = cell_bytes;
+    rs->e0 = (uint32_t)out->size();
+    for (const auto& ps : passes) {
+        for (const uint32_t i : ps) {
+            out->push_back((w[1 + i] & 0xFu) | (slot[i] << 4) | (i << 16));
+            out->push_back(enc(va[i]) | (enc(vb[i]) << 16));
+        }
+        for (size_t k = ps.size(); k < lpc; ++k) {
+            out->push_back(kRootNop);
+            out->push_back(0);
+        }
+    }
+    rs->ok = true;
+    rs->lpc = lpc;
+    rs->pass = n_pass;
+    rs->nconst = nk;
+    rs->res = enc(res);
+    rs->slots = n_slot;
+    rs->table_bytes = table_bytes;
+    rs->cell_bytes = cell_bytes;
+    rs->cpb = cpb;
     return true;
+}
+
+RootArgs root_args(const TapeObj* tp, const TreeObj* t, const RootSched& rs, uint32_t mm_stride,
+                   uint32_t cells_per_block) {
+    RootArgs r{};
+    r.a = level_args(t, 0, mm_stride);
+    r.sched = tp->d_sched + rs.p0;
+    r.tape = tp->d_words;
+    r.n = tp->words[0];
+    r.n_pass = rs.pass;
+    r.n_const = rs.nconst;
+    r.c0 = rs.c0;
+    r.m0 = rs.m0;
+    r.e0 = rs.e0;
+    r.n_slot = rs.slots;
+    r.res = rs.res;
+    r.table_bytes = rs.table_bytes;
+    r.cell_bytes = rs.cell_bytes;
+    r.cells_per_block = cells_per_block;
+    r.lanes_per_cell = rs.lpc;
+    return r;
+}
+
+// A pass is bound by one wavefront's latency, and about 4 wavefronts per
+// SIMD overlap it for free: cost = passes x rounds of 4 wavefronts per SIMD.
+double root_cost(const RootSched& rs, uint32_t cells, uint32_t cu_count) {
+    const double slots = 16.0 * std::max<uint32_t>(cu_count, 1);
+    const double waves = (double)cells * rs.lpc / 64.0;
+    return (double)rs.pass * std::max(1.0, waves / slots);
+}
+
+int pick_root_schedule(const TapeObj* tp, uint32_t cells, uint32_t cu_count) {
+    int best = -1;
+    double best_cost = 0.0;
+    for (uint32_t v = 0; v < kRootVariants; ++v) {
+        const RootSched& rs = tp->root[v];
+        if (!rs.ok) continue;
+        const double cost = root_cost(rs, cells, cu_count);
+        if (best < 0 || cost < best_cost) {
+            best = (int)v;
+            best_cost = cost;
+        }
+    }
+    return best;
 }
 
 frep_status fail(frep_ctx* c, frep_status s, const std::string& msg) {
@@ -256,6 +385,12 @@ frep_status frep_init(int device, frep_ctx** out) {
     hipDeviceProp_t prop;
     HIP_TRY(hipGetDeviceProperties(&prop, device));
     c->cu_count = (uint32_t)prop.multiProcessorCount;
+    {
+        int lds = 0;
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) ==
+                hipSuccess && lds > 0)
+            c->lds_per_block = std::min<size_t>((size_t)lds, 160 * 1024);
+    }
     // persistent level kernels: 8 blocks of 4 wavefronts per CU
     c->level_grid = c->cu_count * 8;
     const char* na = std::getenv("FREP_NO_ASM");
@@ -363,10 +498,23 @@ frep_status frep_tape_upload(frep_ctx* ctx, const uint32_t* words, uint32_t n_wo
     HIP_TRY(hipMalloc(&t->d_consts, (size_t)(n_consts + kPoolPad) * 4));
     if (n_consts)
         HIP_TRY(hipMemcpy(t->d_consts, consts, (size_t)n_consts * 4, hipMemcpyHostToDevice));
-    std::vector<uint32_t> sched;
-    if (build_root_schedule(words, &sched, t.get())) {
-        HIP_TRY(hipMalloc(&t->d_sched, sched.size() * 4));
-        HIP_TRY(hipMemcpy(t->d_sched, sched.data(), sched.size() * 4, hipMemcpyHostToDevice));
+    {
+        std::vector<uint32_t> all, one;
+        bool any = false;
+        for (uint32_t v = 0; v < kRootVariants; ++v) {
+            if (!build_root_schedule(words, kRootLanes[v], ctx->lds_per_block, &one, &t->root[v]))
+                continue;
+            any = true;
+            if (all.size() & 1u) all.push_back(0);     // entries stay 8-byte aligned
+            t->root[v].p0 = (uint32_t)all.size();
+            all.insert(all.end(), one.begin(), one.end());
+        }
+        if (any) {
+            if (all.empty()) all.push_back(0);         // no passes: the result is an input
+
+            HIP_TRY(hipMalloc(&t->d_sched, all.size() * 4));
+            HIP_TRY(hipMemcpy(t->d_sched, all.data(), all.size() * 4, hipMemcpyHostToDevice));
+        }
     }
     ctx->tapes.push_back(std::move(t));
     *out = (frep_tape_t)ctx->tapes.size();
@@ -584,10 +732,20 @@ constexpr uint32_t kTuneBlocksPerCu[] = {8, 4, 2};   // candidates, 4 wavefronts
 constexpr uint32_t kTuneTrials = sizeof(kTuneBlocksPerCu) / sizeof(kTuneBlocksPerCu[0]);
 constexpr size_t kTuneMaxEntries = 64;
 
+// A batched root launch uses one schedule variant for all its trees (one
+// block shape): trees with both variants, the variant of least total cost.
+bool root_job(const BatchPlan& P, const Job& j) {
+    if (!P.root_ops || j.t->grid.n_levels < 2) return false;
+    for (uint32_t v = 0; v < kRootVariants; ++v)
+        if (!j.tp->root[v].ok) return false;
+    return true;
+}
+
 BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
-                     uint32_t lanes) {
+                     uint32_t lanes, bool root_ops, uint32_t cu_count) {
     BatchPlan P;
     P.m = (uint32_t)idx.size();
+    P.root_ops = root_ops;
     // group boundaries: lane cuts (long tapes first), then chunks of kBatchMax
     std::vector<std::pair<uint32_t, uint32_t>> spans;    // (end, lane)
     if (lanes > 1 && P.m >= 8) {
@@ -616,12 +774,50 @@ BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& 
         for (uint32_t k = 0; k < G.count; ++k)
             max_nl = std::max(max_nl, jobs[idx[G.first + k]].t->grid.n_levels);
         for (uint32_t L = 0; L < max_nl; ++L) {
+            if (L == 0) {
+                BatchLaunch la{g, 0, false, 0, 8, 1, 0};
+                la.root = true;
+                la.cpb = 16;
+                double cost[kRootVariants] = {};
+                for (uint32_t k = 0; k < G.count; ++k) {
+                    const Job& j = jobs[idx[G.first + k]];
+                    if (!root_job(P, j)) continue;
+                    ++la.count;
+                    la.mm = std::max(la.mm, j.mm_stride);
+                    const uint32_t d = j.t->grid.dims[0];
+                    for (uint32_t v = 0; v < kRootVariants; ++v)
+                        cost[v] += root_cost(j.tp->root[v], d * d * d, cu_count);
+                }
+                for (uint32_t v = 1; v < kRootVariants; ++v)
+                    if (cost[v] < cost[la.variant]) la.variant = v;
+                for (uint32_t k = 0; k < G.count; ++k) {
+                    const Job& j = jobs[idx[G.first + k]];
+                    if (root_job(P, j)) la.cpb = std::min(la.cpb, j.tp->root[la.variant].cpb);
+                }
+                if (la.count) {
+                    for (uint32_t k = 0; k < G.count; ++k) {
+                        const Job& j = jobs[idx[G.first + k]];
+                        if (!root_job(P, j)) continue;
+                        const uint32_t d = j.t->grid.dims[0];
+                        la.blocks += d * d * d / la.cpb;
+                        const RootSched& rs = j.tp->root[la.variant];
+                        la.lds = std::max(la.lds, (size_t)rs.table_bytes +
+                                                      (size_t)la.cpb * rs.cell_bytes);
+                    }
+                    la.off = P.bytes;
+                    P.bytes = align16(P.bytes + la.count * sizeof(RootArgs));
+                    la.starts_off = P.bytes;
+                    P.bytes = align16(P.bytes + (la.count + 1) * sizeof(uint32_t));
+                    P.launches.push_back(la);
+                }
+            }
             for (int pt = 0; pt < 2; ++pt) {
                 BatchLaunch la{g, L, pt == 1, 0, 8, 1, P.bytes};
                 for (uint32_t k = 0; k < G.count; ++k) {
                     const Job& j = jobs[idx[G.first + k]];
                     const uint32_t nl = j.t->grid.n_levels;
                     if (L >= nl || (L + 1 == nl) != (pt == 1)) continue;
+                    if (L == 0 && root_job(P, j)) continue;
                     ++la.count;
                     la.maxs = std::max(la.maxs, j.t->maxs);
                     la.mm = std::max(la.mm, j.mm_stride);
@@ -650,12 +846,28 @@ void fill_batch(const BatchPlan& P, const std::vector<Job>& jobs, const std::vec
     }
     for (const BatchLaunch& la : P.launches) {
         const BatchGroup& G = P.groups[la.group];
+        if (la.root) {
+            RootArgs* ra = reinterpret_cast<RootArgs*>(hs + la.off);
+            uint32_t* starts = reinterpret_cast<uint32_t*>(hs + la.starts_off);
+            uint32_t c = 0, b = 0;
+            starts[0] = 0;
+            for (uint32_t k = 0; k < G.count; ++k) {
+                const Job& j = jobs[idx[G.first + k]];
+                if (!root_job(P, j)) continue;
+                ra[c] = root_args(j.tp, j.t, j.tp->root[la.variant], la.mm, la.cpb);
+                const uint32_t d = j.t->grid.dims[0];
+                b += d * d * d / la.cpb;
+                starts[++c] = b;
+            }
+            continue;
+        }
         LevelArgs* out = reinterpret_cast<LevelArgs*>(hs + la.off);
         uint32_t c = 0;
         for (uint32_t k = 0; k < G.count; ++k) {
             const Job& j = jobs[idx[G.first + k]];
             const uint32_t nl = j.t->grid.n_levels;
             if (la.level >= nl || (la.level + 1 == nl) != la.point) continue;
+            if (la.level == 0 && root_job(P, j)) continue;
             LevelArgs a = level_args(j.t, la.level, la.mm);
             if (la.point) {
                 a.pk = point_k_used(la.maxs, a.dim);
@@ -675,7 +887,8 @@ uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) 
 uint64_t batch_key(const BatchPlan& P, const uint8_t* hs) {
     uint64_t h = fnv1a(hs, P.bytes);
     for (const BatchLaunch& la : P.launches) {
-        const uint32_t v[6] = {la.group, la.level, la.point ? 1u : 0u, la.count, la.maxs, la.mm};
+        const uint32_t v[8] = {la.group, la.level, la.point ? 1u : 0u, la.count, la.maxs, la.mm,
+                               la.root ? la.cpb : 0u, la.variant};
         h = fnv1a(v, sizeof(v), h);
     }
     return fnv1a(&P.m, sizeof(P.m), h);
@@ -748,7 +961,12 @@ frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const Ba
         for (; li < P.launches.size() && P.launches[li].group == g; ++li) {
             const BatchLaunch& la = P.launches[li];
             BatchArgs b{reinterpret_cast<const LevelArgs*>(ds + la.off), la.count, la.mm};
-            if (la.point)
+            if (la.root)
+                HIP_TRY(launch_root_interval_batch(
+                    gs, reinterpret_cast<const RootArgs*>(ds + la.off),
+                    reinterpret_cast<const uint32_t*>(ds + la.starts_off), la.count, la.blocks,
+                    la.cpb, kRootLanes[la.variant], la.lds));
+            else if (la.point)
                 HIP_TRY(launch_level_point_batch(gs, b, la.maxs, ctx->cu_count, grid[li]));
             else
                 HIP_TRY(launch_level_interval_batch(gs, b, la.maxs, ctx->cu_count, grid[li]));
@@ -789,7 +1007,11 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
     std::stable_sort(idx.begin(), idx.end(), [&](uint32_t x, uint32_t y) {
         return jobs[x].tp->words[0] > jobs[y].tp->words[0];
     });
-    const BatchPlan P = plan_batch(jobs, idx, kBatchLanes);
+    if (ctx->root_ops_required)
+        for (const uint32_t k : idx)
+            if (jobs[k].t->grid.n_levels > 1 && !root_job(BatchPlan{0, true}, jobs[k]))
+                return fail(ctx, FREP_UNSUPPORTED, "tape has no level-0 schedule (FREP_ROOT_OPS=2)");
+    const BatchPlan P = plan_batch(jobs, idx, kBatchLanes, ctx->root_ops, ctx->cu_count);
     if (P.m == 0) return FREP_OK;
     BatchStage* S = &ctx->stage;
     if (S->stage_bytes < P.bytes || S->cnt_words < (size_t)P.m * CNT_WORDS) {
@@ -959,22 +1181,15 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         for (uint32_t L = 0; L < nl; ++L) la[L] = level_args(t, L, j.mm_stride);
         // level 0 in passes over the tape's instructions when the tape has
         // a schedule (k_root_interval)
-        const bool use_root = ctx->root_ops && tp->d_sched && nl > 1;
+        const uint32_t d0 = grid->dims[0];
+        const int rv = (ctx->root_ops && tp->d_sched && nl > 1)
+                           ? pick_root_schedule(tp, d0 * d0 * d0, ctx->cu_count)
+                           : -1;
+        const bool use_root = rv >= 0;
         if (ctx->root_ops_required && nl > 1 && !use_root)
             return fail(ctx, FREP_UNSUPPORTED, "tape has no level-0 schedule (FREP_ROOT_OPS=2)");
         RootArgs ra{};
-        if (use_root) {
-            ra.a = la[0];
-            ra.sched = tp->d_sched;
-            ra.tape = tp->d_words;
-            ra.n_pass = tp->root_pass;
-            ra.n_const = tp->root_const;
-            ra.c0 = tp->root_c0;
-            ra.e0 = tp->root_e0;
-            ra.nv = 4 + tp->words[0];
-            ra.res = tp->root_res;
-            ra.cell_bytes = tp->root_cell_bytes;
-        }
+        if (use_root) ra = root_args(tp, t, tp->root[rv], j.mm_stride, tp->root[rv].cpb);
         // init, the level launches and the counter readback; with timing on,
         // an event before the init and after each launch
         auto enqueue = [&](bool events) -> frep_status {
@@ -1007,7 +1222,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             key = fnv1a(la, sizeof(LevelArgs) * nl);
             const uintptr_t v[9] = {(uintptr_t)tp->d_words, (uintptr_t)t->h_counters, nl,
                                     t->maxs, t->slot_count, t->asm_ok ? 1u : 0u,
-                                    ctx->cu_count, ctx->level_grid, use_root ? 1u : 0u};
+                                    ctx->cu_count, ctx->level_grid, (uintptr_t)(rv + 1)};
             key = fnv1a(v, sizeof(v), key);
         }
         const bool replay = key != 0 && ((t->vexec && t->vkey == key) || t->vkey_seen == key);

@@ -20,6 +20,18 @@ constexpr uint32_t kRootOff = 16;      // words of slack before the root tape
 constexpr uint32_t kPoolPad = 64;      // words of slack after the pool end
 constexpr uint64_t kMaxPoolWords = 0xFFFFFF00ull;
 
+// One level-0 schedule of a tape (k_root_interval; RootArgs describes the
+// layout).  Offsets are words from p0, the schedule's start in d_sched.
+struct RootSched {
+    bool ok = false;
+    uint32_t lpc = 0;                  // lanes per cell: the largest pass
+    uint32_t p0 = 0;
+    uint32_t pass = 0, nconst = 0, c0 = 0, m0 = 0, e0 = 0, res = 0, slots = 0;
+    uint32_t table_bytes = 0, cell_bytes = 0, cpb = 0;
+};
+constexpr uint32_t kRootVariants = 2;
+constexpr uint32_t kRootLanes[kRootVariants] = {64, 16};
+
 struct TapeObj {
     std::vector<uint32_t> words;       // [n, instrs]
     std::vector<float> consts;
@@ -28,11 +40,10 @@ struct TapeObj {
     uint32_t* d_words = nullptr;
     float* d_consts = nullptr;
     uint64_t uid = 0;                  // unique per upload in its context
-    // level-0 schedule of k_root_interval (null when the tape does not
-    // qualify; RootArgs describes the layout)
+    // level-0 schedules of k_root_interval with 64 and 16 lanes per cell
+    // (d_sched null when neither qualifies)
     uint32_t* d_sched = nullptr;
-    uint32_t root_pass = 0, root_const = 0, root_c0 = 0, root_e0 = 0, root_res = 0;
-    uint32_t root_cell_bytes = 0;
+    RootSched root[kRootVariants];
 };
 
 struct TreeObj {
@@ -133,7 +144,13 @@ struct BatchLaunch {
     uint32_t group, level;
     bool point;
     uint32_t count, maxs, mm;
-    size_t off;                        // byte offset of its LevelArgs table
+    size_t off;                        // byte offset of its LevelArgs (RootArgs) table
+    // root launch (level 0 of the trees whose tape has a schedule)
+    bool root = false;
+    uint32_t cpb = 0, blocks = 0;      // cells per block (all trees), total blocks
+    uint32_t variant = 0;              // schedule variant (lanes per cell kRootLanes[variant])
+    size_t lds = 0;                    // dynamic LDS per block
+    size_t starts_off = 0;             // byte offset of the block starts (count + 1 words)
 };
 struct BatchGroup {
     uint32_t first, count;
@@ -151,6 +168,7 @@ struct LaneSet {
 };
 struct BatchPlan {
     uint32_t m = 0;
+    bool root_ops = false;             // level 0 by k_root_interval where the tape has a schedule
     std::vector<BatchGroup> groups;
     std::vector<BatchLaunch> launches;
     size_t bytes = 0;
@@ -172,6 +190,7 @@ struct frep_ctx {
     bool graphs = true;                // FREP_NO_GRAPH=1: plain voxelization launched directly
     bool root_ops = true;              // FREP_ROOT_OPS=0: level 0 by the per-cell kernel
     bool root_ops_required = false;    // FREP_ROOT_OPS=2: a tape without a level-0 schedule is an error
+    size_t lds_per_block = 65536;      // device limit (hipDeviceAttributeMaxSharedMemoryPerBlock)
     uint64_t next_uid = 1;
     hipEvent_t ev[frep::kMaxLevels + 4] = {};
     frep_timing last{};
@@ -204,12 +223,20 @@ struct frep_ctx {
 namespace frep {
 
 uint32_t maxs_for(uint32_t slot_count);
-// Level-0 schedule of k_root_interval for tape words w ([n, instrs]):
-// fills out and the root_* fields of t, or returns false when the tape reads
-// a slot above 3 before writing it, has more than kRootMaxInstr
-// instructions, would average fewer than two instructions per pass, or
-// needs more LDS per cell than 8 cells per block allow.
-bool build_root_schedule(const uint32_t* w, std::vector<uint32_t>* out, TapeObj* t);
+// Level-0 schedule of k_root_interval for tape words w ([n, instrs]) with
+// passes of at most lpc instructions: fills out and rs (offsets from the
+// start of out), or returns false when the tape reads a slot above 3 before
+// writing it, has more than kRootMaxInstr instructions, would average fewer
+// than two instructions per pass, or needs more LDS than 8 cells per block
+// allow.
+bool build_root_schedule(const uint32_t* w, uint32_t lpc, size_t lds_per_block,
+                         std::vector<uint32_t>* out, RootSched* rs);
+// Arguments of k_root_interval for level 0 of tree t with schedule rs of tp.
+RootArgs root_args(const TapeObj* tp, const TreeObj* t, const RootSched& rs, uint32_t mm_stride,
+                   uint32_t cells_per_block);
+// The schedule of tp for a root of `cells` cells with the lower modelled
+// cost (passes x rounds of wavefronts over the GPU's SIMDs), or -1.
+int pick_root_schedule(const TapeObj* tp, uint32_t cells, uint32_t cu_count);
 frep_status fail(frep_ctx* c, frep_status s, const std::string& msg);
 TapeObj* get_tape(frep_ctx* ctx, frep_tape_t h);
 TreeObj* get_tree(frep_ctx* ctx, frep_tree_t h);
@@ -236,7 +263,8 @@ LevelArgs level_args(const TreeObj* t, uint32_t L, uint32_t mm_stride);
 // eighth of them, the next quarter and the rest form separate groups on
 // lanes 0, 1, 2 (a job count below 8 keeps one lane).
 BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
-                     uint32_t lanes = 1);
+                     uint32_t lanes = 1, bool root_ops = false, uint32_t cu_count = 256);
+double root_cost(const RootSched& rs, uint32_t cells, uint32_t cu_count);
 void fill_batch(const BatchPlan& P, const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
                 uint8_t* hs);
 frep_status ensure_stage(frep_ctx* ctx, BatchStage* S, const BatchPlan& P);

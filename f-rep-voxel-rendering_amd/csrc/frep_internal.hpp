@@ -77,25 +77,36 @@ struct LevelArgs {
 
 // Root level evaluated in passes over the tape's instructions
 // (k_root_interval).  sched holds, in u32 words:
-//   [0 .. n_pass]          entry offset of each pass, then the total
-//   [c0 .. c0 + n_const)   CONST instructions: i | const index << 16
-//   [e0 ..)                two words per instruction: op | i << 8, va | vb << 16
-// Values: 0..3 = x, y, z, t; 4 + i = result of instruction i.
+//   [c0 .. c0 + n_const)   constants used: instruction i | pool index << 16
+//   [m0 .. m0 + n_pass)    op types of each pass (bit op set when present)
+//   [e0 ..)                n_pass passes (a multiple of 4) of lanes_per_cell
+//                          entries, two words each: op | out slot << 4 | i << 16,
+//                          operand A | operand B << 16; op kRootNop pads a
+//                          short pass
+// An operand is a value slot (0..3 hold x, y, z, t), or kRootConst | k for
+// the k-th constant.  Slots are reused once a value's last reader has run.
 struct RootArgs {
     LevelArgs a;              // level 0 of the tree
     const uint32_t* sched;
     const uint32_t* tape;     // [n, instrs] of the root tape
+    uint32_t n;               // instructions
     uint32_t n_pass;
     uint32_t n_const;
     uint32_t c0;
+    uint32_t m0;
     uint32_t e0;              // even (entries are read as 8-byte pairs)
-    uint32_t nv;              // 4 + n
-    uint32_t res;             // value in slot 0 at the end of the tape
-    uint32_t cell_bytes;      // LDS per cell: 8 nv (values) + nv (live) + n (choices), 16-aligned
-    uint32_t pad;
+    uint32_t n_slot;
+    uint32_t res;             // operand holding the result (slot 0 at the end of the tape)
+    uint32_t table_bytes;     // LDS of the constant table: 8 n_const (values) + 2 n_const (instrs), 16-aligned
+    uint32_t cell_bytes;      // LDS per cell: 8 n_slot (values) + n rounded to 4 (flags) + n_slot (live), 16-aligned
+    uint32_t cells_per_block; // 16 or 8 (root_cells_per_block)
+    uint32_t lanes_per_cell;  // 64 or 16: the schedule's largest pass
 };
-constexpr uint32_t kRootMaxInstr = 1024;       // schedule limit (16-bit value ids, LDS)
-constexpr size_t kRootMaxLds = 60 * 1024;      // dynamic LDS per block
+constexpr uint32_t kRootConst = 0x8000u;
+constexpr uint32_t kRootNop = 0xFu;
+constexpr uint32_t kRootMaxInstr = 65535;      // 16-bit instruction and constant ids
+constexpr uint32_t kRootMaxSlots = 4095;       // 12-bit slot ids
+constexpr size_t kRootStaticLds = 256;          // bound on the kernel's static LDS
 
 // One tree of a batch: what the init kernel needs.
 struct InitArgs {
@@ -161,10 +172,18 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
 // slots: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                               uint32_t slots, uint32_t cu_count, uint32_t max_grid, bool asm_ok);
-// Cells per block of k_root_interval for a per-cell LDS size: 16 or 8, or
-// 0 when not even 8 cells fit.
-uint32_t root_cells_per_block(uint32_t cell_bytes);
+// Cells per block of k_root_interval for its LDS sizes and the device's
+// LDS per block: 16 or 8, or 0 when not even 8 cells fit.
+uint32_t root_cells_per_block(uint32_t table_bytes, uint32_t cell_bytes, size_t lds_per_block);
 hipError_t launch_root_interval(hipStream_t st, const RootArgs& r);
+// Roots of n <= kBatchMax trees: d_args[j] (cells_per_block and
+// lanes_per_cell the same for all), tree j owning blocks d_starts[j] ..
+// d_starts[j+1]-1 of `blocks`; lds = the largest table_bytes +
+// cells_per_block * cell_bytes.
+hipError_t launch_root_interval_batch(hipStream_t st, const RootArgs* d_args,
+                                      const uint32_t* d_starts, uint32_t n, uint32_t blocks,
+                                      uint32_t cells_per_block, uint32_t lanes_per_cell,
+                                      size_t lds);
 // Batched forms: init of n trees, one level over many shapes, and a copy of
 // the n counter blocks into out (n * CNT_WORDS words) for one readback.
 // max_grid caps the persistent grid (the autotuner varies it).

@@ -412,52 +412,112 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
 // Root level, parallel over the tape's instructions.  The root is one node
 // of dim^3 cells evaluated with the full tape; the per-cell kernel above
 // gives it dim^3/64 wavefronts, each walking the whole tape one instruction
-// at a time.  Here one wavefront owns one cell and its lanes evaluate
-// independent instructions together (the same interval rules,
-// interval.comp:36-160, on the same operand values).
+// at a time.  Here a group of LPC lanes owns one cell (64/LPC cells per
+// wavefront) and evaluates up to LPC independent instructions together (the
+// same interval rules, interval.comp:36-160, on the same operand values).
 //
-// The host renames the tape's slots into values (0..3 = x, y, z, t; 4+i =
-// result of instruction i) and groups the non-CONST instructions into passes
-// by dependency depth, at most 64 per pass, so every operand of a pass was
-// written by an earlier pass; CONST values are written before the first.
-// Values, choices and liveness flags of a cell live in LDS.
+// The host renames the tape's slots into values and groups the
+// instructions the result depends on into passes of at most LPC, so that
+// every operand of a pass was written by an earlier pass (or is an input or
+// a constant).  Values live in LDS slots, reused once a value's last reader
+// has run; constants come from a table the block loads once.
 //
 // Shortening (shorten.comp:164-223) keeps instruction i iff slot out(i) is
-// active when the backward pass reaches i, which is iff a kept later
-// instruction reads, through its min/max choice, the value i wrote.  In the
-// renamed form that is reachability from the result value along chosen
-// operand edges: the passes in reverse order.  Kept instructions are then
-// written in tape order, 64 at a time, at the same place in the cell's
-// (n+1)-word region as the per-cell kernel writes them.
+// active when the backward pass reaches i: iff a kept later instruction
+// reads, through its min/max choice, the value i wrote.  Here that is the
+// same walk over the passes in reverse: a lane reads and clears the live
+// flag of its output slot (the slot's later values were handled in later
+// passes), and if it was set, marks the instruction kept and sets the flags
+// of its chosen operands.  Kept instructions are then written in tape
+// order, LPC at a time, at the same place in the cell's (n+1)-word region
+// as the per-cell kernel writes them.
 //
-// A block of C wavefronts owns C consecutive mask bits (a 16- or 8-bit
-// slice of a mask word, stored as such) and reserves the child ids and tape
-// regions of its ambiguous cells with one atomic.
+// A block of C cells owns C consecutive mask bits (a 16- or 8-bit slice of
+// a mask word, stored as such) and reserves the child ids and tape regions
+// of its ambiguous cells with one atomic.
+//
+// LDS: constant table {c, c}[n_const] then instruction ids u16[n_const];
+// per cell, values float2[n_slot], flags u8[n] (choice | kept << 2, rounded
+// to 4 bytes), live u8[n_slot].
 // ---------------------------------------------------------------------------
 template <int C>
-__device__ __forceinline__ void store_mask_slice(uint32_t* words, uint32_t slice, uint32_t bits) {
-    if constexpr (C == 16) ((uint16_t*)words)[slice] = (uint16_t)bits;
-    else ((uint8_t*)words)[slice] = (uint8_t)bits;
+__device__ __forceinline__ void store_mask_slice(__attribute__((address_space(1))) uint32_t* words,
+                                                 uint32_t slice, uint32_t bits) {
+    if constexpr (C == 16)
+        ((__attribute__((address_space(1))) uint16_t*)words)[slice] = (uint16_t)bits;
+    else
+        ((__attribute__((address_space(1))) uint8_t*)words)[slice] = (uint8_t)bits;
 }
 
-template <int C>
-__global__ __launch_bounds__(64 * C) void k_root_interval(RootArgs r) {
+// Block `blk` of one tree's root: cells blk*C .. blk*C + C - 1.
+template <int C, int LPC>
+__device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
     static_assert(C == 8 || C == 16, "mask slices are bytes or half-words");
+    static_assert(LPC == 16 || LPC == 64, "lanes per cell");
+    constexpr uint32_t G = 64 / LPC;                     // cells per wavefront
     extern __shared__ __align__(16) unsigned char root_lds[];
     __shared__ uint32_t blk_flags[C];
     __shared__ uint32_t blk_amb;
     __shared__ unsigned long long blk_old;
     const LevelArgs& a = r.a;
+    // global-typed views of the argument pointers: plain (not flat) memory
+    // instructions also when the arguments were read from memory
+    typedef __attribute__((address_space(1))) uint32_t gword;
+    const gword* gsched = (const gword*)r.sched;
+    const gword* gtape = (const gword*)r.tape;
+    const __attribute__((address_space(1))) float* gconsts =
+        (const __attribute__((address_space(1))) float*)a.consts;
     const uint32_t lane = lane_id();
-    const uint32_t wib = uni(threadIdx.x >> 6);
-    const uint32_t idx = blockIdx.x * C + wib;           // mask bit of this cell
-    const uint32_t n = r.nv - 4u;
-    float2* V = (float2*)(root_lds + (size_t)wib * r.cell_bytes);
-    uint8_t* live = (uint8_t*)(V + r.nv);
-    uint8_t* chs = live + r.nv;
+    const uint32_t sl = lane % LPC;                      // lane within the cell's group
+    const uint32_t gshift = lane - sl;                   // first lane of the group
+    const uint32_t cib = uni(threadIdx.x >> 6) * G + lane / LPC;   // cell within the block
+    const uint32_t idx = blk * C + cib;                  // mask bit of this cell
+    const uint32_t n = r.n;
+    float2* Kv = (float2*)root_lds;
+    uint16_t* Ki = (uint16_t*)(Kv + r.n_const);
+    unsigned char* cell = root_lds + r.table_bytes + (size_t)cib * r.cell_bytes;
+    float2* V = (float2*)cell;
+    uint8_t* flags = cell + 8u * r.n_slot;
+    uint8_t* live = flags + ((n + 3u) & ~3u);
+    auto opnd = [&](uint32_t v) -> float2 {
+        const float2* q = (v & kRootConst) ? Kv + (v & (kRootConst - 1u)) : V + v;
+        return *q;
+    };
+    auto mark = [&](uint32_t v) {
+        if (v & kRootConst) flags[Ki[v & (kRootConst - 1u)]] = 4u;
+        else live[v] = 1u;
+    };
+    // the group's bits of a ballot, at bit 0
+    auto group_bits = [&](uint64_t b) -> uint64_t {
+        return LPC == 64 ? b : (b >> gshift) & ((1ull << (LPC % 64)) - 1ull);
+    };
+
+    // constant table: kDeep entries per thread in flight (entry, then the
+    // constant it names)
+    constexpr uint32_t kDeep = 8;
+    for (uint32_t k0 = threadIdx.x; k0 < r.n_const; k0 += kDeep * C * LPC) {
+        uint32_t e[kDeep];
+        float c[kDeep];
+#pragma unroll
+        for (uint32_t u = 0; u < kDeep; ++u) {
+            const uint32_t k = k0 + u * C * LPC;
+            e[u] = gsched[r.c0 + (k < r.n_const ? k : r.n_const - 1u)];   // i | pool index << 16
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < kDeep; ++u)
+            c[u] = gconsts[e[u] >> 16];
+#pragma unroll
+        for (uint32_t u = 0; u < kDeep; ++u) {
+            const uint32_t k = k0 + u * C * LPC;
+            if (k < r.n_const) {
+                Kv[k] = make_float2(c[u], c[u]);
+                Ki[k] = (uint16_t)(e[u] & 0xFFFFu);
+            }
+        }
+    }
     const uint32_t dim = a.dim, W = dim * dim * dim / 32;
-    uint32_t* rec = a.nodes;
-    ctape_t hdr = as_const(rec);
+    gword* rec = (gword*)a.nodes;
+    ctape_t hdr = as_const(a.nodes);
     const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
     uint32_t cx, cy, cz;
     cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
@@ -466,45 +526,87 @@ __global__ __launch_bounds__(64 * C) void k_root_interval(RootArgs r) {
     const float lx = a.low[0] + (float)gx * a.cs;
     const float ly = a.low[1] + (float)gy * a.cs;
     const float lz = a.low[2] + (float)gz * a.cs;
+    for (uint32_t q = sl; q < (n + 3u) / 4u; q += LPC) ((uint32_t*)flags)[q] = 0u;
+    for (uint32_t v = sl; v < r.n_slot; v += LPC) live[v] = 0;
+    if (sl < 4) {
+        const float lo = sl == 0 ? lx : (sl == 1 ? ly : (sl == 2 ? lz : a.tape_time));
+        const float hi = sl == 3 ? a.tape_time : lo + a.cs;
+        V[sl] = make_float2(lo, hi);
+    }
+    __syncthreads();                                     // constant table complete
 
     // ---- forward pass ----
-    for (uint32_t v = lane; v < r.nv; v += 64) live[v] = 0;
-    if (lane < 4) {
-        const float lo = lane == 0 ? lx : (lane == 1 ? ly : (lane == 2 ? lz : a.tape_time));
-        const float hi = lane == 3 ? a.tape_time : lo + a.cs;
-        V[lane] = make_float2(lo, hi);
-    }
-    for (uint32_t k = lane; k < r.n_const; k += 64) {
-        const uint32_t e = r.sched[r.c0 + k];            // i | const index << 16
-        const float c = a.consts[e >> 16];
-        V[4 + (e & 0xFFFFu)] = make_float2(c, c);
-    }
-    const uint2* ent = (const uint2*)(r.sched + r.e0);  // {op | i << 8, va | vb << 16}
-    for (uint32_t p = 0; p < r.n_pass; ++p) {
-        const uint32_t b0 = r.sched[p], cnt = r.sched[p + 1] - b0;
-        if (lane < cnt) {
-            const uint2 e = ent[b0 + lane];
-            const uint32_t op = e.x & 0xFFu, i = e.x >> 8;
-            const float2 av = V[e.y & 0xFFFFu], bv = V[e.y >> 16];
-            const Itv A{av.x, av.y}, B{bv.x, bv.y};
-            Itv res;
-            if (op == OP_ADD) {
-                res = Itv{A.lo + B.lo, A.hi + B.hi};
-            } else if (op == OP_SUB) {
-                res = Itv{A.lo - B.hi, A.hi - B.lo};
-            } else if (op == OP_MUL) {
-                res = imul(A, B);
-            } else if (op == OP_MIN || op == OP_MAX) {
-                res = op == OP_MIN ? Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)}
-                                   : Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
-                chs[i] = (uint8_t)mm_choice(op, A, B);
-            } else {
-                res = interval_op(op, A, B, 0.0f);
+    // Pass p's entry of this lane is ent[p * LPC] (kRootNop where the pass
+    // has fewer instructions).  The pass count is a multiple of 4 and each
+    // entry is loaded 3 passes ahead, from an index clamped to the last pass
+    // (never a conditional address).
+    typedef const __attribute__((address_space(1))) unsigned long long gpair;
+    gpair* ent = (gpair*)(gsched + r.e0) + sl;  // {op | out slot << 4 | i << 16, A | B << 16}
+    const uint32_t P = r.n_pass;
+    auto ld = [&](uint32_t p) -> uint2 {
+        const unsigned long long v = ent[(p < P ? p : P - 1u) * LPC];
+        return make_uint2((uint32_t)v, (uint32_t)(v >> 32));
+    };
+    auto forward = [&](const uint2 e, uint32_t p) {
+        // op types of the pass (bit op): the common ones are evaluated for
+        // every lane under a uniform test and selected; others take the
+        // general rule on their own lanes
+        const uint32_t m = gsched[r.m0 + p];
+        const uint32_t op = e.x & 0xFu, so = (e.x >> 4) & 0xFFFu, i = e.x >> 16;
+        const float2 av = opnd(e.y & 0xFFFFu), bv = opnd(e.y >> 16);
+        const Itv A{av.x, av.y}, B{bv.x, bv.y};
+        Itv res = A;                                     // COPY
+        if (m & (1u << OP_ADD)) {
+            const Itv t{A.lo + B.lo, A.hi + B.hi};
+            if (op == OP_ADD) res = t;
+        }
+        if (m & (1u << OP_SUB)) {
+            const Itv t{A.lo - B.hi, A.hi - B.lo};
+            if (op == OP_SUB) res = t;
+        }
+        if (m & (1u << OP_MUL)) {
+            const Itv t = imul(A, B);
+            if (op == OP_MUL) res = t;
+        }
+        if (m & ((1u << OP_MIN) | (1u << OP_MAX))) {
+            const bool mn = op == OP_MIN;
+            const Itv t = mn ? Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)}
+                             : Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
+            if (mn || op == OP_MAX) {
+                res = t;
+                flags[i] = (uint8_t)mm_choice(op, A, B);
             }
-            V[4 + i] = make_float2(res.lo, res.hi);
+        }
+        if (m & (1u << OP_NEG)) {
+            const Itv t{-A.hi, -A.lo};
+            if (op == OP_NEG) res = t;
+        }
+        if (m & (1u << OP_SQRT)) {
+            const Itv t{__builtin_sqrtf(fmax_glm(0.0f, A.lo)), __builtin_sqrtf(fmax_glm(0.0f, A.hi))};
+            if (op == OP_SQRT) res = t;
+        }
+        if (m & ((1u << OP_SIN) | (1u << OP_COS) | (1u << OP_DIV) | (1u << OP_EXP))) {
+            if (op == OP_SIN || op == OP_COS || op == OP_DIV || op == OP_EXP)
+                res = interval_op(op, A, B, 0.0f);
+        }
+        if (op != kRootNop) V[so] = make_float2(res.lo, res.hi);
+    };
+    if (P) {
+        uint2 q0 = ld(0), q1 = ld(1), q2 = ld(2), q3 = ld(3);
+        // each entry is reloaded once its pass has used it, into the same
+        // registers (no copies that would wait for the loads in flight)
+        for (uint32_t p = 0; p < P; p += 4) {
+            forward(q0, p);
+            q0 = ld(p + 4);
+            forward(q1, p + 1);
+            q1 = ld(p + 5);
+            forward(q2, p + 2);
+            q2 = ld(p + 6);
+            forward(q3, p + 3);
+            q3 = ld(p + 7);
         }
     }
-    const float2 R = V[r.res];
+    const float2 R = opnd(r.res);
     // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
     const bool full = R.y < 0.0f;
     const bool empty = R.x > 0.0f;
@@ -512,15 +614,15 @@ __global__ __launch_bounds__(64 * C) void k_root_interval(RootArgs r) {
 
     // ---- block: mask slices and one reservation for the block's children ----
     const uint32_t stride = n + 1;
-    if (lane == 0) blk_flags[wib] = (full ? 1u : 0u) | (amb ? 2u : 0u);
+    if (sl == 0) blk_flags[cib] = (full ? 1u : 0u) | (amb ? 2u : 0u);
     __syncthreads();
-    if (wib == 0) {
+    if (threadIdx.x < 64) {
         const uint32_t f = lane < C ? blk_flags[lane] : 0u;
         const uint64_t bf = __ballot((f & 1u) != 0), ba = __ballot((f & 2u) != 0);
         const uint64_t namb = (uint64_t)__popcll(ba);
         if (lane == 0) {
-            store_mask_slice<C>(rec + 4, blockIdx.x, (uint32_t)bf);
-            store_mask_slice<C>(rec + 4 + W, blockIdx.x, (uint32_t)ba);
+            store_mask_slice<C>(rec + 4, blk, (uint32_t)bf);
+            store_mask_slice<C>(rec + 4 + W, blk, (uint32_t)ba);
             blk_amb = (uint32_t)ba;
             unsigned long long* ctr = (unsigned long long*)(a.counters + CNT_ALLOC + 2);
             blk_old = namb ? atomicAdd(ctr, (namb << kAllocShift) | (namb * stride)) : 0ull;
@@ -532,54 +634,82 @@ __global__ __launch_bounds__(64 * C) void k_root_interval(RootArgs r) {
     if (amb) {
         const uint32_t bam = blk_amb;
         const uint32_t namb = (uint32_t)__popc(bam);
-        const uint32_t rank = (uint32_t)__popc(bam & ((1u << wib) - 1u));
+        const uint32_t rank = (uint32_t)__popc(bam & ((1u << cib) - 1u));
         const uint64_t old = blk_old;
         const uint32_t cbase = alloc_nodes(old);
         const uint64_t pbase = child_tape_base(a.counters, 0) + alloc_words(old);
         const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
         const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
         if (!(nodes_ok && pool_ok)) {
-            if (lane == 0)
+            if (sl == 0)
                 atomicOr(a.counters + CNT_OVERFLOW,
                          (nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL));
         } else {
-            // ---- backward pass: values reachable from the result ----
-            if (lane == 0) live[r.res] = 1;
-            for (uint32_t p = r.n_pass; p-- > 0;) {
-                const uint32_t b0 = r.sched[p], cnt = r.sched[p + 1] - b0;
-                if (lane < cnt) {
-                    const uint2 e = ent[b0 + lane];
-                    const uint32_t op = e.x & 0xFFu, i = e.x >> 8;
-                    if (live[4 + i]) {
+            // ---- backward pass: instructions the result reaches through its choices ----
+            if (sl == 0) mark(r.res);
+            // passes P-1 .. 0, entries loaded 3 passes ahead
+            auto backward = [&](const uint2 e) {
+                if ((e.x & 0xFu) != kRootNop) {
+                    const uint32_t op = e.x & 0xFu, so = (e.x >> 4) & 0xFFFu, i = e.x >> 16;
+                    const uint32_t lv = live[so];
+                    live[so] = 0;
+                    if (lv) {
+                        const uint32_t f = flags[i];
+                        flags[i] = (uint8_t)(f | 4u);
                         uint32_t sel = MM_FIRST;
                         if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
                             sel = MM_BOTH;
                         else if (op == OP_MIN || op == OP_MAX)
-                            sel = chs[i];
-                        if (sel & MM_FIRST) live[e.y & 0xFFFFu] = 1;
-                        if (sel & MM_SECOND) live[e.y >> 16] = 1;
+                            sel = f & 3u;
+                        if (sel & MM_FIRST) mark(e.y & 0xFFFFu);
+                        if (sel & MM_SECOND) mark(e.y >> 16);
                     }
+                }
+            };
+            if (P) {
+                auto ldr = [&](uint32_t t) { return ld(t < P ? P - 1u - t : 0u); };
+                uint2 q0 = ldr(0), q1 = ldr(1), q2 = ldr(2), q3 = ldr(3);
+                for (uint32_t t = 0; t < P; t += 4) {
+                    backward(q0);
+                    q0 = ldr(t + 4);
+                    backward(q1);
+                    q1 = ldr(t + 5);
+                    backward(q2);
+                    q2 = ldr(t + 6);
+                    backward(q3);
+                    q3 = ldr(t + 7);
                 }
             }
             // ---- kept instructions in tape order (shorten.comp:61-88, 188-191) ----
             uint32_t k = 0;
-            for (uint32_t s = 0; s < n; s += 64)
-                k += (uint32_t)__popcll(__ballot(s + lane < n && live[4 + s + lane] != 0));
+            for (uint32_t s = 0; s < n; s += LPC)
+                k += (uint32_t)__popcll(group_bits(
+                    __ballot(s + sl < n && (flags[s + sl < n ? s + sl : n - 1u] & 4u) != 0)));
             const uint64_t mybase = pbase + (uint64_t)rank * stride;
             uint32_t final_off = toff;
             if (k < n) {
-                uint32_t* out = a.pool + mybase + (n - k);
-                if (lane == 0) out[0] = k;
+                gword* out = (gword*)a.pool + mybase + (n - k);
+                if (sl == 0) out[0] = k;
+                // the tape words of kDeep windows are loaded together
                 uint32_t j = 1;
-                for (uint32_t s = 0; s < n; s += 64) {
-                    const uint32_t i = s + lane;
-                    const bool kp = i < n && live[4 + i] != 0;
-                    const uint64_t b = __ballot(kp);
+                for (uint32_t s0 = 0; s0 < n; s0 += kDeep * LPC) {
+                    uint32_t words[kDeep];
+#pragma unroll
+                    for (uint32_t u = 0; u < kDeep; ++u) {
+                        const uint32_t i = s0 + u * LPC + sl;
+                        words[u] = gtape[1 + (i < n ? i : n - 1u)];
+                    }
+#pragma unroll
+                    for (uint32_t u = 0; u < kDeep; ++u) {
+                    const uint32_t i = s0 + u * LPC + sl;
+                    const uint32_t f = flags[i < n ? i : n - 1u];
+                    const bool kp = i < n && (f & 4u) != 0;
+                    const uint64_t b = group_bits(__ballot(kp));
                     if (kp) {
-                        uint32_t wd = r.tape[1 + i];
+                        uint32_t wd = words[u];
                         const uint32_t op = wd & 0xFFu;
                         if (op == OP_MIN || op == OP_MAX) {
-                            const uint32_t c = chs[i];
+                            const uint32_t c = f & 3u;
                             if (c == MM_FIRST) {
                                 wd = (wd & ~0xFFu) | OP_COPY;
                             } else if (c == MM_SECOND) {
@@ -587,15 +717,16 @@ __global__ __launch_bounds__(64 * C) void k_root_interval(RootArgs r) {
                                 wd = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
                             }
                         }
-                        out[j + lane_rank(b)] = wd;
+                        out[j + (uint32_t)__popcll(b & ((1ull << sl) - 1ull))] = wd;
                     }
                     j += (uint32_t)__popcll(b);
+                    }
                 }
                 final_off = (uint32_t)(mybase + n - k);
             }
             child_id = cbase + rank;
-            if (lane == 0) {
-                uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
+            if (sl == 0) {
+                gword* crec = (gword*)a.child_nodes + (size_t)child_id * a.child_ns;
                 crec[0] = final_off;
                 crec[1] = gx;
                 crec[2] = gy;
@@ -603,7 +734,36 @@ __global__ __launch_bounds__(64 * C) void k_root_interval(RootArgs r) {
             }
         }
     }
-    if (lane == 0) rec[4 + 2 * W + idx] = child_id;
+    if (sl == 0) rec[4 + 2 * W + idx] = child_id;
+}
+
+template <int C, int LPC>
+__global__ __launch_bounds__(C * LPC) void k_root_interval(RootArgs r) {
+    root_block<C, LPC>(r, blockIdx.x);
+}
+
+__device__ __forceinline__ RootArgs load_root_args(const RootArgs* p) {
+    static_assert(sizeof(RootArgs) % 4 == 0, "RootArgs is copied as words");
+    typedef const __attribute__((address_space(4))) uint32_t* cw_t;
+    const cw_t w = (cw_t)p;
+    RootArgs r;
+    uint32_t* d = reinterpret_cast<uint32_t*>(&r);
+#pragma unroll
+    for (uint32_t i = 0; i < sizeof(RootArgs) / 4; ++i) d[i] = w[i];
+    return r;
+}
+
+// Roots of several trees in one launch: tree j owns blocks
+// starts[j] .. starts[j+1]-1 (n <= 64 trees, one ballot finds j).
+template <int C, int LPC>
+__global__ __launch_bounds__(C * LPC) void k_root_interval_batch(const RootArgs* args,
+                                                                 const uint32_t* starts,
+                                                                 uint32_t n) {
+    const uint32_t lane = lane_id();
+    const uint32_t end = lane < n ? starts[lane + 1] : 0xFFFFFFFFu;
+    const uint32_t j = uni((uint32_t)__popcll(__ballot(blockIdx.x >= end)));
+    const RootArgs r = load_root_args(args + j);
+    root_block<C, LPC>(r, blockIdx.x - starts[j]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1279,22 +1439,54 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
     return hipGetLastError();
 }
 
-uint32_t root_cells_per_block(uint32_t cell_bytes) {
-    if ((size_t)cell_bytes * 16 <= kRootMaxLds) return 16;
-    if ((size_t)cell_bytes * 8 <= kRootMaxLds) return 8;
+// 16 cells when the block fits in 64 KiB, else 8 when it fits the device:
+// blocks above 64 KiB would leave a CU little LDS for the kernels that run
+// beside them (other trees of a batch).
+uint32_t root_cells_per_block(uint32_t table_bytes, uint32_t cell_bytes, size_t lds_per_block) {
+    if (lds_per_block <= kRootStaticLds) return 0;
+    const size_t cap = lds_per_block - kRootStaticLds;
+    if (table_bytes + (size_t)cell_bytes * 16 <= std::min<size_t>(cap, 64 * 1024 - kRootStaticLds))
+        return 16;
+    if (table_bytes + (size_t)cell_bytes * 8 <= cap) return 8;
     return 0;
 }
 
 hipError_t launch_root_interval(hipStream_t st, const RootArgs& r) {
-    const uint32_t c = root_cells_per_block(r.cell_bytes);
+    const uint32_t c = r.cells_per_block, lpc = r.lanes_per_cell;
     const uint32_t cells = r.a.dim * r.a.dim * r.a.dim;
-    if (c == 0 || cells % c != 0 || (r.e0 & 1u) || r.nv < 4 || r.res >= r.nv)
+    const bool res_ok = (r.res & kRootConst) ? (r.res & (kRootConst - 1u)) < r.n_const
+                                              : r.res < r.n_slot;
+    if ((c != 16 && c != 8) || (lpc != 16 && lpc != 64) || (c * lpc) % 64 != 0 || r.n_pass % 4 ||
+        cells % c != 0 || (r.e0 & 1u) || r.n_slot < 4 || r.n_slot > kRootMaxSlots + 1 ||
+        r.n == 0 || r.n > kRootMaxInstr || r.n_const >= kRootConst || !res_ok ||
+        r.table_bytes < 10 * r.n_const ||
+        r.cell_bytes < 8 * r.n_slot + ((r.n + 3u) & ~3u) + r.n_slot)
         return hipErrorInvalidValue;
-    const size_t lds = (size_t)c * r.cell_bytes;
-    if (c == 16)
-        k_root_interval<16><<<cells / 16, 1024, lds, st>>>(r);
+    const size_t lds = r.table_bytes + (size_t)c * r.cell_bytes;
+    const uint32_t blocks = cells / c;
+    if (c == 16 && lpc == 64) k_root_interval<16, 64><<<blocks, 1024, lds, st>>>(r);
+    else if (c == 8 && lpc == 64) k_root_interval<8, 64><<<blocks, 512, lds, st>>>(r);
+    else if (c == 16) k_root_interval<16, 16><<<blocks, 256, lds, st>>>(r);
+    else k_root_interval<8, 16><<<blocks, 128, lds, st>>>(r);
+    return hipGetLastError();
+}
+
+hipError_t launch_root_interval_batch(hipStream_t st, const RootArgs* d_args,
+                                      const uint32_t* d_starts, uint32_t n, uint32_t blocks,
+                                      uint32_t cells_per_block, uint32_t lanes_per_cell,
+                                      size_t lds) {
+    if (n == 0 || n > kBatchMax || blocks == 0) return hipErrorInvalidValue;
+    const uint32_t c = cells_per_block, lpc = lanes_per_cell;
+    if (c == 16 && lpc == 64)
+        k_root_interval_batch<16, 64><<<blocks, 1024, lds, st>>>(d_args, d_starts, n);
+    else if (c == 8 && lpc == 64)
+        k_root_interval_batch<8, 64><<<blocks, 512, lds, st>>>(d_args, d_starts, n);
+    else if (c == 16 && lpc == 16)
+        k_root_interval_batch<16, 16><<<blocks, 256, lds, st>>>(d_args, d_starts, n);
+    else if (c == 8 && lpc == 16)
+        k_root_interval_batch<8, 16><<<blocks, 128, lds, st>>>(d_args, d_starts, n);
     else
-        k_root_interval<8><<<cells / 8, 512, lds, st>>>(r);
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

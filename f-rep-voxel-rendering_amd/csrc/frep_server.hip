@@ -400,7 +400,7 @@ frep_status enqueue_slot(frep_server* srv, Slot& S) {
             if (e != FREP_OK) return e;
             idx[k] = (uint32_t)k;
         }
-        const BatchPlan P = plan_batch(S.jobs, idx);
+        const BatchPlan P = plan_batch(S.jobs, idx, 1, ctx->root_ops, ctx->cu_count);
         const frep_status es = ensure_stage(ctx, &S.stage, P);
         if (es != FREP_OK) return es;
         fill_batch(P, S.jobs, idx, static_cast<uint8_t*>(S.stage.h_stage));
@@ -431,7 +431,7 @@ frep_status rebuild(frep_server* srv, Slot& S, size_t k) {
             if (st != FREP_OK) break;
             st = ensure_job(ctx, &one[0]);
             if (st != FREP_OK) break;
-            const BatchPlan P = plan_batch(one, idx);
+            const BatchPlan P = plan_batch(one, idx, 1, ctx->root_ops, ctx->cu_count);
             st = ensure_stage(ctx, &tmp, P);
             if (st != FREP_OK) break;
             fill_batch(P, one, idx, static_cast<uint8_t*>(tmp.h_stage));

@@ -191,7 +191,8 @@ def _slot0_unwritten_tape():
     return Tape(words=words, consts=np.array([0.25], dtype=np.float32), slot_count=6)
 
 
-ROOT_CASES = CASES + [("edge_x", [8, 4]), ("spheres8", [4, 8])]
+ROOT_CASES = CASES + [("edge_x", [8, 4]), ("spheres8", [4, 8]), ("union400", [8, 4, 4]),
+                      ("union700", [8, 4, 4]), ("gyroid", [16, 4])]
 
 
 def test_root_pass_kernel_equals_per_cell_kernel(oracle):
@@ -199,7 +200,10 @@ def test_root_pass_kernel_equals_per_cell_kernel(oracle):
     (one wavefront per root cell; FREP_ROOT_OPS=2 makes its use mandatory)
     gives the same tree as the per-cell kernel (FREP_ROOT_OPS=0) and as the
     oracle, across root sizes 64 to 4096 cells, a non power-of-two root,
-    every op type at the root (trig), and a result that is an input."""
+    every op type at the root (trig), a result that is an input, unions of
+    6k and 10k instructions (constant table, reused value slots, 8-cell
+    blocks for their LDS, the larger one above 64 KiB per block) and the
+    gyroid blend (a long dependency chain)."""
     import os
     ctxs = {}
     for mode in ("0", "2"):
@@ -208,7 +212,10 @@ def test_root_pass_kernel_equals_per_cell_kernel(oracle):
             ctxs[mode] = lib.Context(0)
         finally:
             del os.environ["FREP_ROOT_OPS"]
-    tapes = dict(TAPES, edge_x=_slot0_unwritten_tape)
+    tapes = dict(TAPES, edge_x=_slot0_unwritten_tape,
+                 union400=lambda: shapes.spheres_union(400, 1035),   # 6k instructions
+                 union700=lambda: shapes.spheres_union(700, 1036),   # 10k instructions
+                 gyroid=shapes.gyroid_blend)                          # sin/cos chain, 266 passes
     try:
         for name, dims in ROOT_CASES:
             tape = tapes[name]()
@@ -373,6 +380,38 @@ def test_voxelize_batch_heterogeneous_vs_oracle(ctx, oracle):
         assert list(ctx.tree_info(tr).node_count[:len(grid["dims"])]) == ref.counts()
         assert np.array_equal(ctx.export(tr), ref.canonical())
         ctx.tree_free(tr)
+
+
+def test_voxelize_batch_root_levels_vs_oracle(oracle):
+    """Level 0 of every multi-level tree of a batch by the batched root
+    kernel (FREP_ROOT_OPS=2 makes it mandatory): one launch over trees that
+    differ in tape, root size and LDS layout, where a 10k-instruction union
+    sets 8-cell blocks for the whole launch.  Every tree equals the
+    oracle's."""
+    import os
+    os.environ["FREP_ROOT_OPS"] = "2"
+    try:
+        c = lib.Context(0)
+    finally:
+        del os.environ["FREP_ROOT_OPS"]
+    jobs = [
+        (shapes.spheres_union(700, 1036), shapes.make_grid([8, 4, 4])),
+        (TAPES["csg24"](), shapes.make_grid([16, 8, 8])),
+        (TAPES["trig"](), shapes.make_grid([8, 8, 4], tape_time=0.5)),
+        (shapes.spheres_union(3, 11), shapes.make_grid([4, 4, 4, 4])),
+        (TAPES["csg24"](), shapes.make_grid([12, 8])),
+        (shapes.gyroid_blend(), shapes.make_grid([16, 4])),
+        (shapes.spheres_union(5, 13), shapes.make_grid([8])),
+    ]
+    try:
+        ths = [c.upload(t) for t, _ in jobs]
+        trees = c.voxelize_batch(ths, [g for _, g in jobs])
+        for (tape, grid), tr in zip(jobs, trees):
+            ref = oracle.Tree(tape, grid)
+            assert list(c.tree_info(tr).node_count[:len(grid["dims"])]) == ref.counts()
+            assert np.array_equal(c.export(tr), ref.canonical())
+    finally:
+        c.close()
 
 
 def test_voxelize_batch_tuned_and_graph_replays(ctx, oracle):

@@ -1156,6 +1156,25 @@ RenderArgs render_args(const TreeObj* t, const frep_camera* cam, uint8_t* rgba,
 
 extern "C" {
 
+frep_status frep_root_schedule(const uint32_t* words, uint32_t n_words, uint32_t lanes_per_cell,
+                               uint32_t lds_per_block, uint32_t* info, uint32_t* out,
+                               uint32_t out_cap) {
+    if (!words || !info || n_words < 1 || (lanes_per_cell != 16 && lanes_per_cell != 64))
+        return FREP_INVALID_ARG;
+    if ((uint64_t)words[0] + 1 > n_words) return FREP_INVALID_ARG;
+    for (uint32_t k = 0; k < words[0]; ++k)
+        if ((words[1 + k] & 0xFFu) > OP_COPY) return FREP_BAD_TAPE;
+    std::vector<uint32_t> sched;
+    RootSched rs;
+    const bool ok = build_root_schedule(words, lanes_per_cell, lds_per_block, &sched, &rs);
+    const uint32_t v[12] = {ok ? 1u : 0u, rs.pass,        rs.nconst,     rs.c0,
+                            rs.m0,        rs.e0,          rs.res,        rs.slots,
+                            rs.table_bytes, rs.cell_bytes, rs.cpb,       (uint32_t)sched.size()};
+    std::memcpy(info, v, sizeof(v));
+    if (ok && out && out_cap >= sched.size()) std::memcpy(out, sched.data(), sched.size() * 4);
+    return FREP_OK;
+}
+
 frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, frep_tree_t h) {
     if (!ctx) return FREP_INVALID_ARG;
     std::lock_guard<std::mutex> lk(ctx->mu);

@@ -25,7 +25,7 @@ EXPORTED = [
     "frep_device_info", "frep_tape_upload", "frep_tape_free", "frep_tree_create",
     "frep_voxelize", "frep_voxelize_batch", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
     "frep_camera_look", "frep_render", "frep_eval_points", "frep_eval_intervals",
-    "frep_eval_gradients", "frep_math",
+    "frep_eval_gradients", "frep_math", "frep_root_schedule",
     "frep_server_create", "frep_server_destroy", "frep_server_last_error", "frep_server_scene",
     "frep_server_submit", "frep_server_wait", "frep_server_poll", "frep_server_stats_get",
     "frep_server_reset_stats",
@@ -131,6 +131,8 @@ def load():
                                           f32p, u8p]),
         "frep_eval_gradients": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint32, C.c_float, f32p]),
         "frep_math": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint32, f32p]),
+        "frep_root_schedule": (C.c_int, [u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p,
+                                         C.c_uint32]),
         "frep_server_create": (C.c_int, [vp, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
         "frep_server_destroy": (C.c_int, [vp]),
         "frep_server_last_error": (C.c_char_p, [vp]),
@@ -353,6 +355,31 @@ class Context:
         out = np.zeros_like(x)
         self._check(self.L.frep_math(self.h, fn, _p(x, C.c_float), len(x), _p(out, C.c_float)))
         return out
+
+
+ROOT_SCHEDULE_FIELDS = ("scheduled", "passes", "consts", "c0", "m0", "e0", "res", "slots",
+                        "table_bytes", "cell_bytes", "cells_per_block", "words")
+
+
+def root_schedule(words, lanes_per_cell=64, lds_per_block=160 * 1024):
+    """The level-0 schedule frep_tape_upload builds for a tape (host only):
+    (info dict, schedule words), or (info, None) when the tape has none."""
+    w = np.ascontiguousarray(words, dtype=np.uint32)
+    info = np.zeros(12, dtype=np.uint32)
+    L = load()
+    st = L.frep_root_schedule(_p(w, C.c_uint32), len(w), lanes_per_cell, lds_per_block,
+                              _p(info, C.c_uint32), None, 0)
+    if st != FREP_OK:
+        raise FrepError(st, "frep_root_schedule failed")
+    d = {k: int(v) for k, v in zip(ROOT_SCHEDULE_FIELDS, info)}
+    if not d["scheduled"]:
+        return d, None
+    out = np.zeros(max(1, d["words"]), dtype=np.uint32)
+    st = L.frep_root_schedule(_p(w, C.c_uint32), len(w), lanes_per_cell, lds_per_block,
+                              _p(info, C.c_uint32), _p(out, C.c_uint32), len(out))
+    if st != FREP_OK:
+        raise FrepError(st, "frep_root_schedule failed")
+    return d, out[:d["words"]]
 
 
 def camera_look(cam: Camera, initial_up, fov_deg) -> Camera:

@@ -181,6 +181,20 @@ frep_status frep_eval_gradients(frep_ctx* ctx, frep_tape_t tape, const float* xy
 /* fn: 0 sin, 1 cos, 2 exp -- the device implementations used by the tapes. */
 frep_status frep_math(frep_ctx* ctx, uint32_t fn, const float* x, uint32_t n, float* out);
 
+/* ---- level-0 schedule (host only, no device needed) ----
+ * No counterpart in the reference: the reference evaluates the root tape one
+ * instruction at a time (vulkan/shaders/tape/interval.comp:121-160).  This
+ * returns the schedule frep_tape_upload builds for the pass-parallel root
+ * level, for inspection and tests.  words: [n, instrs] (n_words >= n + 1);
+ * lanes_per_cell: 64 or 16; lds_per_block: the device's LDS per block in
+ * bytes.  info (12 words): scheduled (0/1), passes, constants, constant
+ * offset, pass-mask offset, entry offset, result operand, value slots,
+ * constant-table bytes, bytes per cell, cells per block, schedule words.
+ * The schedule is copied to out when out_cap >= info[11]. */
+frep_status frep_root_schedule(const uint32_t* words, uint32_t n_words, uint32_t lanes_per_cell,
+                               uint32_t lds_per_block, uint32_t* info, uint32_t* out,
+                               uint32_t out_cap);
+
 /* ---- serving: request loop with continuous batching ----
  * The reference has no serving layer (its application voxelizes and renders
  * one scene per frame, vulkan/src/application.cpp:60-93); these entry

@@ -822,6 +822,20 @@ BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& 
                     la.maxs = std::max(la.maxs, j.t->maxs);
                     la.mm = std::max(la.mm, j.mm_stride);
                 }
+                if (la.count && la.point) {
+                    // the assembly-only kernel when every shape takes the
+                    // assembly interpreter (fill_batch's use_asm rule)
+                    bool all_asm = la.maxs <= 16;
+                    uint32_t slots = 0;
+                    for (uint32_t k = 0; k < G.count && all_asm; ++k) {
+                        const Job& j = jobs[idx[G.first + k]];
+                        const uint32_t nl = j.t->grid.n_levels;
+                        if (L >= nl || L + 1 != nl) continue;
+                        all_asm = j.t->asm_ok && point_k_used(la.maxs, j.t->grid.dims[L]) == 8;
+                        slots = std::max(slots, j.t->slot_count);
+                    }
+                    la.asm_slots = all_asm ? slots : 0u;
+                }
                 if (la.count) {
                     P.bytes = align16(P.bytes + la.count * sizeof(LevelArgs));
                     P.launches.push_back(la);
@@ -887,8 +901,8 @@ uint64_t fnv1a(const void* data, size_t n, uint64_t h = 1469598103934665603ull) 
 uint64_t batch_key(const BatchPlan& P, const uint8_t* hs) {
     uint64_t h = fnv1a(hs, P.bytes);
     for (const BatchLaunch& la : P.launches) {
-        const uint32_t v[8] = {la.group, la.level, la.point ? 1u : 0u, la.count, la.maxs, la.mm,
-                               la.root ? la.cpb : 0u, la.variant};
+        const uint32_t v[9] = {la.group, la.level, la.point ? 1u : 0u, la.count, la.maxs, la.mm,
+                               la.root ? la.cpb : 0u, la.variant, la.asm_slots};
         h = fnv1a(v, sizeof(v), h);
     }
     return fnv1a(&P.m, sizeof(P.m), h);
@@ -967,7 +981,8 @@ frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const Ba
                     reinterpret_cast<const uint32_t*>(ds + la.starts_off), la.count, la.blocks,
                     la.cpb, kRootLanes[la.variant], la.lds));
             else if (la.point)
-                HIP_TRY(launch_level_point_batch(gs, b, la.maxs, ctx->cu_count, grid[li]));
+                HIP_TRY(launch_level_point_batch(gs, b, la.maxs, ctx->cu_count, grid[li],
+                                                 la.asm_slots));
             else
                 HIP_TRY(launch_level_interval_batch(gs, b, la.maxs, ctx->cu_count, grid[li]));
             if (ev) HIP_TRY(hipEventRecord(ev[1 + li], gs));

@@ -920,6 +920,37 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_interval_batch(BatchArgs 
     }
 }
 
+// The same walk when every shape of the launch takes the assembly
+// interpreter (8 voxels per lane, slot file MS): without the C++ paths the
+// kernel needs only the assembly path's registers (more wavefronts per SIMD).
+template <int MS>
+__global__ __launch_bounds__(kLevelBlock) void k_level_point_batch_asm(BatchArgs b) {
+    const uint32_t lane = lane_id();
+    const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
+    const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
+    uint64_t unused;
+    const uint32_t incl = wave_incl_scan(plan_lane<true>(b, lane, &unused), lane);
+    const uint32_t total = rdlane(incl, 63);
+    cargs_t shapes = (cargs_t)b.shapes;
+    uint32_t cur = 0xFFFFFFFFu, start = 0, cpw = 1;
+    bool skip = false;
+    LevelArgs a{};
+    for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
+        const uint32_t s = (uint32_t)__popcll(__ballot(incl <= chunk));
+        if (s != cur) {
+            cur = s;
+            a = load_args(shapes + s);
+            start = s ? rdlane(incl, s - 1) : 0u;
+            cpw = a.dim * a.dim * a.dim / (64u * 8u);
+            skip = (uni(load_counter(a.counters + CNT_OVERFLOW)) & ovf_before(a.level)) != 0;
+        }
+        if (skip) continue;
+        const uint32_t local = chunk - start;
+        const uint32_t node = local / cpw, sub = local - node * cpw;
+        point_chunk<MS, 8, true>(a, node, sub, lane);
+    }
+}
+
 template <int MAXS>
 __global__ __launch_bounds__(kLevelBlock) void k_level_point_batch(BatchArgs b) {
     constexpr int KM = MAXS <= 16 ? 8 : (MAXS <= 32 ? 4 : 1);
@@ -1640,8 +1671,23 @@ hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint3
 }
 
 hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
-                                    uint32_t cu_count, uint32_t max_grid) {
+                                    uint32_t cu_count, uint32_t max_grid, uint32_t asm_slots) {
     if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
+    if (asm_slots && asm_slots <= 16) {
+        if (asm_slots <= 8)
+            k_level_point_batch_asm<8><<<batch_grid((const void*)k_level_point_batch_asm<8>, 0,
+                                                    cu_count, max_grid),
+                                         kLevelBlock, 0, st>>>(b);
+        else if (asm_slots <= 10)
+            k_level_point_batch_asm<10><<<batch_grid((const void*)k_level_point_batch_asm<10>, 0,
+                                                     cu_count, max_grid),
+                                          kLevelBlock, 0, st>>>(b);
+        else
+            k_level_point_batch_asm<16><<<batch_grid((const void*)k_level_point_batch_asm<16>, 0,
+                                                     cu_count, max_grid),
+                                          kLevelBlock, 0, st>>>(b);
+        return hipGetLastError();
+    }
     FREP_DISPATCH_MAXS(maxs, (k_level_point_batch<M><<<
                                   batch_grid((const void*)k_level_point_batch<M>, 0, cu_count,
                                              max_grid),

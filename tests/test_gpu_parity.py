@@ -414,6 +414,27 @@ def test_voxelize_batch_root_levels_vs_oracle(oracle):
         c.close()
 
 
+@pytest.mark.parametrize("jobs", [
+    [("csg24", [16, 8, 8]), ("spheres8", [8, 8, 8]), ("csg24", [8, 8, 8])],   # <= 10 slots
+    [(20, 2, [8, 8, 8]), (48, 5, [8, 8, 8]), ("spheres8", [8, 8])],            # 11..13 slots
+    [("spheres8", [8, 4, 8]), (20, 2, [16, 8])],                                 # <= 8 and 11
+])
+def test_voxelize_batch_assembly_point_kernels_vs_oracle(ctx, oracle, jobs):
+    """Batches whose every last level takes the assembly point interpreter
+    run the assembly-only batched kernel with the 8-, 10- or 16-slot file;
+    every tree equals the oracle's."""
+    tapes = []
+    for j in jobs:
+        tape = TAPES[j[0]]() if isinstance(j[0], str) else shapes.random_csg(j[0], j[1])
+        tapes.append((tape, shapes.make_grid(j[-1])))
+    ths = [ctx.upload(t) for t, _ in tapes]
+    trees = ctx.voxelize_batch(ths, [g for _, g in tapes])
+    for (tape, grid), tr in zip(tapes, trees):
+        ref = oracle.Tree(tape, grid)
+        assert np.array_equal(ctx.export(tr), ref.canonical())
+        ctx.tree_free(tr)
+
+
 def test_voxelize_batch_tuned_and_graph_replays(ctx, oracle):
     """The same batch run repeatedly: the first calls time the autotuner's
     candidate grid sizes, later ones replay a captured graph.  Every call,

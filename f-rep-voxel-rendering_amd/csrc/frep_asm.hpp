@@ -59,6 +59,24 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", "v76", \
         "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", "v88",    \
         "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99", "v100"
+// Clobbers of the slot files minus v[WB + 0] and v[WB + 3] (window 0 and
+// its constants, passed in): WB = 96 (8 slots), 112 (10), 160 (16).
+#define FREP_ASM_CLOBBER_64_95                                                                \
+    "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", "v76", \
+        "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", "v88",    \
+        "v89", "v90", "v91", "v92", "v93", "v94", "v95"
+#define FREP_ASM_CLOBBER_W8 "v97", "v98", "v100"
+#define FREP_ASM_CLOBBER_101_111                                                              \
+    "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110", "v111"
+#define FREP_ASM_CLOBBER_W10 "v113", "v114", "v116"
+#define FREP_ASM_CLOBBER_101_159                                                              \
+    "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",            \
+        "v111", "v112", "v113", "v114", "v115", "v116", "v117", "v118", "v119", "v120", "v121", \
+        "v122", "v123", "v124", "v125", "v126", "v127", "v128", "v129", "v130", "v131", "v132", \
+        "v133", "v134", "v135", "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143", \
+        "v144", "v145", "v146", "v147", "v148", "v149", "v150", "v151", "v152", "v153", "v154", \
+        "v155", "v156", "v157", "v158", "v159"
+#define FREP_ASM_CLOBBER_W16 "v161", "v162", "v164"
 #define FREP_ASM_CLOBBER_101_116                                                               \
     "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",            \
         "v111", "v112", "v113", "v114", "v115", "v116"
@@ -158,14 +176,15 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "s_cbranch_scc1 .Lfa_end%=\n"                                                             \
     "s_sub_u32 %[nm1], %[n], 1\n"                                                             \
     "s_mov_b32 %[s], 0\n"                                                                     \
-    "s_mov_b32 %[sidx], 0\n"                                                                  \
-    FREP_ASM_LOADW("WB + 0")                                                                  \
+    /* window 0 and its constants arrive in v[WB + 0] and v[WB + 3]; the                      \
+       next two windows are loaded only for tapes longer than 64 words */                     \
+    "s_cmp_le_u32 %[n], 64\n"                                                                 \
+    "s_cbranch_scc1 .Lfa_win%=\n"                                                             \
     "s_mov_b32 %[sidx], 64\n"                                                                 \
     FREP_ASM_LOADW("WB + 1")                                                                  \
     "s_mov_b32 %[sidx], 128\n"                                                                \
     FREP_ASM_LOADW("WB + 2")                                                                  \
     "s_waitcnt vmcnt(0)\n"                                                                    \
-    FREP_ASM_GATHER("WB + 3", "WB + 0")                                                       \
     FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
     "s_waitcnt vmcnt(0)\n"                                                                    \
     ".Lfa_win%=:\n"                                                                           \
@@ -336,13 +355,44 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "s_waitcnt vmcnt(0)\n"                                                                    \
     ".Lfa_end%=:\n"
 
+// Prefetch around the interpreter (run_point_tape_asm8_pf): issued before
+// the body, waited after it, so nothing is in flight when the block ends.
+//   pfw  lane l's word of 64 at pfa (the next tape's window 0)
+//   pfn  the word at pfb (the next tape's length)
+//   pfh  four words at pfc (the header of the node after that)
+#define FREP_ASM_PF_ISSUE                                                                     \
+    "v_mbcnt_lo_u32_b32 %[pft], -1, 0\n"                                                      \
+    "v_mbcnt_hi_u32_b32 %[pft], -1, %[pft]\n"                                                 \
+    "v_lshlrev_b32_e32 %[pft], 2, %[pft]\n"                                                   \
+    "global_load_dword %[pfw], %[pft], %[pfa]\n"                                              \
+    "s_load_dword %[pfn], %[pfb], 0x0\n"                                                      \
+    "s_load_dwordx4 %[pfh], %[pfc], 0x0\n"
+#define FREP_ASM_PF_WAIT "s_waitcnt vmcnt(0) lgkmcnt(0)\n"
+
+// Window 0 of a tape for the assembly interpreter: lane l's instruction
+// word min(l, n - 1) and the constant it names if it is a CONST (the pool's
+// first constant otherwise, and for an empty tape).
+__device__ __forceinline__ void point_window0(const uint32_t* tape, const float* consts,
+                                              uint32_t lane, uint32_t* w0, float* c0,
+                                              uint32_t* len = nullptr) {
+    typedef const __attribute__((address_space(4))) uint32_t* cword_t;
+    const uint32_t n = ((cword_t)tape)[0];
+    if (len) *len = n;
+    const uint32_t nm1 = n ? n - 1u : 0u;
+    const uint32_t w = tape[1 + (lane < nm1 ? lane : nm1)];
+    *w0 = w;
+    *c0 = consts[(n != 0 && (w & 0xFFu) == 0u) ? (w >> 16) : 0u];
+}
+
 // x, y, z: the 8 points of this lane (one register each); returns their
 // densities.  tape: [n, instrs]; consts: the shape's pool (index 0 readable
-// even when empty).  MAXS must be 8 or 16 (slot file <= 128 registers).
-// Every lane of the wavefront must be active.
+// even when empty).  w0, c0: the first window (lane l: instruction word
+// min(l, n - 1)) and its constants (point_window0).  MAXS must be 8, 10 or
+// 16.  Every lane of the wavefront must be active.
 template <int MAXS>
 __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const float* consts,
-                                                   float t, v8f x, v8f y, v8f z) {
+                                                   float t, v8f x, v8f y, v8f z, uint32_t w0,
+                                                   float c0) {
     static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
     typedef const __attribute__((address_space(4))) uint32_t* cword_t;
     const uint32_t n = ((cword_t)tape)[0];
@@ -352,17 +402,19 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
     if constexpr (MAXS == 16) {
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v160}"(w0), "+{v163}"(c0),
                        [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
                        [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
                        [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
                        [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
                      : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_164,
-                       "vcc", "memory");
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
+                       FREP_ASM_CLOBBER_W16, "vcc", "memory");
     } else if constexpr (MAXS == 10) {
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v112}"(w0), "+{v115}"(c0),
                        [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
                        [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
                        [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
@@ -370,18 +422,83 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
                      : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
-                       FREP_ASM_CLOBBER_101_116, "vcc", "memory");
+                       FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
     } else {
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v96}"(w0), "+{v99}"(c0),
                        [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
                        [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
                        [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
                        [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
                      : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(3)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, "vcc", "memory");
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
+                       "vcc", "memory");
     }
+    return x;
+}
+
+// The same with the tape's length n given, loading while the tape runs:
+// pf_win[lane] into *w1, pf_len[0] into *n1 and pf_hdr[0..3] into h2 (all
+// readable addresses).
+template <int MAXS>
+__device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, const float* consts,
+                                                   float t, v8f x, v8f y, v8f z, uint32_t w0,
+                                                   float c0, uint32_t n, const uint32_t* pf_win,
+                                                   const uint32_t* pf_len, const uint32_t* pf_hdr,
+                                                   uint32_t* w1, uint32_t* n1, uint32_t h2[4]) {
+    static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
+    const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
+    uint32_t w, a, b, o, j, m, s, sidx, nm1, cls, tiny, sgn;
+    uint64_t p0, p1, p2, p3;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    uint32_t pfw, pft, pfn;
+    u32x4 pfh;
+    const uint64_t pfa = (uint64_t)pf_win, pfb = (uint64_t)pf_len, pfc = (uint64_t)pf_hdr;
+    if constexpr (MAXS == 16) {
+        asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v160}"(w0), "+{v163}"(c0),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
+                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(4)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
+                       FREP_ASM_CLOBBER_W16, "vcc", "memory");
+    } else if constexpr (MAXS == 10) {
+        asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v112}"(w0), "+{v115}"(c0),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
+                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(4)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
+                       FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
+    } else {
+        asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v96}"(w0), "+{v99}"(c0),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
+                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(3)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
+                       "vcc", "memory");
+    }
+    *w1 = pfw;
+    *n1 = pfn;
+    h2[0] = pfh.x; h2[1] = pfh.y; h2[2] = pfh.z; h2[3] = pfh.w;
     return x;
 }
 

@@ -771,6 +771,46 @@ __global__ __launch_bounds__(C * LPC) void k_root_interval_batch(const RootArgs*
 // (vulkan/shaders/voxelizer/main.comp:74-83).  Each lane evaluates K voxels
 // of the same node (same tape) so instruction decode is shared K ways.
 // ---------------------------------------------------------------------------
+// Chunk `sub` of a last-level node through the assembly interpreter (8
+// voxels per lane), given the node's header words and the tape's window 0
+// with its constants (w0, c0 as run_point_tape_asm8 takes them).  With PF
+// the interpreter also loads pf_win[lane], pf_len[0] and pf_hdr[0..3] into
+// *w1, *n1 and h2 while it runs.
+template <int MAXS, bool PF = false>
+__device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t node, uint32_t sub,
+                                                uint32_t lane, uint32_t toff, uint32_t ncx,
+                                                uint32_t ncy, uint32_t ncz, uint32_t w0,
+                                                float c0, const uint32_t* pf_win = nullptr,
+                                                const uint32_t* pf_len = nullptr,
+                                                const uint32_t* pf_hdr = nullptr,
+                                                uint32_t* w1 = nullptr, uint32_t* n1 = nullptr,
+                                                uint32_t* h2 = nullptr, uint32_t n = 0) {
+    const uint32_t dim = a.dim;
+    uint32_t* rec = a.nodes + (size_t)node * a.ns;
+    v8f xs, ys, zs;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint32_t idx = (sub * 8 + (uint32_t)k) * 64 + lane;
+        uint32_t cx, cy, cz;
+        cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
+        // main.comp:77-79: grid low + float(node.coords * dim + cell) * cell size
+        xs[k] = a.low[0] + (float)(ncx * dim + cx) * a.cs;
+        ys[k] = a.low[1] + (float)(ncy * dim + cy) * a.cs;
+        zs[k] = a.low[2] + (float)(ncz * dim + cz) * a.cs;
+    }
+    v8f d;
+    if constexpr (PF)
+        d = run_point_tape_asm8_pf<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0,
+                                         n, pf_win, pf_len, pf_hdr, w1, n1, h2);
+    else
+        d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const uint64_t B = __ballot(d[k] < 0.0f);
+        store_mask64(rec + 4 + (sub * 8 + (uint32_t)k) * 2, B, lane);
+    }
+}
+
 template <int MAXS, int K, bool ASM>
 __device__ __forceinline__ void point_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                             uint32_t lane) {
@@ -794,15 +834,10 @@ __device__ __forceinline__ void point_chunk(const LevelArgs& a, uint32_t node, u
         store_mask64(rec + 4 + (sub * K + (uint32_t)k) * 2, B, lane);
     };
     if constexpr (ASM) {
-        v8f xs, ys, zs;
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            const float3 p = init(k);
-            xs[k] = p.x; ys[k] = p.y; zs[k] = p.z;
-        }
-        const v8f d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs);
-#pragma unroll
-        for (int k = 0; k < 8; ++k) out(k, d[k]);
+        uint32_t w0;
+        float c0;
+        point_window0(a.pool + toff, a.consts, lane, &w0, &c0);
+        point_chunk_asm<MAXS>(a, node, sub, lane, toff, ncx, ncy, ncz, w0, c0);
     } else {
         run_point_tape<MAXS, K>(as_const(a.pool + toff), as_const(a.consts), a.tape_time, init,
                                 out);
@@ -819,9 +854,49 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
     const uint32_t cpw = a.dim * a.dim * a.dim / (64 * K);
     const uint32_t total = n_nodes * cpw;
-    for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
-        const uint32_t node = chunk / cpw, sub = chunk - node * cpw;
-        point_chunk<MAXS, K, ASM>(a, node, sub, lane);
+    if constexpr (ASM) {
+        // Software pipeline over this wavefront's chunks: while the
+        // interpreter runs chunk c it loads the next chunk's tape window (64
+        // words from the tape's first instruction; the pool has 64 words of
+        // padding past its end) and length, and the header of the chunk after
+        // that.  The next chunk's constants are gathered after it, for its
+        // lanes below its length.
+        uint32_t c = gwave;
+        if (c >= total) return;
+        auto header = [&](uint32_t ch, uint32_t h[4]) {
+            ctape_t r = as_const(a.nodes + (size_t)(ch / cpw) * a.ns);
+            h[0] = r[0]; h[1] = r[1]; h[2] = r[2]; h[3] = r[3];
+        };
+        uint32_t h0[4], h1[4];
+        header(c, h0);
+        header(c + nwaves < total ? c + nwaves : c, h1);
+        uint32_t w0, n0;
+        float c0;
+        point_window0(a.pool + h0[0], a.consts, lane, &w0, &c0, &n0);
+        for (;;) {
+            const uint32_t c1 = c + nwaves, c2 = c1 + nwaves;
+            const uint32_t* pf_len = a.pool + (c1 < total ? h1[0] : h0[0]);
+            const uint32_t* pf_hdr = a.nodes + (size_t)((c2 < total ? c2 : c) / cpw) * a.ns;
+            uint32_t w1, n1, h2[4];
+            const uint32_t node = c / cpw;
+            point_chunk_asm<MAXS, true>(a, node, c - node * cpw, lane, h0[0], h0[1], h0[2], h0[3],
+                                        w0, c0, pf_len + 1, pf_len, pf_hdr, &w1, &n1, h2, n0);
+            if (c1 >= total) break;
+            c0 = a.consts[(lane < n1 && (w1 & 0xFFu) == 0u) ? (w1 >> 16) : 0u];
+            w0 = w1;
+            n0 = n1;
+            c = c1;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                h0[q] = h1[q];
+                h1[q] = h2[q];
+            }
+        }
+    } else {
+        for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
+            const uint32_t node = chunk / cpw, sub = chunk - node * cpw;
+            point_chunk<MAXS, K, ASM>(a, node, sub, lane);
+        }
     }
 }
 
@@ -1317,7 +1392,10 @@ __global__ __launch_bounds__(256) void k_eval_points_asm(const uint32_t* tape, c
         const uint32_t q = p < n ? p : n - 1;
         xs[k] = xyz[3 * q]; ys[k] = xyz[3 * q + 1]; zs[k] = xyz[3 * q + 2];
     }
-    const v8f d = run_point_tape_asm8<MAXS>(tape, consts, t, xs, ys, zs);
+    uint32_t w0;
+    float c0;
+    point_window0(tape, consts, threadIdx.x & 63u, &w0, &c0);
+    const v8f d = run_point_tape_asm8<MAXS>(tape, consts, t, xs, ys, zs, w0, c0);
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const uint32_t p = blockIdx.x * 2048u + (uint32_t)k * 256u + threadIdx.x;

@@ -788,15 +788,28 @@ __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t nod
     const uint32_t dim = a.dim;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
     v8f xs, ys, zs;
+    // main.comp:77-79: grid low + float(node.coords * dim + cell) * cell size
+    if (dim == 8) {
+        // one chunk per node: bit index k*64 + lane is cell (k, lane / 8,
+        // lane % 8), so y and z are the same for the lane's 8 voxels
+        const float y = a.low[1] + (float)(ncy * 8u + (lane >> 3)) * a.cs;
+        const float z = a.low[2] + (float)(ncz * 8u + (lane & 7u)) * a.cs;
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint32_t idx = (sub * 8 + (uint32_t)k) * 64 + lane;
-        uint32_t cx, cy, cz;
-        cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
-        // main.comp:77-79: grid low + float(node.coords * dim + cell) * cell size
-        xs[k] = a.low[0] + (float)(ncx * dim + cx) * a.cs;
-        ys[k] = a.low[1] + (float)(ncy * dim + cy) * a.cs;
-        zs[k] = a.low[2] + (float)(ncz * dim + cz) * a.cs;
+        for (int k = 0; k < 8; ++k) {
+            xs[k] = a.low[0] + (float)(ncx * 8u + (uint32_t)k) * a.cs;
+            ys[k] = y;
+            zs[k] = z;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const uint32_t idx = (sub * 8 + (uint32_t)k) * 64 + lane;
+            uint32_t cx, cy, cz;
+            cell_of(idx, dim, a.dim_shift, &cx, &cy, &cz);
+            xs[k] = a.low[0] + (float)(ncx * dim + cx) * a.cs;
+            ys[k] = a.low[1] + (float)(ncy * dim + cy) * a.cs;
+            zs[k] = a.low[2] + (float)(ncz * dim + cz) * a.cs;
+        }
     }
     v8f d;
     if constexpr (PF)

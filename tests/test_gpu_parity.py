@@ -115,6 +115,7 @@ CASES = [
     ("csg24", [8, 8, 4]),
     ("csg24", [16, 8, 8]),         # 1024^3, the headline workload
     ("csg24", [12, 8]),            # non power-of-two branching
+    ("csg24", [8, 16]),            # 16-wide last level: 8 chunks per node
     ("trig", [8, 8, 4]),
     ("trig", [16, 4]),
 ]

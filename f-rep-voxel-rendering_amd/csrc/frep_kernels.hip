@@ -161,6 +161,11 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     const float lz = a.low[2] + (float)gz * a.cs;
     const uint32_t n = as_const(a.pool + toff)[0];
     const uint32_t* tw = a.pool + toff + 1;             // instruction words
+    // The level ends with its longest tapes: their wavefronts take issue
+    // priority over the shorter ones on the same SIMD (restored at the end).
+    if (n >= 96) __builtin_amdgcn_s_setprio(3);
+    else if (n >= 64) __builtin_amdgcn_s_setprio(2);
+    else if (n >= 32) __builtin_amdgcn_s_setprio(1);
 
     // ---- forward pass: interval evaluation (interval.comp:121-160) ----
     float ilo, ihi;
@@ -365,6 +370,7 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
         }
     }
     rec[4 + 2 * W + idx] = child_id;
+    __builtin_amdgcn_s_setprio(0);
 }
 
 // Node count of `level` and first pool word of level+1's tapes (tapes of

@@ -368,6 +368,26 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "s_load_dword %[pfn], %[pfb], 0x0\n"                                                      \
     "s_load_dwordx4 %[pfh], %[pfc], 0x0\n"
 #define FREP_ASM_PF_WAIT "s_waitcnt vmcnt(0) lgkmcnt(0)\n"
+// Two nodes deep (run_point_tape_asm8_pf2): also the constants of the next
+// node, from its window pw1 (lane l's word of a tape of pn1 words): the
+// constant a CONST word names for lanes below pn1, the pool's first
+// constant otherwise, into pfk.
+#define FREP_ASM_PF2_ISSUE                                                                    \
+    "v_mbcnt_lo_u32_b32 %[pft], -1, 0\n"                                                      \
+    "v_mbcnt_hi_u32_b32 %[pft], -1, %[pft]\n"                                                 \
+    "v_cmp_gt_u32_e64 %[pfs], %[pn1], %[pft]\n"                                               \
+    "v_lshlrev_b32_e32 %[pft], 2, %[pft]\n"                                                   \
+    "global_load_dword %[pfw], %[pft], %[pfa]\n"                                              \
+    "s_load_dword %[pfn], %[pfb], 0x0\n"                                                      \
+    "s_load_dwordx4 %[pfh], %[pfc], 0x0\n"                                                    \
+    "v_and_b32_e32 %[pft], 0xff, %[pw1]\n"                                                    \
+    "v_cmp_eq_u32_e32 vcc, 0, %[pft]\n"                                                       \
+    "s_and_b64 vcc, vcc, %[pfs]\n"                                                            \
+    "v_lshrrev_b32_e32 %[pft], 16, %[pw1]\n"                                                  \
+    "s_nop 1\n"                                                                               \
+    "v_cndmask_b32_e32 %[pft], 0, %[pft], vcc\n"                                              \
+    "v_lshlrev_b32_e32 %[pft], 2, %[pft]\n"                                                   \
+    "global_load_dword %[pfk], %[pft], %[cst]\n"
 
 // Window 0 of a tape for the assembly interpreter: lane l's instruction
 // word min(l, n - 1) and the constant it names if it is a CONST (the pool's
@@ -498,6 +518,76 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
     }
     *w1 = pfw;
     *n1 = pfn;
+    h2[0] = pfh.x; h2[1] = pfh.y; h2[2] = pfh.z; h2[3] = pfh.w;
+    return x;
+}
+
+// Two nodes deep: as run_point_tape_asm8_pf, and also the constants of the
+// next node (its window nx_w, lane l's word of a tape of nx_n words) into
+// *nx_c, as point_window0 gathers them.
+template <int MAXS>
+__device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, const float* consts,
+                                                   float t, v8f x, v8f y, v8f z, uint32_t w0,
+                                                   float c0, uint32_t n, const uint32_t* pf_win,
+                                                   const uint32_t* pf_len, const uint32_t* pf_hdr,
+                                                   uint32_t* w1, uint32_t* n1, uint32_t h2[4],
+                                                   uint32_t nx_w, uint32_t nx_n, float* nx_c) {
+    static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
+    const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
+    uint32_t w, a, b, o, j, m, s, sidx, nm1, cls, tiny, sgn;
+    uint64_t p0, p1, p2, p3;
+    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+    uint32_t pfw, pft, pfn;
+    float pfk;
+    uint64_t pfs;
+    u32x4 pfh;
+    const uint64_t pfa = (uint64_t)pf_win, pfb = (uint64_t)pf_len, pfc = (uint64_t)pf_hdr;
+    if constexpr (MAXS == 16) {
+        asm volatile(FREP_ASM_PF2_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v160}"(w0), "+{v163}"(c0),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
+                       [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
+                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(4)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
+                       FREP_ASM_CLOBBER_W16, "vcc", "memory");
+    } else if constexpr (MAXS == 10) {
+        asm volatile(FREP_ASM_PF2_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v112}"(w0), "+{v115}"(c0),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
+                       [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
+                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(4)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
+                       FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
+    } else {
+        asm volatile(FREP_ASM_PF2_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
+                       "+{v96}"(w0), "+{v99}"(c0),
+                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
+                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
+                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
+                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
+                       [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
+                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                       [lg] "n"(3)
+                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
+                       "vcc", "memory");
+    }
+    *w1 = pfw;
+    *n1 = pfn;
+    *nx_c = pfk;
     h2[0] = pfh.x; h2[1] = pfh.y; h2[2] = pfh.z; h2[3] = pfh.w;
     return x;
 }

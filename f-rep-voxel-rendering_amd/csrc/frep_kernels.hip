@@ -781,7 +781,8 @@ __global__ __launch_bounds__(C * LPC) void k_root_interval_batch(const RootArgs*
 // voxels per lane), given the node's header words and the tape's window 0
 // with its constants (w0, c0 as run_point_tape_asm8 takes them).  With PF
 // the interpreter also loads pf_win[lane], pf_len[0] and pf_hdr[0..3] into
-// *w1, *n1 and h2 while it runs.
+// *w1, *n1 and h2 while it runs, and gathers the constants of the window
+// nx_w (nx_n words) into *nx_c.
 template <int MAXS, bool PF = false>
 __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                 uint32_t lane, uint32_t toff, uint32_t ncx,
@@ -790,7 +791,9 @@ __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t nod
                                                 const uint32_t* pf_len = nullptr,
                                                 const uint32_t* pf_hdr = nullptr,
                                                 uint32_t* w1 = nullptr, uint32_t* n1 = nullptr,
-                                                uint32_t* h2 = nullptr, uint32_t n = 0) {
+                                                uint32_t* h2 = nullptr, uint32_t n = 0,
+                                                uint32_t nx_w = 0, uint32_t nx_n = 0,
+                                                float* nx_c = nullptr) {
     const uint32_t dim = a.dim;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
     v8f xs, ys, zs;
@@ -819,8 +822,9 @@ __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t nod
     }
     v8f d;
     if constexpr (PF)
-        d = run_point_tape_asm8_pf<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0,
-                                         n, pf_win, pf_len, pf_hdr, w1, n1, h2);
+        d = run_point_tape_asm8_pf2<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0,
+                                          n, pf_win, pf_len, pf_hdr, w1, n1, h2, nx_w, nx_n,
+                                          nx_c);
     else
         d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0);
 #pragma unroll
@@ -874,41 +878,50 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
     const uint32_t cpw = a.dim * a.dim * a.dim / (64 * K);
     const uint32_t total = n_nodes * cpw;
     if constexpr (ASM) {
-        // Software pipeline over this wavefront's chunks: while the
-        // interpreter runs chunk c it loads the next chunk's tape window (64
-        // words from the tape's first instruction; the pool has 64 words of
-        // padding past its end) and length, and the header of the chunk after
-        // that.  The next chunk's constants are gathered after it, for its
-        // lanes below its length.
+        // Software pipeline over this wavefront's chunks c, c1, c2, c3: while
+        // the interpreter runs chunk c it gathers c1's constants (from c1's
+        // window, loaded during the previous chunk), loads c2's tape window
+        // (64 words from the tape's first instruction; the pool has 64 words
+        // of padding past its end) and length, and c3's header.
         uint32_t c = gwave;
         if (c >= total) return;
         auto header = [&](uint32_t ch, uint32_t h[4]) {
             ctape_t r = as_const(a.nodes + (size_t)(ch / cpw) * a.ns);
             h[0] = r[0]; h[1] = r[1]; h[2] = r[2]; h[3] = r[3];
         };
-        uint32_t h0[4], h1[4];
+        uint32_t h0[4], h1[4], h2[4];
         header(c, h0);
         header(c + nwaves < total ? c + nwaves : c, h1);
+        header(c + 2 * nwaves < total ? c + 2 * nwaves : c, h2);
         uint32_t w0, n0;
         float c0;
         point_window0(a.pool + h0[0], a.consts, lane, &w0, &c0, &n0);
+        uint32_t w1 = a.pool[h1[0] + 1 + lane];
+        uint32_t n1 = c + nwaves < total ? as_const(a.pool + h1[0])[0] : 0u;
         for (;;) {
-            const uint32_t c1 = c + nwaves, c2 = c1 + nwaves;
-            const uint32_t* pf_len = a.pool + (c1 < total ? h1[0] : h0[0]);
-            const uint32_t* pf_hdr = a.nodes + (size_t)((c2 < total ? c2 : c) / cpw) * a.ns;
-            uint32_t w1, n1, h2[4];
+            const uint32_t c1 = c + nwaves, c2 = c1 + nwaves, c3 = c2 + nwaves;
+            const uint32_t* pf_len = a.pool + (c2 < total ? h2[0] : h0[0]);
+            const uint32_t* pf_hdr = a.nodes + (size_t)((c3 < total ? c3 : c) / cpw) * a.ns;
+            uint32_t w2, n2, h3[4];
+            float c0n;
             const uint32_t node = c / cpw;
             point_chunk_asm<MAXS, true>(a, node, c - node * cpw, lane, h0[0], h0[1], h0[2], h0[3],
-                                        w0, c0, pf_len + 1, pf_len, pf_hdr, &w1, &n1, h2, n0);
+                                        w0, c0, pf_len + 1, pf_len, pf_hdr, &w2, &n2, h3, n0, w1,
+                                        n1, &c0n);
             if (c1 >= total) break;
-            c0 = a.consts[(lane < n1 && (w1 & 0xFFu) == 0u) ? (w1 >> 16) : 0u];
+            // the assembly's scalar outputs are wave-uniform: say so, so that
+            // they stay in SGPRs for the next block's scalar inputs
             w0 = w1;
             n0 = n1;
+            c0 = c0n;
+            w1 = w2;
+            n1 = c2 < total ? uni(n2) : 0u;
             c = c1;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
                 h0[q] = h1[q];
                 h1[q] = h2[q];
+                h2[q] = uni(h3[q]);
             }
         }
     } else {

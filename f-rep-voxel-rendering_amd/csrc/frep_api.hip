@@ -820,6 +820,7 @@ BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& 
                     if (L == 0 && root_job(P, j)) continue;
                     ++la.count;
                     la.maxs = std::max(la.maxs, j.t->maxs);
+                    la.slots = std::max(la.slots, j.t->slot_count);
                     la.mm = std::max(la.mm, j.mm_stride);
                 }
                 if (la.count && la.point) {
@@ -984,7 +985,8 @@ frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const Ba
                 HIP_TRY(launch_level_point_batch(gs, b, la.maxs, ctx->cu_count, grid[li],
                                                  la.asm_slots));
             else
-                HIP_TRY(launch_level_interval_batch(gs, b, la.maxs, ctx->cu_count, grid[li]));
+                HIP_TRY(launch_level_interval_batch(gs, b, la.maxs, la.slots, ctx->cu_count,
+                                                    grid[li]));
             if (ev) HIP_TRY(hipEventRecord(ev[1 + li], gs));
         }
         HIP_TRY(launch_gather_counters(gs, d_init, G.count, S->d_cnt + (size_t)G.first * CNT_WORDS));
@@ -1238,8 +1240,8 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
                 else if (L == 0 && use_root)
                     HIP_TRY(launch_root_interval(ctx->stream, ra));
                 else
-                    HIP_TRY(launch_level_interval(ctx->stream, la[L], t->maxs, ctx->cu_count,
-                                                  ctx->level_grid, t->asm_ok));
+                    HIP_TRY(launch_level_interval(ctx->stream, la[L], t->maxs, t->slot_count,
+                                                  ctx->cu_count, ctx->level_grid, t->asm_ok));
                 if (events) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
             }
             HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4,

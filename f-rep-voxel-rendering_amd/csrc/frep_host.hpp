@@ -149,6 +149,7 @@ struct BatchLaunch {
     bool root = false;
     uint32_t cpb = 0, blocks = 0;      // cells per block (all trees), total blocks
     uint32_t variant = 0;              // schedule variant (lanes per cell kRootLanes[variant])
+    uint32_t slots = 0;                // largest slot count of its shapes
     uint32_t asm_slots = 0;            // point launch: every shape on the assembly interpreter with
                                        // at most this many slots (0: not all)
     size_t lds = 0;                    // dynamic LDS per block

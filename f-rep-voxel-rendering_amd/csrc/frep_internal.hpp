@@ -166,8 +166,10 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
 // Level kernels run a persistent grid of min(max_grid, cu_count x resident
 // blocks per CU) blocks; max_grid bounds the per-wave scratch rows.
 // asm_ok: every op of the tape is in the assembly interpreters' set.
+// slot_count: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t cu_count, uint32_t max_grid, bool asm_ok);
+                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
+                                 bool asm_ok);
 // asm_ok: every op of the tape is in the assembly interpreter's set.
 // slots: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
@@ -189,7 +191,7 @@ hipError_t launch_root_interval_batch(hipStream_t st, const RootArgs* d_args,
 // max_grid caps the persistent grid (the autotuner varies it).
 hipError_t launch_init_batch(hipStream_t st, const InitArgs* d_args, uint32_t n);
 hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
-                                       uint32_t cu_count, uint32_t max_grid);
+                                       uint32_t slot_count, uint32_t cu_count, uint32_t max_grid);
 // asm_slots > 0: every shape takes the assembly interpreter (8 voxels per
 // lane) and has at most asm_slots (<= 16) slots.
 hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,

@@ -142,6 +142,15 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 // second pass.  If nothing was removed the child reuses the parent's tape
 // (shorten.comp:188-191).
 // ---------------------------------------------------------------------------
+// Slot index kept inside a register file of MAXS slots (tapes are validated
+// on upload, so this only guards the register-file access): a mask for
+// power-of-two files, a clamp otherwise.
+template <int MAXS>
+__device__ __forceinline__ uint32_t slot_in(uint32_t x) {
+    if constexpr ((MAXS & (MAXS - 1)) == 0) return x & (MAXS - 1);
+    else return x < (uint32_t)MAXS ? x : (uint32_t)(MAXS - 1);
+}
+
 template <int MAXS, bool ASM, class Reserve>
 __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                uint64_t child_base, uint32_t lane, uint4* scr,
@@ -195,8 +204,8 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
             const uint32_t w = rdlane(fw0, j);
             // operand fields are slot indices except for CONST (pool
             // index); masking keeps every register-file access in range
-            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-            const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+            const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
+            const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu), ib = slot_in<MAXS>(w >> 24);
             const Itv A{slo[ia], shi[ia]};
             FREP_FENCE();
             Itv r{0.0f, 0.0f};
@@ -317,9 +326,9 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
                 const int32_t lo = s > 0 ? s : 0;
                 for (int32_t i = e - 1; i >= lo; --i) {
                     const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
-                    const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-                    const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1);
-                    const uint32_t ib = (w >> 24) & (MAXS - 1);
+                    const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
+                    const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu);
+                    const uint32_t ib = slot_in<MAXS>(w >> 24);
                     const bool is_mm = (op == OP_MIN || op == OP_MAX);
                     uint32_t ch = 0;
                     if (is_mm) {
@@ -381,8 +390,12 @@ __device__ __forceinline__ uint64_t child_tape_base(const uint32_t* counters, ui
     return b;
 }
 
+// Slot files of up to 12 slots are built for 8 wavefronts per SIMD (at most
+// 64 VGPRs and a scalar budget to match); larger ones keep the compiler's
+// choice.
 template <int MAXS, bool ASM>
-__global__ __launch_bounds__(kLevelBlock) void k_level_interval(LevelArgs a) {
+__global__ __launch_bounds__(kLevelBlock)
+__attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval(LevelArgs a) {
     // min/max choice ballots of this wavefront's current chunk: mm_stride rows
     // of {FIRST lanes, SECOND lanes} (two 64-bit masks) per wavefront
     extern __shared__ uint4 lds_choice[];
@@ -992,7 +1005,8 @@ __device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
 }  // namespace
 
 template <int MAXS>
-__global__ __launch_bounds__(kLevelBlock) void k_level_interval_batch(BatchArgs b) {
+__global__ __launch_bounds__(kLevelBlock)
+__attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval_batch(BatchArgs b) {
     extern __shared__ uint4 lds_choice[];
     const uint32_t lane = lane_id();
     const uint32_t wib = uni(threadIdx.x >> 6);
@@ -1558,7 +1572,8 @@ size_t interval_lds_bytes(uint32_t mm_stride) {
 constexpr bool kIntervalAsm = false;
 
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t cu_count, uint32_t max_grid, bool asm_ok) {
+                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
+                                 bool asm_ok) {
     const size_t lds = interval_lds_bytes(a.mm_stride);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
     if constexpr (kIntervalAsm) {
@@ -1574,6 +1589,13 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
         }
     }
     (void)asm_ok;
+    // a 12-slot file frees 8 registers per lane over the 16-slot one, which
+    // is what 8 wavefronts per SIMD need
+    if (maxs == 16 && slot_count <= 12) {
+        k_level_interval<12, false><<<interval_grid<12, false>(cu_count, max_grid, lds),
+                                      kLevelBlock, lds, st>>>(a);
+        return hipGetLastError();
+    }
     FREP_DISPATCH_MAXS(maxs, (k_level_interval<M, false><<<interval_grid<M, false>(
                                                                cu_count, max_grid, lds),
                                                            kLevelBlock, lds, st>>>(a)));
@@ -1769,10 +1791,16 @@ hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32
 }
 
 hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
-                                       uint32_t cu_count, uint32_t max_grid) {
+                                       uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
     if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
     const size_t lds = interval_lds_bytes(b.mm_stride);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
+    if (maxs == 16 && slot_count <= 12) {
+        k_level_interval_batch<12><<<batch_grid((const void*)k_level_interval_batch<12>, lds,
+                                                cu_count, max_grid),
+                                     kLevelBlock, lds, st>>>(b);
+        return hipGetLastError();
+    }
     FREP_DISPATCH_MAXS(maxs, (k_level_interval_batch<M><<<
                                   batch_grid((const void*)k_level_interval_batch<M>, lds, cu_count,
                                              max_grid),

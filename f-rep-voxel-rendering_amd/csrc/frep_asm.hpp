@@ -14,13 +14,17 @@
 // pinned operands) and the density leaves in slot 0.
 //
 // Tape words: staged in VGPR windows of 64 words (lane l of W0 holds word
-// s + l), fetched with one v_readlane per instruction; the constant of every
-// CONST word of a window is gathered into C0 once per window, so the loop
-// never waits on a scalar load.  Two windows are kept ahead.
+// s + l); the constant of every CONST word of a window is gathered into C0
+// once per window, so the interpreter never waits on a scalar load.  Two
+// windows are kept ahead.
 //
-// Per instruction: v_readlane + 6 SALU to decode (slot fields masked to
-// MAXS-1), a balanced branch on the op-code bits, then (every operand copy
-// and result move being 4 v_mov_b64)
+// Dispatch is threaded: once per window every lane decodes its own word
+// (slot fields masked to MAXS-1 and scaled by 8, and the address of the
+// word's handler in a table of fixed-size slots, one per op and in-place
+// variant), and each handler reads the next word's decoded fields and
+// handler with two v_readlane before its arithmetic and ends with an
+// s_setpc_b64 to it: one jump per instruction, no decode or compare chain.
+// Handlers (every operand copy and result move being 4 v_mov_b64):
 //   CONST          4 64-bit moves of {c, c} through the DST window;
 //   ADD/SUB/MUL    in place (SRC0+DST window on the output slot) when the
 //                  output slot is one of the operand slots, else operand A
@@ -60,15 +64,16 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
         "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", "v88",    \
         "v89", "v90", "v91", "v92", "v93", "v94", "v95", "v96", "v97", "v98", "v99", "v100"
 // Clobbers of the slot files minus v[WB + 0] and v[WB + 3] (window 0 and
-// its constants, passed in): WB = 96 (8 slots), 112 (10), 160 (16).
+// its constants, passed in): WB = 96 (8 slots), 112 (10), 160 (16); the
+// W lists are the other window registers and NS / NT.
 #define FREP_ASM_CLOBBER_64_95                                                                \
     "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71", "v72", "v73", "v74", "v75", "v76", \
         "v77", "v78", "v79", "v80", "v81", "v82", "v83", "v84", "v85", "v86", "v87", "v88",    \
         "v89", "v90", "v91", "v92", "v93", "v94", "v95"
-#define FREP_ASM_CLOBBER_W8 "v97", "v98", "v100"
+#define FREP_ASM_CLOBBER_W8 "v97", "v98", "v100", "v101", "v102"
 #define FREP_ASM_CLOBBER_101_111                                                              \
     "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110", "v111"
-#define FREP_ASM_CLOBBER_W10 "v113", "v114", "v116"
+#define FREP_ASM_CLOBBER_W10 "v113", "v114", "v116", "v117", "v118"
 #define FREP_ASM_CLOBBER_101_159                                                              \
     "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",            \
         "v111", "v112", "v113", "v114", "v115", "v116", "v117", "v118", "v119", "v120", "v121", \
@@ -76,7 +81,7 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
         "v133", "v134", "v135", "v136", "v137", "v138", "v139", "v140", "v141", "v142", "v143", \
         "v144", "v145", "v146", "v147", "v148", "v149", "v150", "v151", "v152", "v153", "v154", \
         "v155", "v156", "v157", "v158", "v159"
-#define FREP_ASM_CLOBBER_W16 "v161", "v162", "v164"
+#define FREP_ASM_CLOBBER_W16 "v161", "v162", "v164", "v165", "v166"
 #define FREP_ASM_CLOBBER_101_116                                                               \
     "v101", "v102", "v103", "v104", "v105", "v106", "v107", "v108", "v109", "v110",            \
         "v111", "v112", "v113", "v114", "v115", "v116"
@@ -90,18 +95,29 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 
 // Building blocks of the assembly text.  Symbols: FS first slot register,
 // MS slot-file size, LG log2(MS), RA / RB operand and result registers,
-// WB tape windows (W0 W1 W2 = WB+0..2, C0 C1 = WB+3..4).
+// WB tape windows (W0 W1 W2 = WB+0..2, C0 C1 = WB+3..4), NS / NT the
+// decoded fields and handler offsets of the window's words shifted down one
+// lane (lane l: word l + 1).
 // Operand and result moves go two registers at a time (v_mov_b64 issues at
 // the rate of v_mov_b32, and the index window offsets 64-bit operands like
 // 32-bit ones); the slot file, RA and RB start at even registers.
+//
+// A word's decoded fields sit in one SGPR: 8*a in bits 0..7, 8*b in 8..15,
+// 8*o in 16..23.  s_set_gpr_idx_on takes its index from bits 0..7 of the
+// register, so operand A indexes from it directly and B / O after a shift.
+#define FREP_ASM_IDX_B "  s_lshr_b32 %[tmp], %[sel], 8\n"
+#define FREP_ASM_IDX_O "  s_lshr_b32 %[tmp], %[sel], 16\n"
 #define FREP_ASM_READ(R, S)                                                                   \
     "  s_set_gpr_idx_on " S ", gpr_idx(SRC0)\n"                                               \
     "  .irp k, 0,2,4,6\n"                                                                     \
     "    v_mov_b64 v[" R " + \\k:" R " + \\k + 1], v[FS + \\k:FS + \\k + 1]\n"                 \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"
+#define FREP_ASM_READ_A(R) FREP_ASM_READ(R, "%[sel]")
+#define FREP_ASM_READ_B(R) FREP_ASM_IDX_B FREP_ASM_READ(R, "%[tmp]")
 #define FREP_ASM_WRITE(R)                                                                     \
-    "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
     "  .irp k, 0,2,4,6\n"                                                                     \
     "    v_mov_b64 v[FS + \\k:FS + \\k + 1], v[" R " + \\k:" R " + \\k + 1]\n"                 \
     "  .endr\n"                                                                               \
@@ -125,39 +141,122 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  v_cndmask_b32_e32 v[RA + 3], 0, v[RA + 3], vcc\n"                                      \
     "  v_lshlrev_b32_e32 v[RA + 3], 2, v[RA + 3]\n"                                           \
     "  global_load_dword v[" C "], v[RA + 3], %[cst]\n"
-// In-place binary op on the output slot: v[slot] = v[slot] OP v[R].
+// Decode window W0 (words %[s] + lane), all lanes at once: RA+5 the fields,
+// RA+7 the handler offset -- (op & 15) * 1024 + variant * 256, variant 1
+// when the output slot is operand A's, else 2 when it is B's, else 0, kept
+// only for ops with in-place forms (the mask of allowed variants is 2 bits
+// per op in the word 0x1100FC0: ADD, SUB, MUL both, NEG and COPY the first)
+// -- or the end for words past the tape.  NS / NT get them shifted down a
+// lane, NT's last lane the next window's decode (ADV) or the end; the first
+// word's fields and offset go to %[sel] and %[ntg], %[j] = 0.
+#define FREP_ASM_DECODE                                                                       \
+    "  v_mbcnt_lo_u32_b32 v[RA + 1], -1, 0\n"                                                 \
+    "  v_mbcnt_hi_u32_b32 v[RA + 1], -1, v[RA + 1]\n"                                         \
+    "  v_bfe_u32 v[RA + 2], v[WB + 0], 16, LG\n"                                              \
+    "  v_bfe_u32 v[RA + 3], v[WB + 0], 24, LG\n"                                              \
+    "  v_bfe_u32 v[RA + 4], v[WB + 0], 8, LG\n"                                               \
+    "  v_lshlrev_b32_e32 v[RA + 5], 3, v[RA + 2]\n"                                           \
+    "  v_lshl_or_b32 v[RA + 5], v[RA + 3], 11, v[RA + 5]\n"                                   \
+    "  v_lshl_or_b32 v[RA + 5], v[RA + 4], 19, v[RA + 5]\n"                                   \
+    "  v_cmp_eq_u32_e32 vcc, v[RA + 4], v[RA + 3]\n"                                          \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e64 v[RA + 6], 0, 2, vcc\n"                                              \
+    "  v_cmp_eq_u32_e32 vcc, v[RA + 4], v[RA + 2]\n"                                          \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e64 v[RA + 6], v[RA + 6], 1, vcc\n"                                      \
+    "  v_and_b32_e32 v[RA + 7], 15, v[WB + 0]\n"                                              \
+    "  v_lshlrev_b32_e32 v[RA + 2], 1, v[RA + 7]\n"                                           \
+    "  s_mov_b32 %[tmp], 0x1100fc0\n"                                                         \
+    "  v_lshrrev_b32_e64 v[RA + 2], v[RA + 2], %[tmp]\n"                                      \
+    "  v_and_b32_e32 v[RA + 6], v[RA + 6], v[RA + 2]\n"                                       \
+    "  v_lshlrev_b32_e32 v[RA + 7], 10, v[RA + 7]\n"                                          \
+    "  v_lshl_or_b32 v[RA + 7], v[RA + 6], 8, v[RA + 7]\n"                                    \
+    "  v_add_u32_e32 v[RA + 2], %[s], v[RA + 1]\n"                                            \
+    "  v_cmp_gt_u32_e32 vcc, %[n], v[RA + 2]\n"                                               \
+    "  v_mov_b32_e32 v[RA + 3], .Lfa_end%= - .Lfa_base%=\n"                                   \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e32 v[RA + 7], v[RA + 3], v[RA + 7], vcc\n"                              \
+    "  s_nop 1\n"                                                                             \
+    "  v_mov_b32_dpp v[NS], v[RA + 5] wave_shl:1 bound_ctrl:0\n"                              \
+    "  v_mov_b32_dpp v[NT], v[RA + 7] wave_shl:1 bound_ctrl:0\n"                              \
+    "  s_add_u32 %[tmp], %[s], 64\n"                                                          \
+    "  s_cmp_lt_u32 %[tmp], %[n]\n"                                                           \
+    "  s_mov_b32 %[tmp], .Lfa_end%= - .Lfa_base%=\n"                                          \
+    "  s_cselect_b32 %[tmp], .Lfa_adv%= - .Lfa_base%=, %[tmp]\n"                              \
+    "  v_writelane_b32 v[NT], %[tmp], 63\n"                                                   \
+    "  v_readlane_b32 %[sel], v[RA + 5], 0\n"                                                 \
+    "  v_readlane_b32 %[ntg], v[RA + 7], 0\n"                                                 \
+    "  s_mov_b32 %[j], 0\n"                                                                   \
+    "  s_add_u32 vcc_lo, %[blo], %[ntg]\n"                                                    \
+    "  s_addc_u32 vcc_hi, %[bhi], 0\n"                                                        \
+    "  s_setpc_b64 vcc\n"
+// Every handler starts by reading the next word's fields and offset (lane
+// %[j] of NS / NT) and ends by jumping to it.
+#define FREP_ASM_NEXT                                                                         \
+    "  v_readlane_b32 %[nsel], v[NS], %[j]\n"                                                 \
+    "  v_readlane_b32 %[ntg], v[NT], %[j]\n"
+#define FREP_ASM_GO                                                                           \
+    "  s_add_u32 vcc_lo, %[blo], %[ntg]\n"                                                    \
+    "  s_addc_u32 vcc_hi, %[bhi], 0\n"                                                        \
+    "  s_mov_b32 %[sel], %[nsel]\n"                                                           \
+    "  s_add_u32 %[j], %[j], 1\n"                                                             \
+    "  s_setpc_b64 vcc\n"
+// Handler slot of op OP, variant V (1 KiB per op, 256 bytes per variant;
+// .org fails the build if a handler outgrows its slot).
+#define FREP_ASM_SLOT(OP, V) "  .org .Lfa_base%= + " #OP " * 1024 + " #V " * 256\n"
+#define FREP_ASM_NOP FREP_ASM_NEXT FREP_ASM_GO
+// In-place binary op: v[S] = v[S] OP v[R].
 #define FREP_ASM_INPLACE(OPC, S, R)                                                           \
     "  s_set_gpr_idx_on " S ", gpr_idx(SRC0,DST)\n"                                           \
     "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
     "    " OPC " v[FS + \\k], v[FS + \\k], v[" R " + \\k]\n"                                 \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"
-// General binary op: RB = v[b] OP RA, then moved to the output slot.
-#define FREP_ASM_GENERAL(OPC)                                                                 \
-    FREP_ASM_READ("RA", "%[a]")                                                               \
-    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0)\n"                                                \
+// Binary op with variants: 0 RB = v[b] OP RA (RA = a), moved to the output
+// slot; 1 in place on a (B copied to RB); 2 in place on b (A copied to RA).
+// OPC_GEN / OPC_A / OPC_B are the opcodes of the three forms.
+#define FREP_ASM_BINARY(OP, OPC_GEN, OPC_A, OPC_B)                                            \
+    FREP_ASM_SLOT(OP, 0)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM_READ_A("RA")                                                                     \
+    FREP_ASM_IDX_B                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(SRC0)\n"                                              \
     "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
-    "    " OPC " v[RB + \\k], v[FS + \\k], v[RA + \\k]\n"                                    \
+    "    " OPC_GEN " v[RB + \\k], v[FS + \\k], v[RA + \\k]\n"                                \
     "  .endr\n"                                                                               \
-    "  s_set_gpr_idx_off\n" FREP_ASM_WRITE("RB")
-// Commutative op (ADD, MUL): in place on whichever operand slot is the
-// output slot.
-#define FREP_ASM_COMMUTATIVE(NAME, OPC)                                                       \
-    ".Lfa_" NAME "%=:\n"                                                                      \
-    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
-    "  s_cbranch_scc1 .Lfa_" NAME "_oa%=\n"                                                   \
-    "  s_cmp_eq_u32 %[o], %[b]\n"                                                             \
-    "  s_cbranch_scc1 .Lfa_" NAME "_ob%=\n"                                                   \
-    FREP_ASM_GENERAL(OPC)                                                                     \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    ".Lfa_" NAME "_oa%=:\n"                                                                   \
-    FREP_ASM_READ("RB", "%[b]")                                                               \
-    FREP_ASM_INPLACE(OPC, "%[a]", "RB")                                                       \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    ".Lfa_" NAME "_ob%=:\n"                                                                   \
-    FREP_ASM_READ("RA", "%[a]")                                                               \
-    FREP_ASM_INPLACE(OPC, "%[b]", "RA")                                                       \
-    "  s_branch .Lfa_next%=\n"
+    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM_WRITE("RB")                                                                      \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(OP, 1)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM_READ_B("RB")                                                                     \
+    FREP_ASM_INPLACE(OPC_A, "%[sel]", "RB")                                                   \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(OP, 2)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM_READ_A("RA")                                                                     \
+    FREP_ASM_IDX_B                                                                            \
+    FREP_ASM_INPLACE(OPC_B, "%[tmp]", "RA")                                                   \
+    FREP_ASM_GO
+// min(A,B) = B < A ? B : A (LT_X < LT_Y selects RB); four compares into four
+// SGPR pairs, then selects, the result in RB moved to the output slot.
+#define FREP_ASM_MINMAX(LT_X, LT_Y)                                                           \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM_READ_A("RA")                                                                     \
+    FREP_ASM_READ_B("RB")                                                                     \
+    "  .irp q, 0,4\n"                                                                         \
+    "    v_cmp_lt_f32_e64 %[p0], v[" LT_X " + \\q + 0], v[" LT_Y " + \\q + 0]\n"              \
+    "    v_cmp_lt_f32_e64 %[p1], v[" LT_X " + \\q + 1], v[" LT_Y " + \\q + 1]\n"              \
+    "    v_cmp_lt_f32_e64 %[p2], v[" LT_X " + \\q + 2], v[" LT_Y " + \\q + 2]\n"              \
+    "    v_cmp_lt_f32_e64 %[p3], v[" LT_X " + \\q + 3], v[" LT_Y " + \\q + 3]\n"              \
+    "    s_nop 1\n"                                                                           \
+    "    v_cndmask_b32_e64 v[RB + \\q + 0], v[RA + \\q + 0], v[RB + \\q + 0], %[p0]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 1], v[RA + \\q + 1], v[RB + \\q + 1], %[p1]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 2], v[RA + \\q + 2], v[RB + \\q + 2], %[p2]\n"        \
+    "    v_cndmask_b32_e64 v[RB + \\q + 3], v[RA + \\q + 3], v[RB + \\q + 3], %[p3]\n"        \
+    "  .endr\n"                                                                               \
+    FREP_ASM_WRITE("RB")                                                                      \
+    FREP_ASM_GO
 
 #define FREP_ASM_POINT_BODY                                                                   \
     ".set FS, 16\n"                                                                           \
@@ -166,12 +265,19 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     ".set RA, FS + 8*MS\n"                                                                    \
     ".set RB, RA + 8\n"                                                                       \
     ".set WB, RB + 8\n"                                                                       \
+    ".set NS, WB + 5\n"                                                                       \
+    ".set NT, WB + 6\n"                                                                       \
     ".irp k, 0,1,2,3,4,5,6,7\n"                                                               \
     "  v_mov_b32 v[FS + 24 + \\k], %[t]\n"                                                    \
     ".endr\n"                                                                                 \
     "s_mov_b32 %[cls], 0x260\n"                                                               \
     "s_mov_b32 %[tiny], 0xf800000\n"                                                          \
     "s_mov_b32 %[sgn], 0x80000000\n"                                                          \
+    /* handler table base: the address after s_getpc plus the distance */                    \
+    "s_getpc_b64 vcc\n"                                                                       \
+    ".Lfa_pc%=:\n"                                                                            \
+    "s_add_u32 %[blo], vcc_lo, .Lfa_base%= - .Lfa_pc%=\n"                                     \
+    "s_addc_u32 %[bhi], vcc_hi, 0\n"                                                          \
     "s_cmp_eq_u32 %[n], 0\n"                                                                  \
     "s_cbranch_scc1 .Lfa_end%=\n"                                                             \
     "s_sub_u32 %[nm1], %[n], 1\n"                                                             \
@@ -179,7 +285,7 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     /* window 0 and its constants arrive in v[WB + 0] and v[WB + 3]; the                      \
        next two windows are loaded only for tapes longer than 64 words */                     \
     "s_cmp_le_u32 %[n], 64\n"                                                                 \
-    "s_cbranch_scc1 .Lfa_win%=\n"                                                             \
+    "s_cbranch_scc1 .Lfa_dec%=\n"                                                             \
     "s_mov_b32 %[sidx], 64\n"                                                                 \
     FREP_ASM_LOADW("WB + 1")                                                                  \
     "s_mov_b32 %[sidx], 128\n"                                                                \
@@ -187,122 +293,83 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "s_waitcnt vmcnt(0)\n"                                                                    \
     FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
     "s_waitcnt vmcnt(0)\n"                                                                    \
-    ".Lfa_win%=:\n"                                                                           \
-    "  s_sub_u32 %[m], %[n], %[s]\n"                                                          \
-    "  s_min_u32 %[m], %[m], 64\n"                                                            \
-    "  s_mov_b32 %[j], 0\n"                                                                   \
-    ".Lfa_loop%=:\n"                                                                          \
-    "  v_readlane_b32 %[w], v[WB + 0], %[j]\n"                                                \
-    "  s_bfe_u32 %[a], %[w], (LG << 16) | 16\n"                                               \
-    "  s_lshl_b32 %[a], %[a], 3\n"                                                            \
-    "  s_bfe_u32 %[b], %[w], (LG << 16) | 24\n"                                               \
-    "  s_lshl_b32 %[b], %[b], 3\n"                                                            \
-    "  s_bfe_u32 %[o], %[w], (LG << 16) | 8\n"                                                \
-    "  s_lshl_b32 %[o], %[o], 3\n"                                                            \
-    "  s_bitcmp1_b32 %[w], 3\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_hi%=\n"                                                            \
-    "  s_bitcmp1_b32 %[w], 2\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_mid%=\n"                                                           \
-    "  s_bitcmp1_b32 %[w], 1\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_add%=\n"                                                           \
-    /* op 0: CONST -- the window's constant of this word as the pair {c, c} in  \
+    ".Lfa_dec%=:\n"                                                                           \
+    FREP_ASM_DECODE                                                                           \
+    ".Lfa_base%=:\n"                                                                          \
+    /* op 0: CONST -- the window's constant of this word as the pair {c, c} in                \
        vcc, written to the slot two registers at a time */                                    \
+    FREP_ASM_NEXT                                                                             \
     "  v_readlane_b32 vcc_lo, v[WB + 3], %[j]\n"                                              \
+    FREP_ASM_IDX_O                                                                            \
     "  s_mov_b32 vcc_hi, vcc_lo\n"                                                            \
     "  s_nop 1\n"                                                                             \
-    "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
     "  .irp k, 0,2,4,6\n"                                                                     \
     "    v_mov_b64 v[FS + \\k:FS + \\k + 1], vcc\n"                                             \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    /* op 3: ADD */                                                                           \
-    FREP_ASM_COMMUTATIVE("add", "v_add_f32_e32")                                              \
-    /* ops 4 SUB, 5 MUL, 7 MIN */                                                             \
-    ".Lfa_mid%=:\n"                                                                           \
-    "  s_bitcmp1_b32 %[w], 1\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_min%=\n"                                                           \
-    "  s_bitcmp1_b32 %[w], 0\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_mul%=\n"                                                           \
-    /* SUB: a - b as v_sub (A in the window) or v_subrev (B in the window) */                 \
-    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
-    "  s_cbranch_scc1 .Lfa_sub_oa%=\n"                                                        \
-    "  s_cmp_eq_u32 %[o], %[b]\n"                                                             \
-    "  s_cbranch_scc1 .Lfa_sub_ob%=\n"                                                        \
-    FREP_ASM_GENERAL("v_subrev_f32_e32")                                                      \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    ".Lfa_sub_oa%=:\n"                                                                        \
-    FREP_ASM_READ("RB", "%[b]")                                                               \
-    FREP_ASM_INPLACE("v_sub_f32_e32", "%[a]", "RB")                                           \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    ".Lfa_sub_ob%=:\n"                                                                        \
-    FREP_ASM_READ("RA", "%[a]")                                                               \
-    FREP_ASM_INPLACE("v_subrev_f32_e32", "%[b]", "RA")                                        \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    FREP_ASM_COMMUTATIVE("mul", "v_mul_f32_e32")                                              \
-    /* min(A,B) = B < A ? B : A; four compares into four SGPR pairs, then selects */          \
-    ".Lfa_min%=:\n"                                                                           \
-    FREP_ASM_READ("RA", "%[a]")                                                               \
-    FREP_ASM_READ("RB", "%[b]")                                                               \
-    "  .irp q, 0,4\n"                                                                         \
-    "    v_cmp_lt_f32_e64 %[p0], v[RB + \\q + 0], v[RA + \\q + 0]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p1], v[RB + \\q + 1], v[RA + \\q + 1]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p2], v[RB + \\q + 2], v[RA + \\q + 2]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p3], v[RB + \\q + 3], v[RA + \\q + 3]\n"                          \
-    "    s_nop 1\n"                                                                           \
-    "    v_cndmask_b32_e64 v[RB + \\q + 0], v[RA + \\q + 0], v[RB + \\q + 0], %[p0]\n"        \
-    "    v_cndmask_b32_e64 v[RB + \\q + 1], v[RA + \\q + 1], v[RB + \\q + 1], %[p1]\n"        \
-    "    v_cndmask_b32_e64 v[RB + \\q + 2], v[RA + \\q + 2], v[RB + \\q + 2], %[p2]\n"        \
-    "    v_cndmask_b32_e64 v[RB + \\q + 3], v[RA + \\q + 3], v[RB + \\q + 3], %[p3]\n"        \
-    "  .endr\n"                                                                               \
-    FREP_ASM_WRITE("RB")                                                                      \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    /* ops 8 MAX, 10 NEG, 11 SQRT, 12 COPY */                                                 \
-    ".Lfa_hi%=:\n"                                                                            \
-    "  s_bitcmp1_b32 %[w], 2\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_copy%=\n"                                                          \
-    "  s_bitcmp1_b32 %[w], 1\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_negsqrt%=\n"                                                       \
-    /* max(A,B) = A < B ? B : A */                                                            \
-    FREP_ASM_READ("RA", "%[a]")                                                               \
-    FREP_ASM_READ("RB", "%[b]")                                                               \
-    "  .irp q, 0,4\n"                                                                         \
-    "    v_cmp_lt_f32_e64 %[p0], v[RA + \\q + 0], v[RB + \\q + 0]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p1], v[RA + \\q + 1], v[RB + \\q + 1]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p2], v[RA + \\q + 2], v[RB + \\q + 2]\n"                          \
-    "    v_cmp_lt_f32_e64 %[p3], v[RA + \\q + 3], v[RB + \\q + 3]\n"                          \
-    "    s_nop 1\n"                                                                           \
-    "    v_cndmask_b32_e64 v[RB + \\q + 0], v[RA + \\q + 0], v[RB + \\q + 0], %[p0]\n"        \
-    "    v_cndmask_b32_e64 v[RB + \\q + 1], v[RA + \\q + 1], v[RB + \\q + 1], %[p1]\n"        \
-    "    v_cndmask_b32_e64 v[RB + \\q + 2], v[RA + \\q + 2], v[RB + \\q + 2], %[p2]\n"        \
-    "    v_cndmask_b32_e64 v[RB + \\q + 3], v[RA + \\q + 3], v[RB + \\q + 3], %[p3]\n"        \
-    "  .endr\n"                                                                               \
-    FREP_ASM_WRITE("RB")                                                                      \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    ".Lfa_negsqrt%=:\n"                                                                       \
-    "  s_bitcmp1_b32 %[w], 0\n"                                                               \
-    "  s_cbranch_scc1 .Lfa_sqrt%=\n"                                                          \
-    /* NEG: sign flip, in place when the output slot is the operand slot */                   \
-    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
-    "  s_cbranch_scc1 .Lfa_neg_oa%=\n"                                                        \
-    "  s_set_gpr_idx_on %[a], gpr_idx(SRC0)\n"                                                \
+    FREP_ASM_GO                                                                               \
+    /* ops 1 SIN and 2 COS never reach this interpreter */                                    \
+    FREP_ASM_SLOT(1, 0) FREP_ASM_NOP                                                          \
+    FREP_ASM_SLOT(2, 0) FREP_ASM_NOP                                                          \
+    /* op 3 ADD, 4 SUB (a - b as v_sub with A in place, v_subrev otherwise), 5 MUL */         \
+    FREP_ASM_BINARY(3, "v_add_f32_e32", "v_add_f32_e32", "v_add_f32_e32")                     \
+    FREP_ASM_BINARY(4, "v_subrev_f32_e32", "v_sub_f32_e32", "v_subrev_f32_e32")               \
+    FREP_ASM_BINARY(5, "v_mul_f32_e32", "v_mul_f32_e32", "v_mul_f32_e32")                     \
+    FREP_ASM_SLOT(6, 0) FREP_ASM_NOP                                                          \
+    /* op 7 MIN: B < A ? B : A; op 8 MAX: A < B ? B : A */                                    \
+    FREP_ASM_SLOT(7, 0) FREP_ASM_MINMAX("RB", "RA")                                           \
+    FREP_ASM_SLOT(8, 0) FREP_ASM_MINMAX("RA", "RB")                                           \
+    FREP_ASM_SLOT(9, 0) FREP_ASM_NOP                                                          \
+    /* op 10 NEG: sign flip, in place when the output slot is the operand slot */             \
+    FREP_ASM_SLOT(10, 0)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    "  s_set_gpr_idx_on %[sel], gpr_idx(SRC0)\n"                                              \
     "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
     "    v_xor_b32_e64 v[RB + \\k], v[FS + \\k], %[sgn]\n"                                    \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"                                                                   \
     FREP_ASM_WRITE("RB")                                                                      \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    ".Lfa_neg_oa%=:\n"                                                                        \
-    "  s_set_gpr_idx_on %[a], gpr_idx(SRC0,DST)\n"                                            \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(10, 1)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    "  s_set_gpr_idx_on %[sel], gpr_idx(SRC0,DST)\n"                                          \
     "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
     "    v_xor_b32_e64 v[FS + \\k], v[FS + \\k], %[sgn]\n"                                    \
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    /* correctly rounded sqrt, one point at a time: RB+0..4 temporaries, the  \
+    FREP_ASM_GO                                                                               \
+    /* op 11 SQRT: its body follows the table */                                              \
+    FREP_ASM_SLOT(11, 0)                                                                      \
+    "  s_branch .Lfa_sqrt%=\n"                                                                \
+    /* op 12 COPY: moved through RA, nothing to do in place */                                \
+    FREP_ASM_SLOT(12, 0)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM_READ_A("RA")                                                                     \
+    FREP_ASM_WRITE("RA")                                                                      \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(12, 1) FREP_ASM_NOP                                                         \
+    FREP_ASM_SLOT(13, 0) FREP_ASM_NOP                                                         \
+    FREP_ASM_SLOT(14, 0) FREP_ASM_NOP                                                         \
+    FREP_ASM_SLOT(15, 0) FREP_ASM_NOP                                                         \
+    FREP_ASM_SLOT(16, 0)                                                                      \
+    /* next window: the loads issued one window ago have had a window to                      \
+       arrive */                                                                              \
+    ".Lfa_adv%=:\n"                                                                           \
+    "  s_add_u32 %[s], %[s], 64\n"                                                            \
+    "  s_waitcnt vmcnt(0)\n"                                                                  \
+    "  v_mov_b32 v[WB + 0], v[WB + 1]\n"                                                      \
+    "  v_mov_b32 v[WB + 3], v[WB + 4]\n"                                                      \
+    "  v_mov_b32 v[WB + 1], v[WB + 2]\n"                                                      \
+    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
+    "  s_add_u32 %[sidx], %[s], 128\n"                                                        \
+    FREP_ASM_LOADW("WB + 2")                                                                  \
+    FREP_ASM_DECODE                                                                           \
+    /* correctly rounded sqrt, one point at a time: RB+0..4 temporaries, the                  \
        result replaces the operand copy in RA */                                              \
     ".Lfa_sqrt%=:\n"                                                                          \
-    FREP_ASM_READ("RA", "%[a]")                                                               \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM_READ_A("RA")                                                                     \
     "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
     "    v_mul_f32_e32 v[RB + 0], 0x4f800000, v[RA + \\k]\n"                                  \
     "    v_cmp_gt_f32_e32 vcc, %[tiny], v[RA + \\k]\n"                                        \
@@ -327,33 +394,20 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "    v_cndmask_b32_e32 v[RA + \\k], v[RB + 2], v[RB + 0], vcc\n"                          \
     "  .endr\n"                                                                               \
     FREP_ASM_WRITE("RA")                                                                      \
-    "  s_branch .Lfa_next%=\n"                                                                \
-    /* COPY: nothing to do when the output slot is the operand slot */                        \
-    ".Lfa_copy%=:\n"                                                                          \
-    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
-    "  s_cbranch_scc1 .Lfa_next%=\n"                                                          \
-    FREP_ASM_READ("RA", "%[a]")                                                               \
-    FREP_ASM_WRITE("RA")                                                                      \
-    ".Lfa_next%=:\n"                                                                          \
-    "  s_add_u32 %[j], %[j], 1\n"                                                             \
-    "  s_cmp_lt_u32 %[j], %[m]\n"                                                             \
-    "  s_cbranch_scc1 .Lfa_loop%=\n"                                                          \
-    /* next window: the loads issued one window ago have had a window to arrive */            \
-    "  s_add_u32 %[s], %[s], 64\n"                                                            \
-    "  s_cmp_lt_u32 %[s], %[n]\n"                                                             \
-    "  s_cbranch_scc0 .Lfa_done%=\n"                                                          \
-    "  s_waitcnt vmcnt(0)\n"                                                                  \
-    "  v_mov_b32 v[WB + 0], v[WB + 1]\n"                                                      \
-    "  v_mov_b32 v[WB + 3], v[WB + 4]\n"                                                      \
-    "  v_mov_b32 v[WB + 1], v[WB + 2]\n"                                                      \
-    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
-    "  s_add_u32 %[sidx], %[s], 128\n"                                                        \
-    FREP_ASM_LOADW("WB + 2")                                                                  \
-    "  s_branch .Lfa_win%=\n"                                                                 \
-    ".Lfa_done%=:\n"                                                                          \
+    FREP_ASM_GO                                                                               \
     /* no load may still be writing these registers when the block ends */                    \
-    "s_waitcnt vmcnt(0)\n"                                                                    \
-    ".Lfa_end%=:\n"
+    ".Lfa_end%=:\n"                                                                           \
+    "s_waitcnt vmcnt(0)\n"
+
+// Scalar registers of the point interpreter's assembly block.
+#define FREP_ASM_POINT_DECLS                                                                  \
+    uint32_t sel, nsel, ntg, j, tmp, blo, bhi, s, sidx, nm1, cls, tiny, sgn;                  \
+    uint64_t p0, p1, p2, p3
+#define FREP_ASM_POINT_SCALARS                                                                \
+    [sel] "=&s"(sel), [nsel] "=&s"(nsel), [ntg] "=&s"(ntg), [j] "=&s"(j), [tmp] "=&s"(tmp),   \
+        [blo] "=&s"(blo), [bhi] "=&s"(bhi), [s] "=&s"(s), [sidx] "=&s"(sidx),                 \
+        [nm1] "=&s"(nm1), [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn),             \
+        [p0] "=&s"(p0), [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
 
 // Prefetch around the interpreter (run_point_tape_asm8_pf): issued before
 // the body, waited after it, so nothing is in flight when the block ends.
@@ -417,40 +471,27 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
     typedef const __attribute__((address_space(4))) uint32_t* cword_t;
     const uint32_t n = ((cword_t)tape)[0];
     const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
-    uint32_t w, a, b, o, j, m, s, sidx, nm1, cls, tiny, sgn;
-    uint64_t p0, p1, p2, p3;
+    FREP_ASM_POINT_DECLS;
     if constexpr (MAXS == 16) {
         asm volatile(FREP_ASM_POINT_BODY
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v160}"(w0), "+{v163}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v160}"(w0), "+{v163}"(c0),
+                       FREP_ASM_POINT_SCALARS
                      : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
                        FREP_ASM_CLOBBER_W16, "vcc", "memory");
     } else if constexpr (MAXS == 10) {
         asm volatile(FREP_ASM_POINT_BODY
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v112}"(w0), "+{v115}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v112}"(w0), "+{v115}"(c0),
+                       FREP_ASM_POINT_SCALARS
                      : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
                        FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
     } else {
         asm volatile(FREP_ASM_POINT_BODY
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v96}"(w0), "+{v99}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v96}"(w0), "+{v99}"(c0),
+                       FREP_ASM_POINT_SCALARS
                      : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(3)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
@@ -470,20 +511,15 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
                                                    uint32_t* w1, uint32_t* n1, uint32_t h2[4]) {
     static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
     const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
-    uint32_t w, a, b, o, j, m, s, sidx, nm1, cls, tiny, sgn;
-    uint64_t p0, p1, p2, p3;
+    FREP_ASM_POINT_DECLS;
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     uint32_t pfw, pft, pfn;
     u32x4 pfh;
     const uint64_t pfa = (uint64_t)pf_win, pfb = (uint64_t)pf_len, pfc = (uint64_t)pf_hdr;
     if constexpr (MAXS == 16) {
         asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v160}"(w0), "+{v163}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v160}"(w0), "+{v163}"(c0),
+                       FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
                      : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
@@ -491,12 +527,8 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
                        FREP_ASM_CLOBBER_W16, "vcc", "memory");
     } else if constexpr (MAXS == 10) {
         asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v112}"(w0), "+{v115}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v112}"(w0), "+{v115}"(c0),
+                       FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
                      : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
@@ -504,12 +536,8 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
                        FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
     } else {
         asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v96}"(w0), "+{v99}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v96}"(w0), "+{v99}"(c0),
+                       FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
                      : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
                        [lg] "n"(3)
@@ -534,8 +562,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
                                                    uint32_t nx_w, uint32_t nx_n, float* nx_c) {
     static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
     const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
-    uint32_t w, a, b, o, j, m, s, sidx, nm1, cls, tiny, sgn;
-    uint64_t p0, p1, p2, p3;
+    FREP_ASM_POINT_DECLS;
     typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
     uint32_t pfw, pft, pfn;
     float pfk;
@@ -544,12 +571,8 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
     const uint64_t pfa = (uint64_t)pf_win, pfb = (uint64_t)pf_len, pfc = (uint64_t)pf_hdr;
     if constexpr (MAXS == 16) {
         asm volatile(FREP_ASM_PF2_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v160}"(w0), "+{v163}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v160}"(w0), "+{v163}"(c0),
+                       FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
                        [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
                      : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
@@ -558,12 +581,8 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
                        FREP_ASM_CLOBBER_W16, "vcc", "memory");
     } else if constexpr (MAXS == 10) {
         asm volatile(FREP_ASM_PF2_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v112}"(w0), "+{v115}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v112}"(w0), "+{v115}"(c0),
+                       FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
                        [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
                      : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
@@ -572,12 +591,8 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
                        FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
     } else {
         asm volatile(FREP_ASM_PF2_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), [w] "=&s"(w),
-                       "+{v96}"(w0), "+{v99}"(c0),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3),
+                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v96}"(w0), "+{v99}"(c0),
+                       FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
                        [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
                      : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),

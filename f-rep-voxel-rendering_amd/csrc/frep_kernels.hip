@@ -930,12 +930,9 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
             w1 = w2;
             n1 = c2 < total ? uni(n2) : 0u;
             c = c1;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                h0[q] = h1[q];
-                h1[q] = h2[q];
-                h2[q] = uni(h3[q]);
-            }
+            h0[0] = h1[0]; h0[1] = h1[1]; h0[2] = h1[2]; h0[3] = h1[3];
+            h1[0] = h2[0]; h1[1] = h2[1]; h1[2] = h2[2]; h1[3] = h2[3];
+            h2[0] = uni(h3[0]); h2[1] = uni(h3[1]); h2[2] = uni(h3[2]); h2[3] = uni(h3[3]);
         }
     } else {
         for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {

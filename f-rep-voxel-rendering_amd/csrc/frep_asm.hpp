@@ -22,8 +22,8 @@
 // (slot fields masked to MAXS-1 and scaled by 8, and the address of the
 // word's handler in a table of fixed-size slots, one per op and in-place
 // variant), and each handler reads the next word's decoded fields and
-// handler with two v_readlane before its arithmetic and ends with an
-// s_setpc_b64 to it: one jump per instruction, no decode or compare chain.
+// handler with two v_readlane and ends with an s_setpc_b64 to it: one jump
+// per instruction, no decode or compare chain.
 // Handlers (every operand copy and result move being 4 v_mov_b64):
 //   CONST          4 64-bit moves of {c, c} through the DST window;
 //   ADD/SUB/MUL    in place (SRC0+DST window on the output slot) when the
@@ -190,15 +190,14 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  s_add_u32 vcc_lo, %[blo], %[ntg]\n"                                                    \
     "  s_addc_u32 vcc_hi, %[bhi], 0\n"                                                        \
     "  s_setpc_b64 vcc\n"
-// Every handler starts by reading the next word's fields and offset (lane
-// %[j] of NS / NT) and ends by jumping to it.
-#define FREP_ASM_NEXT                                                                         \
-    "  v_readlane_b32 %[nsel], v[NS], %[j]\n"                                                 \
-    "  v_readlane_b32 %[ntg], v[NT], %[j]\n"
+// Every handler starts by reading the next word's handler offset (lane %[j]
+// of NT) and ends, once it is done with %[sel], by reading the next word's
+// fields into it (lane %[j] of NS) and jumping to the handler.
+#define FREP_ASM_NEXT "  v_readlane_b32 %[ntg], v[NT], %[j]\n"
 #define FREP_ASM_GO                                                                           \
+    "  v_readlane_b32 %[sel], v[NS], %[j]\n"                                                  \
     "  s_add_u32 vcc_lo, %[blo], %[ntg]\n"                                                    \
     "  s_addc_u32 vcc_hi, %[bhi], 0\n"                                                        \
-    "  s_mov_b32 %[sel], %[nsel]\n"                                                           \
     "  s_add_u32 %[j], %[j], 1\n"                                                             \
     "  s_setpc_b64 vcc\n"
 // Handler slot of op OP, variant V (1 KiB per op, 256 bytes per variant;
@@ -401,10 +400,10 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 
 // Scalar registers of the point interpreter's assembly block.
 #define FREP_ASM_POINT_DECLS                                                                  \
-    uint32_t sel, nsel, ntg, j, tmp, blo, bhi, s, sidx, nm1, cls, tiny, sgn;                  \
+    uint32_t sel, ntg, j, tmp, blo, bhi, s, sidx, nm1, cls, tiny, sgn;                        \
     uint64_t p0, p1, p2, p3
 #define FREP_ASM_POINT_SCALARS                                                                \
-    [sel] "=&s"(sel), [nsel] "=&s"(nsel), [ntg] "=&s"(ntg), [j] "=&s"(j), [tmp] "=&s"(tmp),   \
+    [sel] "=&s"(sel), [ntg] "=&s"(ntg), [j] "=&s"(j), [tmp] "=&s"(tmp),                       \
         [blo] "=&s"(blo), [bhi] "=&s"(bhi), [s] "=&s"(s), [sidx] "=&s"(sidx),                 \
         [nm1] "=&s"(nm1), [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn),             \
         [p0] "=&s"(p0), [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)

@@ -92,6 +92,33 @@ __device__ __forceinline__ void store_mask64(uint32_t* dst, uint64_t m, uint32_t
     if (lane < 2) dst[lane] = lane ? (uint32_t)(m >> 32) : (uint32_t)m;
 }
 
+// Eight consecutive 64-bit masks as one store: word i of m[0..7] is placed in
+// lane i, and lanes 0..15 write it.
+__device__ __forceinline__ void store_mask64x8(uint32_t* dst, const uint64_t (&m)[8],
+                                               uint32_t lane) {
+    uint32_t v;
+    // the masks were just written by VALU compares: wait states before VALU
+    // reads them
+    asm volatile(
+        "s_nop 4\n"
+        "v_writelane_b32 %0, %1, 0\n v_writelane_b32 %0, %2, 1\n"
+        "v_writelane_b32 %0, %3, 2\n v_writelane_b32 %0, %4, 3\n"
+        "v_writelane_b32 %0, %5, 4\n v_writelane_b32 %0, %6, 5\n"
+        "v_writelane_b32 %0, %7, 6\n v_writelane_b32 %0, %8, 7\n"
+        "v_writelane_b32 %0, %9, 8\n v_writelane_b32 %0, %10, 9\n"
+        "v_writelane_b32 %0, %11, 10\n v_writelane_b32 %0, %12, 11\n"
+        "v_writelane_b32 %0, %13, 12\n v_writelane_b32 %0, %14, 13\n"
+        "v_writelane_b32 %0, %15, 14\n v_writelane_b32 %0, %16, 15\n"
+        : "=v"(v)
+        : "s"((uint32_t)m[0]), "s"((uint32_t)(m[0] >> 32)), "s"((uint32_t)m[1]),
+          "s"((uint32_t)(m[1] >> 32)), "s"((uint32_t)m[2]), "s"((uint32_t)(m[2] >> 32)),
+          "s"((uint32_t)m[3]), "s"((uint32_t)(m[3] >> 32)), "s"((uint32_t)m[4]),
+          "s"((uint32_t)(m[4] >> 32)), "s"((uint32_t)m[5]), "s"((uint32_t)(m[5] >> 32)),
+          "s"((uint32_t)m[6]), "s"((uint32_t)(m[6] >> 32)), "s"((uint32_t)m[7]),
+          "s"((uint32_t)(m[7] >> 32)));
+    if (lane < 16) dst[lane] = v;
+}
+
 // Cell (x, y, z) of mask bit idx: INDEX3D = z + dim*y + dim*dim*x
 // (vulkan/shaders/utils.comp:6).  Shifts when dim is a power of two.
 __device__ __forceinline__ void cell_of(uint32_t idx, uint32_t dim, uint32_t sh, uint32_t* x,
@@ -840,11 +867,10 @@ __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t nod
                                           nx_c);
     else
         d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0);
+    uint64_t B[8];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const uint64_t B = __ballot(d[k] < 0.0f);
-        store_mask64(rec + 4 + (sub * 8 + (uint32_t)k) * 2, B, lane);
-    }
+    for (int k = 0; k < 8; ++k) B[k] = __ballot(d[k] < 0.0f);
+    store_mask64x8(rec + 4 + sub * 16, B, lane);
 }
 
 template <int MAXS, int K, bool ASM>

@@ -300,6 +300,7 @@ void free_tree_buffers(frep_ctx* ctx, TreeObj* t) {
 
 frep_status ensure_level(frep_ctx* ctx, TreeObj* t, uint32_t L, uint32_t cap) {
     if (t->d_nodes[L] && t->node_cap[L] >= cap) return FREP_OK;
+    ++t->dev_gen;
     ctx->dpool.free(t->d_nodes[L]);
     t->d_nodes[L] = nullptr;
     t->node_cap[L] = 0;
@@ -312,6 +313,7 @@ frep_status ensure_level(frep_ctx* ctx, TreeObj* t, uint32_t L, uint32_t cap) {
 
 frep_status ensure_pool(frep_ctx* ctx, TreeObj* t, uint64_t words) {
     if (t->d_pool && t->pool_cap >= words) return FREP_OK;
+    ++t->dev_gen;
     ctx->dpool.free(t->d_pool);
     t->d_pool = nullptr;
     t->pool_cap = 0;
@@ -596,6 +598,7 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
                             std::equal(grid->dims, grid->dims + nl, t->grid.dims);
     if (!same_shape) {
         // record sizes change with the dims: drop the level buffers
+        ++t->dev_gen;
         for (uint32_t L = 0; L < kMaxLevels; ++L) {
             ctx->dpool.free(t->d_nodes[L]);
             t->d_nodes[L] = nullptr;
@@ -855,6 +858,7 @@ void fill_batch(const BatchPlan& P, const std::vector<Job>& jobs, const std::vec
         InitArgs* ia = reinterpret_cast<InitArgs*>(hs + G.init_off);
         for (uint32_t k = 0; k < G.count; ++k) {
             const Job& j = jobs[idx[G.first + k]];
+            ++j.t->dev_gen;   // the batched build rewrites the tree's buffers
             ia[k] = InitArgs{j.t->d_counters, j.t->d_nodes[0], j.t->d_pool, j.tp->d_words, kRootOff,
                              0};
         }
@@ -1215,6 +1219,13 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         }
         LevelArgs la[kMaxLevels];
         for (uint32_t L = 0; L < nl; ++L) la[L] = level_args(t, L, j.mm_stride);
+        la[nl - 1].finish = 1;
+        la[nl - 1].host_counters = t->h_counters;
+        // the buffers are as k_init_tree leaves them for this tape when the
+        // last plain build of it into them completed (finish_tree)
+        const bool skip_init = t->ready_gen == t->dev_gen && t->ready_tape == tp->uid;
+        t->ready_gen = 0;   // set again only when this build completes
+        t->h_counters[CNT_DONE] = 0;   // finish_tree marks the mirror written
         // level 0 in passes over the tape's instructions when the tape has
         // a schedule (k_root_interval)
         const uint32_t d0 = grid->dims[0];
@@ -1230,8 +1241,9 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         // an event before the init and after each launch
         auto enqueue = [&](bool events) -> frep_status {
             if (events) HIP_TRY(hipEventRecord(ctx->ev[0], ctx->stream));
-            HIP_TRY(launch_init_tree(ctx->stream, t->d_counters, t->d_nodes[0], t->d_pool,
-                                     tp->d_words, kRootOff, nl));
+            if (!skip_init)
+                HIP_TRY(launch_init_tree(ctx->stream, t->d_counters, t->d_nodes[0], t->d_pool,
+                                         tp->d_words, kRootOff, nl));
             if (events) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
             for (uint32_t L = 0; L < nl; ++L) {
                 if (L + 1 == nl)
@@ -1244,8 +1256,6 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
                                                   ctx->cu_count, ctx->level_grid, t->asm_ok));
                 if (events) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
             }
-            HIP_TRY(hipMemcpyAsync(t->h_counters, t->d_counters, CNT_WORDS * 4,
-                                   hipMemcpyDeviceToHost, ctx->stream));
             return FREP_OK;
         };
         // a repeated voxelization (same buffers, tape, grid and time) is
@@ -1256,9 +1266,10 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         uint64_t key = 0;
         if (!ctx->timing && ctx->graphs) {
             key = fnv1a(la, sizeof(LevelArgs) * nl);
-            const uintptr_t v[9] = {(uintptr_t)tp->d_words, (uintptr_t)t->h_counters, nl,
-                                    t->maxs, t->slot_count, t->asm_ok ? 1u : 0u,
-                                    ctx->cu_count, ctx->level_grid, (uintptr_t)(rv + 1)};
+            const uintptr_t v[10] = {(uintptr_t)tp->d_words, (uintptr_t)t->h_counters, nl,
+                                     t->maxs, t->slot_count, t->asm_ok ? 1u : 0u,
+                                     ctx->cu_count, ctx->level_grid, (uintptr_t)(rv + 1),
+                                     skip_init ? 1u : 0u};
             key = fnv1a(v, sizeof(v), key);
         }
         const bool replay = key != 0 && ((t->vexec && t->vkey == key) || t->vkey_seen == key);
@@ -1287,7 +1298,14 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
             HIP_TRY(hipGraphLaunch(t->vexec, ctx->stream));
         }
         HIP_TRY(hipStreamSynchronize(ctx->stream));
-        if (t->h_counters[CNT_OVERFLOW] == 0) break;
+        if (t->h_counters[CNT_DONE] != kCountersWritten)
+            return fail(ctx, FREP_DEVICE, "the build's counters did not reach the host");
+        if (t->h_counters[CNT_OVERFLOW] == 0) {
+            t->ready_gen = t->dev_gen;
+            t->ready_tape = tp->uid;
+            break;
+        }
+        t->ready_gen = 0;
         const frep_status s = regrow_job(ctx, &j, attempt);
         if (s != FREP_OK) return s;
     }

@@ -68,6 +68,14 @@ struct TreeObj {
     hipGraphExec_t vexec = nullptr;
     uint64_t vkey = 0;
     uint64_t vkey_seen = 0;            // key of the last direct launch: captured when it repeats
+    // Device-state generation: bumped whenever a node or pool buffer is
+    // (re)allocated or a batched build writes the tree.  A plain build that
+    // completes without overflow leaves the buffers as k_init_tree would for
+    // its tape (finish_tree) and records (generation, tape upload); a plain
+    // build that finds both unchanged skips the init launch.
+    uint64_t dev_gen = 1;
+    uint64_t ready_gen = 0;
+    uint64_t ready_tape = 0;
 };
 
 // Caching sub-allocator (frep_pool.hip): device or pinned host blocks carved

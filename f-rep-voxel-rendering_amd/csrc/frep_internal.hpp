@@ -27,9 +27,11 @@ enum : uint32_t { MM_NONE = 0, MM_FIRST = 1, MM_SECOND = 2, MM_BOTH = 3 };
 //   [17]        overflow flags, two bits per level L (bits 2L, 2L+1): the
 //               launch of level L ran out of node capacity (OVF_NODES) or
 //               tape pool (OVF_POOL) for level L+1
+//   [18]        blocks of the last level's launch that have finished
+//               (finish_tree)
 // Level L's tapes occupy [base_L, base_L + words_L) with base_1 = [16] and
 // base_{L+1} = base_L + words_L.
-enum : uint32_t { CNT_ALLOC = 0, CNT_BASE1 = 16, CNT_OVERFLOW = 17, CNT_WORDS = 32 };
+enum : uint32_t { CNT_ALLOC = 0, CNT_BASE1 = 16, CNT_OVERFLOW = 17, CNT_DONE = 18, CNT_WORDS = 32 };
 enum : uint32_t { OVF_NODES = 1u, OVF_POOL = 2u };
 // Overflow bits of the levels before `level`: a launch of `level` has nothing
 // to do for a tree whose earlier level overflowed (the host rebuilds it).
@@ -73,7 +75,12 @@ struct LevelArgs {
     float tape_time;
     uint32_t pk;              // point level: voxels per lane (8, 4 or 1)
     uint32_t use_asm;         // point level: assembly interpreter (pk == 8, slot file <= 16)
+    uint32_t finish;          // last level of a plain build: finish_tree (counters copied
+                              // to host_counters, then reset for the next build)
+    uint32_t* host_counters;  // pinned host mirror of the counters (finish)
 };
+// Word CNT_DONE of a host mirror written by finish_tree.
+constexpr uint32_t kCountersWritten = 0xC0DE0001u;
 
 // Root level evaluated in passes over the tape's instructions
 // (k_root_interval).  sched holds, in u32 words:

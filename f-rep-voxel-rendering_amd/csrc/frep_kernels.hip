@@ -906,8 +906,8 @@ __device__ __forceinline__ void point_chunk(const LevelArgs& a, uint32_t node, u
     }
 }
 
-template <int MAXS, int K, bool ASM = false>
-__global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
+template <int MAXS, int K, bool ASM>
+__device__ __forceinline__ void level_point_body(const LevelArgs& a) {
     if ((load_counter(a.counters + CNT_OVERFLOW) & ovf_before(a.level)) != 0) return;
     const uint32_t lane = lane_id();
     const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
@@ -966,6 +966,39 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
             point_chunk<MAXS, K, ASM>(a, node, sub, lane);
         }
     }
+}
+
+// End of a tree's last level (a.finish): every block arrives once; the last
+// to arrive writes the counters to the host mirror a.host_counters (system
+// scope, word CNT_DONE replaced by kCountersWritten so the host can tell the
+// copy arrived) and, when the build did not overflow, resets them to what
+// k_init_tree writes for the same tape (all zero but CNT_BASE1), so the next
+// build of that tape into the same buffers needs no init launch.  The
+// arrival count CNT_DONE is zero again afterwards either way.  No fence on
+// arrival: the counters are read-only during this launch (its waves read
+// them before arriving), and the reset is visible to later launches at the
+// launch boundary.
+__device__ __forceinline__ void finish_tree(const LevelArgs& a) {
+    __shared__ uint32_t last;
+    __syncthreads();
+    if (threadIdx.x == 0) last = atomicAdd(a.counters + CNT_DONE, 1u) == gridDim.x - 1 ? 1u : 0u;
+    __syncthreads();
+    if (last == 0 || threadIdx.x >= 64) return;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t v = lane < CNT_WORDS ? a.counters[lane] : 0u;
+    const uint32_t ovf = (uint32_t)__shfl((int)v, CNT_OVERFLOW);
+    if (lane < CNT_WORDS) {
+        __hip_atomic_store(a.host_counters + lane, lane == CNT_DONE ? kCountersWritten : v,
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        if ((ovf == 0 && lane != CNT_BASE1) || lane == CNT_DONE) a.counters[lane] = 0u;
+    }
+    __threadfence_system();
+}
+
+template <int MAXS, int K, bool ASM = false>
+__global__ __launch_bounds__(kLevelBlock) void k_level_point(LevelArgs a) {
+    level_point_body<MAXS, K, ASM>(a);
+    if (a.finish) finish_tree(a);
 }
 
 // ---------------------------------------------------------------------------

@@ -291,6 +291,53 @@ def test_voxelize_deterministic_and_reusable(ctx, oracle):
     ctx.tree_free(tree)
 
 
+def test_builds_without_init_stay_exact(ctx, oracle):
+    """A plain build leaves the tree's counters as the init kernel would for
+    its tape, and the next plain build of that tape into the same buffers
+    skips the init launch.  Every build below, whichever path preceded it
+    (repeat, batched build of another tape into the same tree, regrowth,
+    another tape, a grid with more nodes), equals the oracle's tree, and a
+    render of a tree built that way equals one of a fresh tree."""
+    a = shapes.random_csg(24, 2)
+    b = shapes.spheres_union(8, 1)
+    ga = shapes.make_grid([16, 8, 8])
+    gb = shapes.make_grid([8, 8, 4])
+    want_a = oracle.Tree(a, ga).canonical()
+    want_b = oracle.Tree(b, gb).canonical()
+    ha, hb = ctx.upload(a), ctx.upload(b)
+    tree = ctx.tree_create()
+    for k in range(3):
+        ctx.voxelize(ha, ga, tree)
+        assert np.array_equal(ctx.export(tree), want_a), k
+        assert list(ctx.tree_info(tree).node_count[:3]) == oracle.Tree(a, ga).counts(), k
+    ctx.voxelize_batch([hb], [gb], [tree])
+    assert np.array_equal(ctx.export(tree), want_b)
+    for k in range(2):
+        ctx.voxelize(ha, ga, tree)
+        assert np.array_equal(ctx.export(tree), want_a), ("after batch", k)
+    g2 = shapes.make_grid([16, 16, 8])
+    ctx.voxelize(ha, g2, tree)
+    assert np.array_equal(ctx.export(tree), oracle.Tree(a, g2).canonical())
+    ctx.voxelize(hb, gb, tree)
+    ctx.voxelize(hb, gb, tree)
+    assert np.array_equal(ctx.export(tree), want_b)
+    for k in range(2):
+        ctx.voxelize(ha, ga, tree)
+        assert np.array_equal(ctx.export(tree), want_a), ("after b", k)
+    basis = shapes.camera_basis((1.8, 1.2, 2.4), (-0.55, -0.35, -0.75), (0, 1, 0), 45.0, 96, 64,
+                                ortho=False, extent=(2.4, 2.4))
+    cam = lib.make_camera(basis, 96, 64, spp=1, ortho=False, jitter=False, seed=7)
+    rgba, hits, nrm = ctx.render(tree, cam)
+    fresh = ctx.voxelize(ha, ga)
+    frgba, fhits, fnrm = ctx.render(fresh, cam)
+    assert (hits != MISS).sum() > 0
+    assert np.array_equal(rgba, frgba)
+    assert np.array_equal(hits, fhits)
+    assert same_bits(nrm, fnrm)
+    ctx.tree_free(fresh)
+    ctx.tree_free(tree)
+
+
 def test_edge_shapes(ctx, oracle):
     b = Builder()
     empty = compile_tape(b, b.add(b.const(1.0), b.mul(b.x(), b.const(0.0))))

@@ -488,10 +488,21 @@ frep_status frep_tape_upload(frep_ctx* ctx, const uint32_t* words, uint32_t n_wo
     t->slot_count = slot_count;
     t->maxs = maxs_for(slot_count);
     t->asm_ok = true;
+    // does any instruction read slot 3 while it still holds the input t?
+    // Shortening keeps the instruction that last wrote each operand a kept
+    // instruction reads, so a shortened tape reads t only if its root does.
+    bool t_written = false;
+    t->reads_t = false;
     for (uint32_t k = 0; k < n; ++k) {
-        const uint32_t op = words[1 + k] & 0xFFu;
+        const uint32_t w = words[1 + k], op = w & 0xFFu;
         t->n_mm += (op == OP_MIN || op == OP_MAX);
         t->asm_ok = t->asm_ok && asm_op_ok(op);
+        if (op != OP_CONST && !t_written) {
+            const bool binary = op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV ||
+                                op == OP_MIN || op == OP_MAX;
+            if (((w >> 16) & 0xFFu) == 3u || (binary && (w >> 24) == 3u)) t->reads_t = true;
+        }
+        if (((w >> 8) & 0xFFu) == 3u) t_written = true;
     }
     // padded device copies: scalar loads may read a few words past the end
     HIP_TRY(hipMalloc(&t->d_words, (size_t)(n + 1 + kPoolPad) * 4));
@@ -610,6 +621,7 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
     t->slot_count = tp->slot_count;
     t->maxs = tp->maxs;
     t->asm_ok = tp->asm_ok && !ctx->no_asm;
+    t->reads_t = tp->reads_t;
     // cell sizes by sequential float division (voxelizer.cpp:210-216)
     float c = grid->world_size;
     for (uint32_t L = 0; L < nl; ++L) {
@@ -681,6 +693,7 @@ LevelArgs level_args(const TreeObj* t, uint32_t L, uint32_t mm_stride) {
     a.tape_time = g.tape_time;
     a.pk = 1;
     a.use_asm = 0;
+    a.reads_t = t->reads_t ? 1u : 0u;
     return a;
 }
 

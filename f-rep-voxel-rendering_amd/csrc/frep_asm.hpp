@@ -266,9 +266,13 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     ".set WB, RB + 8\n"                                                                       \
     ".set NS, WB + 5\n"                                                                       \
     ".set NT, WB + 6\n"                                                                       \
+    /* slot 3 is the input t when the tape reads it before writing it */                      \
+    "s_cmp_eq_u32 %[ut], 0\n"                                                                 \
+    "s_cbranch_scc1 .Lfa_nt%=\n"                                                              \
     ".irp k, 0,1,2,3,4,5,6,7\n"                                                               \
     "  v_mov_b32 v[FS + 24 + \\k], %[t]\n"                                                    \
     ".endr\n"                                                                                 \
+    ".Lfa_nt%=:\n"                                                                            \
     "s_mov_b32 %[cls], 0x260\n"                                                               \
     "s_mov_b32 %[tiny], 0xf800000\n"                                                          \
     "s_mov_b32 %[sgn], 0x80000000\n"                                                          \
@@ -460,12 +464,14 @@ __device__ __forceinline__ void point_window0(const uint32_t* tape, const float*
 // x, y, z: the 8 points of this lane (one register each); returns their
 // densities.  tape: [n, instrs]; consts: the shape's pool (index 0 readable
 // even when empty).  w0, c0: the first window (lane l: instruction word
-// min(l, n - 1)) and its constants (point_window0).  MAXS must be 8, 10 or
-// 16.  Every lane of the wavefront must be active.
+// min(l, n - 1)) and its constants (point_window0).  use_t: slot 3 is
+// filled with t (0 when the tape writes slot 3 before any instruction reads
+// it).  MAXS must be 8, 10 or 16.  Every lane of the wavefront must be
+// active.
 template <int MAXS>
 __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const float* consts,
                                                    float t, v8f x, v8f y, v8f z, uint32_t w0,
-                                                   float c0) {
+                                                   float c0, uint32_t use_t = 1) {
     static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
     typedef const __attribute__((address_space(4))) uint32_t* cword_t;
     const uint32_t n = ((cword_t)tape)[0];
@@ -475,7 +481,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v160}"(w0), "+{v163}"(c0),
                        FREP_ASM_POINT_SCALARS
-                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
                        FREP_ASM_CLOBBER_W16, "vcc", "memory");
@@ -483,7 +489,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v112}"(w0), "+{v115}"(c0),
                        FREP_ASM_POINT_SCALARS
-                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
                        FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
@@ -491,7 +497,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
         asm volatile(FREP_ASM_POINT_BODY
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v96}"(w0), "+{v99}"(c0),
                        FREP_ASM_POINT_SCALARS
-                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(3)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
                        "vcc", "memory");
@@ -507,7 +513,8 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
                                                    float t, v8f x, v8f y, v8f z, uint32_t w0,
                                                    float c0, uint32_t n, const uint32_t* pf_win,
                                                    const uint32_t* pf_len, const uint32_t* pf_hdr,
-                                                   uint32_t* w1, uint32_t* n1, uint32_t h2[4]) {
+                                                   uint32_t* w1, uint32_t* n1, uint32_t h2[4],
+                                                   uint32_t use_t = 1) {
     static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
     const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
     FREP_ASM_POINT_DECLS;
@@ -520,7 +527,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v160}"(w0), "+{v163}"(c0),
                        FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
-                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
                        FREP_ASM_CLOBBER_W16, "vcc", "memory");
@@ -529,7 +536,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v112}"(w0), "+{v115}"(c0),
                        FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
-                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
                        FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
@@ -538,7 +545,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, cons
                      : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v96}"(w0), "+{v99}"(c0),
                        FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
-                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(3)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
                        "vcc", "memory");
@@ -558,7 +565,8 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
                                                    float c0, uint32_t n, const uint32_t* pf_win,
                                                    const uint32_t* pf_len, const uint32_t* pf_hdr,
                                                    uint32_t* w1, uint32_t* n1, uint32_t h2[4],
-                                                   uint32_t nx_w, uint32_t nx_n, float* nx_c) {
+                                                   uint32_t nx_w, uint32_t nx_n, float* nx_c,
+                                                   uint32_t use_t = 1) {
     static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
     const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
     FREP_ASM_POINT_DECLS;
@@ -574,7 +582,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
                        FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
                        [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
-                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
                        FREP_ASM_CLOBBER_W16, "vcc", "memory");
@@ -584,7 +592,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
                        FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
                        [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
-                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(4)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
                        FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
@@ -594,7 +602,7 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
                        FREP_ASM_POINT_SCALARS,
                        [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh),
                        [pfk] "=&v"(pfk), [pfs] "=&s"(pfs)
-                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [maxs] "n"(MAXS),
+                     : [pw1] "v"(nx_w), [pn1] "s"(nx_n), [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
                        [lg] "n"(3)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
                        "vcc", "memory");

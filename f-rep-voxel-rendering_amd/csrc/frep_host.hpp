@@ -37,6 +37,7 @@ struct TapeObj {
     std::vector<float> consts;
     uint32_t slot_count = 0, maxs = 0, n_mm = 0;
     bool asm_ok = false;               // all ops in the assembly interpreter's set
+    bool reads_t = true;               // some instruction reads slot 3 (t) before any writes it
     uint32_t* d_words = nullptr;
     float* d_consts = nullptr;
     uint64_t uid = 0;                  // unique per upload in its context
@@ -61,6 +62,7 @@ struct TreeObj {
     uint32_t* h_counters = nullptr;    // pinned mirror
     uint32_t slot_count = 0, maxs = 0;
     bool asm_ok = false;
+    bool reads_t = true;               // the tape reads the input t (slot 3)
     uint32_t retries = 0;
     uint64_t consts_uid = 0;           // upload whose constants d_consts holds (0: none)
     // the plain voxelization's launches (init, levels, counter readback) as

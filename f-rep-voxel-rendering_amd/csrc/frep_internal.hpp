@@ -78,6 +78,8 @@ struct LevelArgs {
     uint32_t finish;          // last level of a plain build: finish_tree (counters copied
                               // to host_counters, then reset for the next build)
     uint32_t* host_counters;  // pinned host mirror of the counters (finish)
+    uint32_t reads_t;         // the tape reads the input t: the assembly interpreter fills
+                              // slot 3 with tape_time (otherwise the tape writes it first)
 };
 // Word CNT_DONE of a host mirror written by finish_tree.
 constexpr uint32_t kCountersWritten = 0xC0DE0001u;

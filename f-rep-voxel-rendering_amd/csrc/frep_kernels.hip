@@ -864,9 +864,10 @@ __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t nod
     if constexpr (PF)
         d = run_point_tape_asm8_pf2<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0,
                                           n, pf_win, pf_len, pf_hdr, w1, n1, h2, nx_w, nx_n,
-                                          nx_c);
+                                          nx_c, a.reads_t);
     else
-        d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0);
+        d = run_point_tape_asm8<MAXS>(a.pool + toff, a.consts, a.tape_time, xs, ys, zs, w0, c0,
+                                      a.reads_t);
     uint64_t B[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) B[k] = __ballot(d[k] < 0.0f);

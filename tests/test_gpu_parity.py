@@ -338,6 +338,39 @@ def test_builds_without_init_stay_exact(ctx, oracle):
     ctx.tree_free(tree)
 
 
+def _pulsing_spheres():
+    """Two spheres whose radii grow with t: an assembly-interpreter tape
+    (CONST ADD SUB MUL SQRT MIN) that reads the input t."""
+    b = Builder()
+    r = b.add(b.const(0.3), b.mul(b.const(0.25), b.t()))
+    d0 = b.sub(b.sqrt(b.add(b.add(b.square(b.x()), b.square(b.y())), b.square(b.z()))), r)
+    xs = b.sub(b.x(), b.const(0.45))
+    d1 = b.sub(b.sqrt(b.add(b.add(b.square(xs), b.square(b.y())), b.square(b.z()))),
+               b.mul(r, b.const(0.7)))
+    return compile_tape(b, b.min(d0, d1))
+
+
+def test_time_dependent_tape_on_assembly_path(ctx, oracle):
+    """Slot 3 is filled with t for tapes that read it (and left alone for
+    those that write it first): a time-dependent tape on the assembly point
+    interpreter equals the oracle at several times, and the times differ."""
+    tape = _pulsing_spheres()
+    th = ctx.upload(tape)
+    tree = ctx.tree_create()
+    seen = []
+    for tt in (0.0, 1.0, 0.5, 1.0):
+        for dims in ([8, 8, 4], [16, 8, 8]):
+            grid = shapes.make_grid(dims, tape_time=tt)
+            ctx.voxelize(th, grid, tree)
+            got = ctx.export(tree)
+            assert np.array_equal(got, oracle.Tree(tape, grid).canonical()), (tt, dims)
+            if dims == [16, 8, 8]:
+                seen.append(got)
+    assert not np.array_equal(seen[0], seen[1])
+    assert np.array_equal(seen[1], seen[3])
+    ctx.tree_free(tree)
+
+
 def test_edge_shapes(ctx, oracle):
     b = Builder()
     empty = compile_tape(b, b.add(b.const(1.0), b.mul(b.x(), b.const(0.0))))

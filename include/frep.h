@@ -139,7 +139,10 @@ frep_status frep_tape_free(frep_ctx* ctx, frep_tape_t tape);
 /* ---- voxelize ---- */
 frep_status frep_tree_create(frep_ctx* ctx, frep_tree_t* out);
 /* Builds the sparse tree of `tape` over `grid` into `tree` (blocking, like
- * Voxelizer::Voxelize).  Buffers of the tree are reused and grown as needed. */
+ * Voxelizer::Voxelize).  Buffers of the tree are reused and grown as needed;
+ * a repeated build of the same tape into the same buffers starts from the
+ * state the previous build left (no init launch).  FREP_DEVICE if the
+ * device's copy of the build counters did not reach the host. */
 frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t tape, const frep_grid* grid,
                           frep_tree_t tree);
 /* Batched form of frep_voxelize (the reference voxelizes one shape per

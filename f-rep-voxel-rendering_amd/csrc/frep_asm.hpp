@@ -141,6 +141,62 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  v_cndmask_b32_e32 v[RA + 3], 0, v[RA + 3], vcc\n"                                      \
     "  v_lshlrev_b32_e32 v[RA + 3], 2, v[RA + 3]\n"                                           \
     "  global_load_dword v[" C "], v[RA + 3], %[cst]\n"
+// CONST words fused with the word after them: a lane whose word is a CONST
+// (slot k) followed, in this window, by an ADD/SUB/MUL that reads slot k
+// once and works in place -- on its other operand (FV_A: a, FV_B: b, slot k
+// still written) or on slot k itself (FC_A: k is operand a, FC_B: k is b) --
+// takes the fused handler of region 13 + (op - 3), slot FV_A 0, FV_B 1,
+// FC_A 2, FC_B 3, which does both words.  NS (the next word's fields) is set
+// here.  Temporaries RB+0..7, masks p0..p3.
+#define FREP_ASM_FUSE                                                                         \
+    "  s_nop 1\n"                                                                             \
+    "  v_mov_b32_dpp v[NS], v[RA + 5] wave_shl:1 bound_ctrl:0\n"                              \
+    "  v_and_b32_e32 v[RB + 0], 0xff, v[WB + 0]\n"                /* op */                    \
+    "  s_nop 1\n"                                                                             \
+    "  v_mov_b32_dpp v[RB + 1], v[RB + 0] wave_shl:1 bound_ctrl:0\n" /* next op */            \
+    "  v_bfe_u32 v[RB + 2], v[NS], 0, 8\n"                       /* next 8a */               \
+    "  v_bfe_u32 v[RB + 3], v[NS], 8, 8\n"                       /* next 8b */               \
+    "  v_bfe_u32 v[RB + 4], v[NS], 16, 8\n"                      /* next 8o */               \
+    "  v_bfe_u32 v[RB + 5], v[RA + 5], 16, 8\n"                  /* 8k: this word's 8o */    \
+    "  v_subrev_u32_e32 v[RB + 6], 3, v[RB + 1]\n"               /* next op - 3 */           \
+    "  v_add_u32_e32 v[RB + 7], %[s], v[RA + 1]\n"               /* word index */            \
+    /* p0: a CONST followed by ADD/SUB/MUL within this window and the tape */                 \
+    "  v_cmp_eq_u32_e64 %[p0], 0, v[RB + 0]\n"                                                \
+    "  v_cmp_gt_u32_e64 %[p1], 3, v[RB + 6]\n"                                                \
+    "  s_and_b64 %[p0], %[p0], %[p1]\n"                                                       \
+    "  v_cmp_gt_u32_e64 %[p1], %[nm1], v[RB + 7]\n"                                           \
+    "  s_and_b64 %[p0], %[p0], %[p1]\n"                                                       \
+    "  v_cmp_gt_u32_e64 %[p1], 63, v[RA + 1]\n"                                               \
+    "  s_and_b64 %[p0], %[p0], %[p1]\n"                                                       \
+    /* p3: ... whose operand a alone is slot k, p2: whose operand b alone is */               \
+    "  v_cmp_eq_u32_e64 %[p1], v[RB + 2], v[RB + 5]\n"                                        \
+    "  v_cmp_eq_u32_e64 %[p2], v[RB + 3], v[RB + 5]\n"                                        \
+    "  s_andn2_b64 %[p3], %[p1], %[p2]\n"                                                     \
+    "  s_andn2_b64 %[p2], %[p2], %[p1]\n"                                                     \
+    "  s_and_b64 %[p3], %[p3], %[p0]\n"                                                       \
+    "  s_and_b64 %[p2], %[p2], %[p0]\n"                                                       \
+    /* slot in v[RB + 7], fused lanes in p0: FC_A 2 and FC_B 3 (next o == k), */              \
+    /* FV_B 1 (a is k, next o == b), FV_A 0 (b is k, next o == a) */                          \
+    "  v_cmp_eq_u32_e64 %[p1], v[RB + 4], v[RB + 5]\n"                                        \
+    "  s_and_b64 vcc, %[p3], %[p1]\n"                                                         \
+    "  v_cndmask_b32_e64 v[RB + 7], 0, 2, vcc\n"                                              \
+    "  s_mov_b64 %[p0], vcc\n"                                                                \
+    "  s_and_b64 vcc, %[p2], %[p1]\n"                                                         \
+    "  v_cndmask_b32_e64 v[RB + 7], v[RB + 7], 3, vcc\n"                                      \
+    "  s_or_b64 %[p0], %[p0], vcc\n"                                                          \
+    "  v_cmp_eq_u32_e64 %[p1], v[RB + 4], v[RB + 3]\n"                                        \
+    "  s_and_b64 vcc, %[p3], %[p1]\n"                                                         \
+    "  v_cndmask_b32_e64 v[RB + 7], v[RB + 7], 1, vcc\n"                                      \
+    "  s_or_b64 %[p0], %[p0], vcc\n"                                                          \
+    "  v_cmp_eq_u32_e64 %[p1], v[RB + 4], v[RB + 2]\n"                                        \
+    "  s_and_b64 vcc, %[p2], %[p1]\n"                                                         \
+    "  s_or_b64 %[p0], %[p0], vcc\n"                                                          \
+    /* fused lanes: region 13 + (next op - 3), slot v[RB + 7] */                              \
+    "  v_add_u32_e32 v[RB + 6], 13, v[RB + 6]\n"                                              \
+    "  v_lshlrev_b32_e32 v[RB + 6], 10, v[RB + 6]\n"                                          \
+    "  v_lshl_or_b32 v[RB + 6], v[RB + 7], 8, v[RB + 6]\n"                                    \
+    "  v_cndmask_b32_e64 v[RA + 7], v[RA + 7], v[RB + 6], %[p0]\n"
+
 // Decode window W0 (words %[s] + lane), all lanes at once: RA+5 the fields,
 // RA+7 the handler offset -- (op & 15) * 1024 + variant * 256, variant 1
 // when the output slot is operand A's, else 2 when it is B's, else 0, kept
@@ -171,13 +227,13 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  v_and_b32_e32 v[RA + 6], v[RA + 6], v[RA + 2]\n"                                       \
     "  v_lshlrev_b32_e32 v[RA + 7], 10, v[RA + 7]\n"                                          \
     "  v_lshl_or_b32 v[RA + 7], v[RA + 6], 8, v[RA + 7]\n"                                    \
+    FREP_ASM_FUSE                                                                             \
     "  v_add_u32_e32 v[RA + 2], %[s], v[RA + 1]\n"                                            \
     "  v_cmp_gt_u32_e32 vcc, %[n], v[RA + 2]\n"                                               \
     "  v_mov_b32_e32 v[RA + 3], .Lfa_end%= - .Lfa_base%=\n"                                   \
     "  s_nop 1\n"                                                                             \
     "  v_cndmask_b32_e32 v[RA + 7], v[RA + 3], v[RA + 7], vcc\n"                              \
     "  s_nop 1\n"                                                                             \
-    "  v_mov_b32_dpp v[NS], v[RA + 5] wave_shl:1 bound_ctrl:0\n"                              \
     "  v_mov_b32_dpp v[NT], v[RA + 7] wave_shl:1 bound_ctrl:0\n"                              \
     "  s_add_u32 %[tmp], %[s], 64\n"                                                          \
     "  s_cmp_lt_u32 %[tmp], %[n]\n"                                                           \
@@ -255,6 +311,63 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "    v_cndmask_b32_e64 v[RB + \\q + 3], v[RA + \\q + 3], v[RB + \\q + 3], %[p3]\n"        \
     "  .endr\n"                                                                               \
     FREP_ASM_WRITE("RB")                                                                      \
+    FREP_ASM_GO
+
+// Fused CONST + binary handlers (FREP_ASM_FUSE): c in vcc_lo, the binary
+// word's fields in %[nx]; the binary word is skipped (%[j] advanced past it
+// before the next word's offset is read).  FV_A / FV_B write {c, c} to slot
+// k and apply the op in place on the other operand with c as a scalar
+// operand; FC_A / FC_B copy the other operand and write the result to slot
+// k.  OPC_* take (c, operand) in that order.
+#define FREP_ASM_FUSED_HEAD                                                                   \
+    "  v_readlane_b32 vcc_lo, v[WB + 3], %[j]\n"                                              \
+    "  v_readlane_b32 %[nx], v[NS], %[j]\n"                                                   \
+    "  s_add_u32 %[j], %[j], 1\n" FREP_ASM_NEXT
+#define FREP_ASM_CONST_WRITE                                                                  \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_mov_b32 vcc_hi, vcc_lo\n"                                                            \
+    "  s_nop 1\n"                                                                             \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
+    "  .irp k, 0,2,4,6\n"                                                                     \
+    "    v_mov_b64 v[FS + \\k:FS + \\k + 1], vcc\n"                                             \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM_SCALAR_OP(OPC, S, R)                                                         \
+    "  s_set_gpr_idx_on " S ", gpr_idx(SRC1,DST)\n"                                           \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    " OPC " v[FS + \\k], vcc_lo, v[" R " + \\k]\n"                                        \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM_FUSED(R, OPC_VA, OPC_VB, OPC_CA, OPC_CB)                                     \
+    FREP_ASM_SLOT(R, 0)                                                                       \
+    FREP_ASM_FUSED_HEAD FREP_ASM_CONST_WRITE                                                  \
+    FREP_ASM_SCALAR_OP(OPC_VA, "%[nx]", "FS")                                                 \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(R, 1)                                                                       \
+    FREP_ASM_FUSED_HEAD FREP_ASM_CONST_WRITE                                                  \
+    "  s_lshr_b32 %[tmp], %[nx], 8\n"                                                         \
+    FREP_ASM_SCALAR_OP(OPC_VB, "%[tmp]", "FS")                                                \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(R, 2)                                                                       \
+    FREP_ASM_FUSED_HEAD                                                                       \
+    "  s_lshr_b32 %[tmp], %[nx], 8\n"                                                         \
+    FREP_ASM_READ("RB", "%[tmp]")                                                             \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    " OPC_CA " v[FS + \\k], vcc_lo, v[RB + \\k]\n"                                         \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(R, 3)                                                                       \
+    FREP_ASM_FUSED_HEAD                                                                       \
+    FREP_ASM_READ("RB", "%[nx]")                                                              \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    " OPC_CB " v[FS + \\k], vcc_lo, v[RB + \\k]\n"                                         \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"                                                                   \
     FREP_ASM_GO
 
 #define FREP_ASM_POINT_BODY                                                                   \
@@ -352,9 +465,11 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     FREP_ASM_WRITE("RA")                                                                      \
     FREP_ASM_GO                                                                               \
     FREP_ASM_SLOT(12, 1) FREP_ASM_NOP                                                         \
-    FREP_ASM_SLOT(13, 0) FREP_ASM_NOP                                                         \
-    FREP_ASM_SLOT(14, 0) FREP_ASM_NOP                                                         \
-    FREP_ASM_SLOT(15, 0) FREP_ASM_NOP                                                         \
+    /* regions 13-15 (op codes no tape uses): a CONST fused with the ADD, SUB or              \
+       MUL after it; SUB is a - b as v_sub (c - x) or v_subrev (x - c) */                     \
+    FREP_ASM_FUSED(13, "v_add_f32_e32", "v_add_f32_e32", "v_add_f32_e32", "v_add_f32_e32")     \
+    FREP_ASM_FUSED(14, "v_subrev_f32_e32", "v_sub_f32_e32", "v_sub_f32_e32", "v_subrev_f32_e32") \
+    FREP_ASM_FUSED(15, "v_mul_f32_e32", "v_mul_f32_e32", "v_mul_f32_e32", "v_mul_f32_e32")     \
     FREP_ASM_SLOT(16, 0)                                                                      \
     /* next window: the loads issued one window ago have had a window to                      \
        arrive */                                                                              \
@@ -404,10 +519,10 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 
 // Scalar registers of the point interpreter's assembly block.
 #define FREP_ASM_POINT_DECLS                                                                  \
-    uint32_t sel, ntg, j, tmp, blo, bhi, s, sidx, nm1, cls, tiny, sgn;                        \
+    uint32_t sel, ntg, nx, j, tmp, blo, bhi, s, sidx, nm1, cls, tiny, sgn;                    \
     uint64_t p0, p1, p2, p3
 #define FREP_ASM_POINT_SCALARS                                                                \
-    [sel] "=&s"(sel), [ntg] "=&s"(ntg), [j] "=&s"(j), [tmp] "=&s"(tmp),                       \
+    [sel] "=&s"(sel), [ntg] "=&s"(ntg), [nx] "=&s"(nx), [j] "=&s"(j), [tmp] "=&s"(tmp),       \
         [blo] "=&s"(blo), [bhi] "=&s"(bhi), [s] "=&s"(s), [sidx] "=&s"(sidx),                 \
         [nm1] "=&s"(nm1), [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn),             \
         [p0] "=&s"(p0), [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)

@@ -80,6 +80,50 @@ def test_eval_points_bit_exact(ctx, oracle, name):
         assert same_bits(got, ref), name
 
 
+def _word(op, o, a, b):
+    return op | o << 8 | a << 16 | b << 24
+
+
+def _fusion_tapes():
+    """Raw tapes (8 slots, constant pool [0.75]) for every CONST + ADD/SUB/MUL
+    pattern the point interpreter fuses, their non-fusable neighbours, and a
+    CONST on either side of a 64-word window edge.  Result in slot 0."""
+    from frep.tape import tape_from_words
+    const = _word(0, 4, 0, 0)                      # slot 4 = 0.75
+    cases = {}
+    for op, name in ((3, "add"), (4, "sub"), (5, "mul")):
+        cases[name + "_in_place_on_a"] = [_word(12, 5, 0, 0), const, _word(op, 5, 5, 4),
+                                          _word(12, 0, 5, 5)]
+        cases[name + "_in_place_on_b"] = [_word(12, 5, 1, 1), const, _word(op, 5, 4, 5),
+                                          _word(12, 0, 5, 5)]
+        cases[name + "_on_const_a"] = [const, _word(op, 4, 4, 1), _word(12, 0, 4, 4)]
+        cases[name + "_on_const_b"] = [const, _word(op, 4, 2, 4), _word(12, 0, 4, 4)]
+        cases[name + "_both_const"] = [const, _word(op, 4, 4, 4), _word(op, 0, 0, 4)]
+        cases[name + "_new_slot"] = [const, _word(op, 5, 4, 1), _word(op, 0, 5, 4)]
+    fill = [_word(12, 6, 0, 0)] * 62
+    cases["const_at_word_62"] = fill + [const, _word(5, 4, 4, 1), _word(3, 0, 4, 0)]
+    cases["const_at_word_63"] = fill + [_word(12, 6, 1, 1), const, _word(5, 4, 4, 1),
+                                        _word(3, 0, 4, 0)]
+    cases["const_last"] = [_word(12, 5, 0, 0), _word(0, 0, 0, 0)]
+    return {k: tape_from_words(np.array([len(w)] + w, dtype=np.uint32),
+                               np.array([0.75], dtype=np.float32), 8)
+            for k, w in cases.items()}
+
+
+def test_fused_const_handlers_bit_exact(ctx, oracle):
+    """Point evaluation (assembly interpreter, 8-slot file) of tapes built to
+    reach each fused CONST handler and the cases next to them, against the
+    oracle, bit for bit."""
+    rng = np.random.default_rng(5)
+    p = rng.uniform(-1.5, 1.5, (4096, 3)).astype(np.float32)
+    for name, tape in _fusion_tapes().items():
+        th = ctx.upload(tape)
+        got = ctx.eval_points(th, p, 0.25)
+        ref = np.array([oracle.eval_point(tape, *q, 0.25) for q in p], dtype=np.float32)
+        assert same_bits(got, ref), name
+        ctx.tape_free(th)
+
+
 @pytest.mark.parametrize("name", sorted(TAPES))
 def test_eval_intervals_bit_exact(ctx, oracle, name):
     tape = TAPES[name]()

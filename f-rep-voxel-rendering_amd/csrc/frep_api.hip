@@ -399,8 +399,11 @@ frep_status frep_init(int device, frep_ctx** out) {
     c->no_asm = na && na[0] == '1';
     const char* nt = std::getenv("FREP_NO_AUTOTUNE");
     c->autotune = !(nt && nt[0] == '1');
-    const char* ng = std::getenv("FREP_NO_GRAPH");
-    c->graphs = !(ng && ng[0] == '1');
+    // plain builds launch directly unless FREP_GRAPH=1: with the init and
+    // counter-copy launches gone, three direct launches submit faster than a
+    // graph replay of them
+    const char* gr = std::getenv("FREP_GRAPH");
+    c->graphs = gr && gr[0] == '1';
     const char* ro = std::getenv("FREP_ROOT_OPS");
     c->root_ops = !(ro && ro[0] == '0');
     c->root_ops_required = ro && ro[0] == '2';

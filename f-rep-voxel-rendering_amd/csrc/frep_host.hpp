@@ -200,7 +200,7 @@ struct frep_ctx {
     std::vector<std::unique_ptr<frep::TreeObj>> trees;
     bool timing = false;
     bool no_asm = false;               // FREP_NO_ASM=1: C++ interpreter only
-    bool graphs = true;                // FREP_NO_GRAPH=1: plain voxelization launched directly
+    bool graphs = false;               // FREP_GRAPH=1: repeated plain builds replayed as a graph
     bool root_ops = true;              // FREP_ROOT_OPS=0: level 0 by the per-cell kernel
     bool root_ops_required = false;    // FREP_ROOT_OPS=2: a tape without a level-0 schedule is an error
     size_t lds_per_block = 65536;      // device limit (hipDeviceAttributeMaxSharedMemoryPerBlock)

@@ -194,18 +194,19 @@ def test_point_slot_files_vs_oracle(ctx, oracle, n_prims, seed, slots):
 
 
 def test_repeated_voxelize_replays_equal_direct(oracle):
-    """Repeating a voxelization with unchanged arguments replays it from a
-    graph (captured on the first repeat); a call with new arguments (tape
-    time) runs directly.  Every call's tree equals the oracle's, and the
-    replays equal the direct launches of a context with graphs off."""
+    """With FREP_GRAPH=1, repeating a voxelization with unchanged arguments
+    replays it from a graph (captured on the first repeat); a call with new
+    arguments (tape time) runs directly.  Every call's tree equals the
+    oracle's, and the replays equal the direct launches of a default
+    context."""
     import os
     tape = TAPES["trig"]()
-    os.environ["FREP_NO_GRAPH"] = "1"
+    direct = lib.Context(0)
+    os.environ["FREP_GRAPH"] = "1"
     try:
-        direct = lib.Context(0)
+        fast = lib.Context(0)
     finally:
-        del os.environ["FREP_NO_GRAPH"]
-    fast = lib.Context(0)
+        del os.environ["FREP_GRAPH"]
     try:
         th = fast.upload(tape)
         tree = fast.tree_create()

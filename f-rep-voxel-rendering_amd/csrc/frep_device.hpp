@@ -145,6 +145,15 @@ __device__ __forceinline__ float window_consts(uint32_t w, const float* consts) 
 // register move.
 #define FREP_FENCE() __builtin_amdgcn_sched_barrier(0)
 
+// Slot index kept inside a register file of MAXS slots (tapes are validated
+// on upload, so this only guards the register-file access): a mask for
+// power-of-two files, a clamp otherwise.
+template <int MAXS>
+__device__ __forceinline__ uint32_t slot_in(uint32_t x) {
+    if constexpr ((MAXS & (MAXS - 1)) == 0) return x & (MAXS - 1);
+    else return x < (uint32_t)MAXS ? x : (uint32_t)(MAXS - 1);
+}
+
 // Evaluates a tape at K <= 8 points per lane.
 //
 // Register file: each point has its own slot array.  The compiler keeps every
@@ -186,8 +195,8 @@ __device__ __forceinline__ void run_point_tape(ctape_t tape, cconst_t consts, fl
             wq[0] = wq[1]; wq[1] = wq[2]; wq[2] = wq[3];
             // slot fields masked to the register file; CONST uses the raw
             // 16-bit pool index
-            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-            const uint32_t a = ((w >> 16) & 0xFFu) & (MAXS - 1), b = (w >> 24) & (MAXS - 1);
+            const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
+            const uint32_t a = slot_in<MAXS>((w >> 16) & 0xFFu), b = slot_in<MAXS>(w >> 24);
 #define FREP_RA(k) float A##k = 0.0f; if constexpr (k < K) { A##k = s##k[a]; }
 #define FREP_RB(k) float B##k = 0.0f; if constexpr (k < K) { B##k = s##k[b]; }
 #define FREP_B(k, EXPR) if constexpr (k < K) { r##k = EXPR; }

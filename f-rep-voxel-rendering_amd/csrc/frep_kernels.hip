@@ -169,15 +169,6 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 // second pass.  If nothing was removed the child reuses the parent's tape
 // (shorten.comp:188-191).
 // ---------------------------------------------------------------------------
-// Slot index kept inside a register file of MAXS slots (tapes are validated
-// on upload, so this only guards the register-file access): a mask for
-// power-of-two files, a clamp otherwise.
-template <int MAXS>
-__device__ __forceinline__ uint32_t slot_in(uint32_t x) {
-    if constexpr ((MAXS & (MAXS - 1)) == 0) return x & (MAXS - 1);
-    else return x < (uint32_t)MAXS ? x : (uint32_t)(MAXS - 1);
-}
-
 template <int MAXS, bool ASM, class Reserve>
 __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                uint64_t child_base, uint32_t lane, uint4* scr,
@@ -1742,6 +1733,13 @@ hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
     if (asm_ok && k == 8 && maxs == 16) {
         k_level_point<16, 8, true><<<point_grid<16, 8, true>(cu_count, max_grid), kLevelBlock, 0,
                                      st>>>(a);
+        return hipGetLastError();
+    }
+    // C++ interpreter, 8 points per lane: a 10-slot file for tapes of at
+    // most 10 slots (4 wavefronts per SIMD where the 16-slot file allows 3)
+    if (k == 8 && maxs == 16 && slots <= 10) {
+        k_level_point<10, 8, false><<<point_grid<10, 8, false>(cu_count, max_grid), kLevelBlock,
+                                      0, st>>>(a);
         return hipGetLastError();
     }
 #define FREP_POINT_LAUNCH(KK)                                                                 \

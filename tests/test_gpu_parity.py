@@ -80,6 +80,20 @@ def test_eval_points_bit_exact(ctx, oracle, name):
         assert same_bits(got, ref), name
 
 
+@pytest.mark.parametrize("dims", [[8, 8], [16, 8]])
+def test_cpp_point_slot_file_10_vs_oracle(ctx, oracle, dims):
+    """The gyroid blend (sin/cos, 9 slots) at an 8-wide last level runs the
+    C++ point interpreter, 8 voxels per lane, on its 10-slot file: its tree
+    equals the oracle's."""
+    tape = shapes.gyroid_blend()
+    assert tape.slot_count == 9
+    grid = shapes.make_grid(dims)
+    th = ctx.upload(tape)
+    tree = ctx.voxelize(th, grid)
+    assert np.array_equal(ctx.export(tree), oracle.Tree(tape, grid).canonical())
+    ctx.tree_free(tree)
+
+
 def _word(op, o, a, b):
     return op | o << 8 | a << 16 | b << 24
 

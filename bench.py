@@ -233,8 +233,9 @@ def main():
     ap.add_argument("--no-served", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(the sharding design is described in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: the voxelizer runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(trees would be sharded by level-0 cell across GPUs, which is not built)",
+              file=sys.stderr)
         sys.exit(2)
 
     from frep import lib, shapes

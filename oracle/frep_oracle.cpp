@@ -19,8 +19,22 @@
  *   orc_voxelize         flattened occupancy == dense point sampling
  *                        (orc_dense_occupancy) on power-of-two grids; node
  *                        counts vs brute force
- *   orc_render           hit voxel == first occupied voxel found by brute
- *                        force ray marching on small trees
+ *   orc_render           DDA: every ray whose first occupied cell is
+ *                        unambiguous (an exact float64 voxel walk over the
+ *                        tree's leaf cells) hits exactly that cell, at the
+ *                        level whose leaf bit covers it -- axis-aligned rays
+ *                        through cell centres and perspective rays on 8^3..
+ *                        64^3 grids; shading: half spaces n.p - d (constant
+ *                        gradient) show the closed-form colour ambient +
+ *                        sum max(dot(-L, n), 0) colour, rounded to 8 bits,
+ *                        on every hit pixel, and k-of-spp averages; misses
+ *                        show the background.  tests/test_oracle_mutants.py
+ *                        plants light-sign, light-colour, ambient, rounding,
+ *                        cell-index, step-sign and descent mistakes in
+ *                        copies of this file and checks each one fails.
+ *                        The reference's "t < 0 -> white" branch
+ *                        (raytracer/main.comp:148-151) is kept but cannot be
+ *                        reached: a hit's t is at least EPS.
  *   orc_dense_occupancy  vs orc_eval_point (same definition, different loop)
  *
  * Canonical tree blob (u32 words), shared with the GPU library's export:

@@ -1269,7 +1269,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
                     HIP_TRY(launch_root_interval(ctx->stream, ra));
                 else
                     HIP_TRY(launch_level_interval(ctx->stream, la[L], t->maxs, t->slot_count,
-                                                  ctx->cu_count, ctx->level_grid, t->asm_ok));
+                                                  ctx->cu_count, ctx->level_grid));
                 if (events) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
             }
             return FREP_OK;

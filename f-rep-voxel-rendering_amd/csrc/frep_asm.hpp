@@ -729,324 +729,4 @@ __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, con
     return x;
 }
 
-// ---------------------------------------------------------------------------
-// Interval forward interpreter (interval.comp:121-160), the same op set as the
-// point interpreter.  One cell per lane: slot s keeps its lower bound in
-// v[16 + s] and its upper bound in v[16 + MS + s], so one s_set_gpr_idx
-// window (M0 = s) reaches both.  Registers above the slot file: RA+0..3
-// operand copies (a.lo a.hi b.lo b.hi), RR+0..1 the result, TT+0..7
-// temporaries, WB+0..4 tape windows (as in the point interpreter), VR+0..3
-// the choice row.
-//
-// Per instruction the rules of the C++ path, bit for bit:
-//   ADD  [a.lo + b.lo, a.hi + b.hi]      SUB  [a.lo - b.hi, a.hi - b.lo]
-//   MUL  four products, GLM min / max of the pairs then of the two results
-//   MIN / MAX  GLM min / max of the bounds, plus the choice ballots
-//        c1 = a.hi < b.lo, c2 = b.hi < a.lo:
-//        MIN  FIRST lanes c1 | !c2, SECOND lanes !c1
-//        MAX  FIRST lanes !c1,      SECOND lanes c1 | !c2
-//        written as one 16-byte row {FIRST lo, FIRST hi, SECOND lo, SECOND hi}
-//        at the wavefront's next LDS choice row
-//   NEG  [-a.hi, -a.lo] (sign flips)     SQRT  sqrt(GLM max(0, bound)) each
-//   CONST [c, c]                         COPY  a
-// ---------------------------------------------------------------------------
-#define FREP_IASM_READ(R, S)                                                                  \
-    "  s_set_gpr_idx_on " S ", gpr_idx(SRC0)\n"                                               \
-    "  v_mov_b32 v[" R "], v[FS]\n"                                                           \
-    "  v_mov_b32 v[" R " + 1], v[FS + MS]\n"                                                  \
-    "  s_set_gpr_idx_off\n"
-#define FREP_IASM_WRITE(R)                                                                    \
-    "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
-    "  v_mov_b32 v[FS], v[" R "]\n"                                                           \
-    "  v_mov_b32 v[FS + MS], v[" R " + 1]\n"                                                  \
-    "  s_set_gpr_idx_off\n"
-// Correctly rounded square root of v[X] into v[Y] (LLVM's fsqrt.f32 lowering,
-// as in the point interpreter); TT+0..4 temporaries.
-#define FREP_IASM_SQRT(X, Y)                                                                  \
-    "  v_mul_f32_e32 v[TT + 0], 0x4f800000, v[" X "]\n"                                       \
-    "  v_cmp_gt_f32_e32 vcc, %[tiny], v[" X "]\n"                                             \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e32 v[TT + 0], v[" X "], v[TT + 0], vcc\n"                               \
-    "  v_sqrt_f32_e32 v[TT + 1], v[TT + 0]\n"                                                 \
-    "  s_nop 0\n"                                                                             \
-    "  v_add_u32_e32 v[TT + 2], -1, v[TT + 1]\n"                                              \
-    "  v_fma_f32 v[TT + 4], -v[TT + 2], v[TT + 1], v[TT + 0]\n"                               \
-    "  v_add_u32_e32 v[TT + 3], 1, v[TT + 1]\n"                                               \
-    "  v_cmp_ge_f32_e64 %[p0], 0, v[TT + 4]\n"                                                \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e64 v[TT + 2], v[TT + 1], v[TT + 2], %[p0]\n"                            \
-    "  v_fma_f32 v[TT + 1], -v[TT + 3], v[TT + 1], v[TT + 0]\n"                               \
-    "  v_cmp_lt_f32_e64 %[p0], 0, v[TT + 1]\n"                                                \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e64 v[TT + 2], v[TT + 2], v[TT + 3], %[p0]\n"                            \
-    "  v_mul_f32_e32 v[TT + 3], 0x37800000, v[TT + 2]\n"                                      \
-    "  v_cndmask_b32_e32 v[TT + 2], v[TT + 2], v[TT + 3], vcc\n"                              \
-    "  v_cmp_class_f32_e64 vcc, v[TT + 0], %[cls]\n"                                          \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e32 v[" Y "], v[TT + 2], v[TT + 0], vcc\n"
-// Choice row of a MIN/MAX from the lane masks in vcc (FIRST, then SECOND):
-// FIRST_OP / SECOND_OP build each mask into vcc from %[p2] (c1), %[p3] (c2).
-#define FREP_IASM_ROW(FIRST_OP, SECOND_OP)                                                    \
-    "  " FIRST_OP "\n"                                                                        \
-    "  v_mov_b32_e32 v[VR + 0], vcc_lo\n"                                                     \
-    "  v_mov_b32_e32 v[VR + 1], vcc_hi\n"                                                     \
-    "  " SECOND_OP "\n"                                                                       \
-    "  v_mov_b32_e32 v[VR + 2], vcc_lo\n"                                                     \
-    "  v_mov_b32_e32 v[VR + 3], vcc_hi\n"                                                     \
-    "  ds_write_b128 %[la], v[VR:VR + 3]\n"                                                   \
-    "  v_add_u32_e32 %[la], 16, %[la]\n"                                                      \
-    "  s_add_u32 %[mm], %[mm], 1\n"
-
-#define FREP_IASM_BODY                                                                        \
-    ".set FS, 16\n"                                                                           \
-    ".set MS, %[maxs]\n"                                                                      \
-    ".set LG, %[lg]\n"                                                                        \
-    ".set RA, FS + 2*MS\n"                                                                    \
-    ".set RR, RA + 4\n"                                                                       \
-    ".set TT, RA + 6\n"                                                                       \
-    ".set WB, RA + 14\n"                                                                      \
-    ".set VR, RA + 20\n"                                                                      \
-    "s_mov_b32 %[cls], 0x260\n"                                                               \
-    "s_mov_b32 %[tiny], 0xf800000\n"                                                          \
-    "s_mov_b32 %[mm], 0\n"                                                                    \
-    "s_cmp_eq_u32 %[n], 0\n"                                                                  \
-    "s_cbranch_scc1 .Lia_end%=\n"                                                             \
-    "s_sub_u32 %[nm1], %[n], 1\n"                                                             \
-    "s_mov_b32 %[s], 0\n"                                                                     \
-    "s_mov_b32 %[sidx], 0\n"                                                                  \
-    FREP_ASM_LOADW("WB + 0")                                                                  \
-    "s_mov_b32 %[sidx], 64\n"                                                                 \
-    FREP_ASM_LOADW("WB + 1")                                                                  \
-    "s_mov_b32 %[sidx], 128\n"                                                                \
-    FREP_ASM_LOADW("WB + 2")                                                                  \
-    "s_waitcnt vmcnt(0)\n"                                                                    \
-    FREP_ASM_GATHER("WB + 3", "WB + 0")                                                       \
-    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
-    "s_waitcnt vmcnt(0)\n"                                                                    \
-    ".Lia_win%=:\n"                                                                           \
-    "  s_sub_u32 %[m], %[n], %[s]\n"                                                          \
-    "  s_min_u32 %[m], %[m], 64\n"                                                            \
-    "  s_mov_b32 %[j], 0\n"                                                                   \
-    ".Lia_loop%=:\n"                                                                          \
-    "  v_readlane_b32 %[w], v[WB + 0], %[j]\n"                                                \
-    "  s_bfe_u32 %[a], %[w], (LG << 16) | 16\n"                                               \
-    "  s_bfe_u32 %[b], %[w], (LG << 16) | 24\n"                                               \
-    "  s_bfe_u32 %[o], %[w], (LG << 16) | 8\n"                                                \
-    "  s_bitcmp1_b32 %[w], 3\n"                                                               \
-    "  s_cbranch_scc1 .Lia_hi%=\n"                                                            \
-    "  s_bitcmp1_b32 %[w], 2\n"                                                               \
-    "  s_cbranch_scc1 .Lia_mid%=\n"                                                           \
-    "  s_bitcmp1_b32 %[w], 1\n"                                                               \
-    "  s_cbranch_scc1 .Lia_add%=\n"                                                           \
-    /* op 0: CONST */                                                                         \
-    "  v_readlane_b32 %[c], v[WB + 3], %[j]\n"                                                \
-    "  s_nop 1\n"                                                                             \
-    "  s_set_gpr_idx_on %[o], gpr_idx(DST)\n"                                                 \
-    "  v_mov_b32 v[FS], %[c]\n"                                                               \
-    "  v_mov_b32 v[FS + MS], %[c]\n"                                                          \
-    "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_branch .Lia_next%=\n"                                                                \
-    /* op 3: ADD, in place on whichever operand slot is the output slot */                    \
-    ".Lia_add%=:\n"                                                                           \
-    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
-    "  s_cbranch_scc1 .Lia_add_oa%=\n"                                                        \
-    "  s_cmp_eq_u32 %[o], %[b]\n"                                                             \
-    "  s_cbranch_scc1 .Lia_add_ob%=\n"                                                        \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0)\n"                                                \
-    "  v_add_f32_e32 v[RR], v[FS], v[RA]\n"                                                   \
-    "  v_add_f32_e32 v[RR + 1], v[FS + MS], v[RA + 1]\n"                                      \
-    "  s_set_gpr_idx_off\n"                                                                   \
-    FREP_IASM_WRITE("RR")                                                                     \
-    "  s_branch .Lia_next%=\n"                                                                \
-    ".Lia_add_oa%=:\n"                                                                        \
-    FREP_IASM_READ("RA + 2", "%[b]")                                                          \
-    "  s_set_gpr_idx_on %[a], gpr_idx(SRC0,DST)\n"                                            \
-    "  v_add_f32_e32 v[FS], v[FS], v[RA + 2]\n"                                               \
-    "  v_add_f32_e32 v[FS + MS], v[FS + MS], v[RA + 3]\n"                                     \
-    "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_branch .Lia_next%=\n"                                                                \
-    ".Lia_add_ob%=:\n"                                                                        \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0,DST)\n"                                            \
-    "  v_add_f32_e32 v[FS], v[FS], v[RA]\n"                                                   \
-    "  v_add_f32_e32 v[FS + MS], v[FS + MS], v[RA + 1]\n"                                     \
-    "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_branch .Lia_next%=\n"                                                                \
-    /* ops 4 SUB, 5 MUL, 7 MIN */                                                             \
-    ".Lia_mid%=:\n"                                                                           \
-    "  s_bitcmp1_b32 %[w], 1\n"                                                               \
-    "  s_cbranch_scc1 .Lia_min%=\n"                                                           \
-    "  s_bitcmp1_b32 %[w], 0\n"                                                               \
-    "  s_cbranch_scc1 .Lia_mul%=\n"                                                           \
-    /* SUB: [a.lo - b.hi, a.hi - b.lo]; in place when the output is slot a */                 \
-    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
-    "  s_cbranch_scc1 .Lia_sub_oa%=\n"                                                        \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    "  s_set_gpr_idx_on %[b], gpr_idx(SRC0)\n"                                                \
-    "  v_subrev_f32_e32 v[RR], v[FS + MS], v[RA]\n"                                           \
-    "  v_subrev_f32_e32 v[RR + 1], v[FS], v[RA + 1]\n"                                        \
-    "  s_set_gpr_idx_off\n"                                                                   \
-    FREP_IASM_WRITE("RR")                                                                     \
-    "  s_branch .Lia_next%=\n"                                                                \
-    ".Lia_sub_oa%=:\n"                                                                        \
-    FREP_IASM_READ("RA + 2", "%[b]")                                                          \
-    "  s_set_gpr_idx_on %[a], gpr_idx(SRC0,DST)\n"                                            \
-    "  v_sub_f32_e32 v[FS], v[FS], v[RA + 3]\n"                                               \
-    "  v_sub_f32_e32 v[FS + MS], v[FS + MS], v[RA + 2]\n"                                     \
-    "  s_set_gpr_idx_off\n"                                                                   \
-    "  s_branch .Lia_next%=\n"                                                                \
-    /* MUL: p = {lo*lo, lo*hi, hi*lo, hi*hi} in TT+0..3 */                                    \
-    ".Lia_mul%=:\n"                                                                           \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    FREP_IASM_READ("RA + 2", "%[b]")                                                          \
-    "  v_mul_f32_e32 v[TT + 0], v[RA + 0], v[RA + 2]\n"                                       \
-    "  v_mul_f32_e32 v[TT + 1], v[RA + 0], v[RA + 3]\n"                                       \
-    "  v_mul_f32_e32 v[TT + 2], v[RA + 1], v[RA + 2]\n"                                       \
-    "  v_mul_f32_e32 v[TT + 3], v[RA + 1], v[RA + 3]\n"                                       \
-    "  v_cmp_lt_f32_e64 %[p0], v[TT + 1], v[TT + 0]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p1], v[TT + 3], v[TT + 2]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p2], v[TT + 0], v[TT + 1]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p3], v[TT + 2], v[TT + 3]\n"                                        \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e64 v[TT + 4], v[TT + 0], v[TT + 1], %[p0]\n" /* min(p0, p1) */          \
-    "  v_cndmask_b32_e64 v[TT + 5], v[TT + 2], v[TT + 3], %[p1]\n" /* min(p2, p3) */          \
-    "  v_cndmask_b32_e64 v[TT + 6], v[TT + 0], v[TT + 1], %[p2]\n" /* max(p0, p1) */          \
-    "  v_cndmask_b32_e64 v[TT + 7], v[TT + 2], v[TT + 3], %[p3]\n" /* max(p2, p3) */          \
-    "  v_cmp_lt_f32_e64 %[p0], v[TT + 5], v[TT + 4]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p1], v[TT + 6], v[TT + 7]\n"                                        \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e64 v[RR], v[TT + 4], v[TT + 5], %[p0]\n"                                \
-    "  v_cndmask_b32_e64 v[RR + 1], v[TT + 6], v[TT + 7], %[p1]\n"                            \
-    FREP_IASM_WRITE("RR")                                                                     \
-    "  s_branch .Lia_next%=\n"                                                                \
-    /* MIN: lo = b.lo < a.lo ? b.lo : a.lo, hi likewise */                                    \
-    ".Lia_min%=:\n"                                                                           \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    FREP_IASM_READ("RA + 2", "%[b]")                                                          \
-    "  v_cmp_lt_f32_e64 %[p0], v[RA + 2], v[RA + 0]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p1], v[RA + 3], v[RA + 1]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p2], v[RA + 1], v[RA + 2]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p3], v[RA + 3], v[RA + 0]\n"                                        \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e64 v[RR], v[RA + 0], v[RA + 2], %[p0]\n"                                \
-    "  v_cndmask_b32_e64 v[RR + 1], v[RA + 1], v[RA + 3], %[p1]\n"                            \
-    FREP_IASM_ROW("s_orn2_b64 vcc, %[p2], %[p3]", "s_not_b64 vcc, %[p2]")                     \
-    FREP_IASM_WRITE("RR")                                                                     \
-    "  s_branch .Lia_next%=\n"                                                                \
-    /* ops 8 MAX, 10 NEG, 11 SQRT, 12 COPY */                                                 \
-    ".Lia_hi%=:\n"                                                                            \
-    "  s_bitcmp1_b32 %[w], 2\n"                                                               \
-    "  s_cbranch_scc1 .Lia_copy%=\n"                                                          \
-    "  s_bitcmp1_b32 %[w], 1\n"                                                               \
-    "  s_cbranch_scc1 .Lia_negsqrt%=\n"                                                       \
-    /* MAX: lo = a.lo < b.lo ? b.lo : a.lo, hi likewise */                                    \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    FREP_IASM_READ("RA + 2", "%[b]")                                                          \
-    "  v_cmp_lt_f32_e64 %[p0], v[RA + 0], v[RA + 2]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p1], v[RA + 1], v[RA + 3]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p2], v[RA + 1], v[RA + 2]\n"                                        \
-    "  v_cmp_lt_f32_e64 %[p3], v[RA + 3], v[RA + 0]\n"                                        \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e64 v[RR], v[RA + 0], v[RA + 2], %[p0]\n"                                \
-    "  v_cndmask_b32_e64 v[RR + 1], v[RA + 1], v[RA + 3], %[p1]\n"                            \
-    FREP_IASM_ROW("s_not_b64 vcc, %[p2]", "s_orn2_b64 vcc, %[p2], %[p3]")                     \
-    FREP_IASM_WRITE("RR")                                                                     \
-    "  s_branch .Lia_next%=\n"                                                                \
-    ".Lia_negsqrt%=:\n"                                                                       \
-    "  s_bitcmp1_b32 %[w], 0\n"                                                               \
-    "  s_cbranch_scc1 .Lia_sqrt%=\n"                                                          \
-    /* NEG: [-a.hi, -a.lo] */                                                                 \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    "  v_xor_b32_e32 v[RR], 0x80000000, v[RA + 1]\n"                                          \
-    "  v_xor_b32_e32 v[RR + 1], 0x80000000, v[RA + 0]\n"                                      \
-    FREP_IASM_WRITE("RR")                                                                     \
-    "  s_branch .Lia_next%=\n"                                                                \
-    /* SQRT: each bound clamped below by GLM max(0, x) = 0 < x ? x : 0 */                     \
-    ".Lia_sqrt%=:\n"                                                                          \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    "  v_cmp_lt_f32_e64 %[p0], 0, v[RA + 0]\n"                                                \
-    "  v_cmp_lt_f32_e64 %[p1], 0, v[RA + 1]\n"                                                \
-    "  s_nop 1\n"                                                                             \
-    "  v_cndmask_b32_e64 v[RA + 0], 0, v[RA + 0], %[p0]\n"                                    \
-    "  v_cndmask_b32_e64 v[RA + 1], 0, v[RA + 1], %[p1]\n"                                    \
-    FREP_IASM_SQRT("RA + 0", "RR + 0")                                                        \
-    FREP_IASM_SQRT("RA + 1", "RR + 1")                                                        \
-    FREP_IASM_WRITE("RR")                                                                     \
-    "  s_branch .Lia_next%=\n"                                                                \
-    /* COPY: nothing to do when the output slot is the operand slot */                        \
-    ".Lia_copy%=:\n"                                                                          \
-    "  s_cmp_eq_u32 %[o], %[a]\n"                                                             \
-    "  s_cbranch_scc1 .Lia_next%=\n"                                                          \
-    FREP_IASM_READ("RA", "%[a]")                                                              \
-    FREP_IASM_WRITE("RA")                                                                     \
-    ".Lia_next%=:\n"                                                                          \
-    "  s_add_u32 %[j], %[j], 1\n"                                                             \
-    "  s_cmp_lt_u32 %[j], %[m]\n"                                                             \
-    "  s_cbranch_scc1 .Lia_loop%=\n"                                                          \
-    "  s_add_u32 %[s], %[s], 64\n"                                                            \
-    "  s_cmp_lt_u32 %[s], %[n]\n"                                                             \
-    "  s_cbranch_scc0 .Lia_done%=\n"                                                          \
-    "  s_waitcnt vmcnt(0)\n"                                                                  \
-    "  v_mov_b32 v[WB + 0], v[WB + 1]\n"                                                      \
-    "  v_mov_b32 v[WB + 3], v[WB + 4]\n"                                                      \
-    "  v_mov_b32 v[WB + 1], v[WB + 2]\n"                                                      \
-    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
-    "  s_add_u32 %[sidx], %[s], 128\n"                                                        \
-    FREP_ASM_LOADW("WB + 2")                                                                  \
-    "  s_branch .Lia_win%=\n"                                                                 \
-    ".Lia_done%=:\n"                                                                          \
-    /* no load may still be writing these registers, and the choice rows are  \
-       complete, when the block ends */                                                       \
-    "s_waitcnt vmcnt(0) lgkmcnt(0)\n"                                                         \
-    ".Lia_end%=:\n"
-
-#define FREP_IASM_CLOBBER_16                                                                  \
-    "v20", "v21", "v22", "v23", "v24", "v25", "v26", "v27", "v28", "v29", "v30", "v31", "v36", \
-        "v37", "v38", "v39", "v40", "v41", "v42", "v43", "v44", "v45", "v46", "v47", "v48",    \
-        "v49", "v50", "v51", "v52", "v53", "v54", "v55", "v56", "v57", "v58", "v59", "v60",    \
-        "v61", "v62", "v63", "v64", "v65", "v66", "v67", "v68", "v69", "v70", "v71"
-#define FREP_IASM_CLOBBER_8                                                                   \
-    "v20", "v21", "v22", "v23", "v28", "v29", "v30", "v31", "v32", "v33", "v34", "v35", "v36", \
-        "v37", "v38", "v39", "v40", "v41", "v42", "v43", "v44", "v45", "v46", "v47", "v48",    \
-        "v49", "v50", "v51", "v52", "v53", "v54", "v55"
-
-typedef float v4f __attribute__((ext_vector_type(4)));
-
-// Interval forward pass of one tape, one cell per lane.  lo / hi: slots 0..3
-// on entry (x, y, z boxes and the time), slot 0 on exit.  tw: instruction
-// words (n of them); consts: the shape's pool (index 0 readable even when
-// empty); la: LDS byte address of the wavefront's first choice row.
-// Returns the number of MIN/MAX rows written.  MAXS must be 8 or 16; every
-// lane of the wavefront must be active.
-template <int MAXS>
-__device__ __forceinline__ uint32_t run_interval_tape_asm(const uint32_t* tw, uint32_t n,
-                                                          const float* consts, uint32_t la,
-                                                          v4f& lo, v4f& hi) {
-    static_assert(MAXS == 8 || MAXS == 16, "assembly path covers 8 or 16 slots");
-    const uint64_t twp = (uint64_t)tw, cp = (uint64_t)consts;
-    uint32_t w, a, b, o, c, j, m, s, sidx, nm1, cls, tiny, mm;
-    uint64_t p0, p1, p2, p3;
-    if constexpr (MAXS == 16) {
-        asm volatile(FREP_IASM_BODY
-                     : "+{v[16:19]}"(lo), "+{v[32:35]}"(hi), [la] "+v"(la), [w] "=&s"(w),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [c] "=&s"(c), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [mm] "=&s"(mm), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
-                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [maxs] "n"(MAXS), [lg] "n"(4)
-                     : FREP_IASM_CLOBBER_16, "vcc", "memory");
-    } else {
-        asm volatile(FREP_IASM_BODY
-                     : "+{v[16:19]}"(lo), "+{v[24:27]}"(hi), [la] "+v"(la), [w] "=&s"(w),
-                       [a] "=&s"(a), [b] "=&s"(b), [o] "=&s"(o), [c] "=&s"(c), [j] "=&s"(j),
-                       [m] "=&s"(m), [s] "=&s"(s), [sidx] "=&s"(sidx), [nm1] "=&s"(nm1),
-                       [cls] "=&s"(cls), [tiny] "=&s"(tiny), [mm] "=&s"(mm), [p0] "=&s"(p0),
-                       [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
-                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [maxs] "n"(MAXS), [lg] "n"(3)
-                     : FREP_IASM_CLOBBER_8, "vcc", "memory");
-    }
-    return mm;
-}
-
 }  // namespace frep

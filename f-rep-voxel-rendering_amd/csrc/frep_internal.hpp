@@ -174,11 +174,9 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
                             const uint32_t* root_tape, uint32_t root_off, uint32_t n_levels);
 // Level kernels run a persistent grid of min(max_grid, cu_count x resident
 // blocks per CU) blocks; max_grid bounds the per-wave scratch rows.
-// asm_ok: every op of the tape is in the assembly interpreters' set.
 // slot_count: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
-                                 bool asm_ok);
+                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid);
 // asm_ok: every op of the tape is in the assembly interpreter's set.
 // slots: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,

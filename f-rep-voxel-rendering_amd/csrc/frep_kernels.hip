@@ -169,7 +169,7 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 // second pass.  If nothing was removed the child reuses the parent's tape
 // (shorten.comp:188-191).
 // ---------------------------------------------------------------------------
-template <int MAXS, bool ASM, class Reserve>
+template <int MAXS, class Reserve>
 __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                uint64_t child_base, uint32_t lane, uint4* scr,
                                                const Reserve& reserve) {
@@ -197,15 +197,6 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     // ---- forward pass: interval evaluation (interval.comp:121-160) ----
     float ilo, ihi;
     uint32_t mm = 0;
-    if constexpr (ASM) {
-        typedef __attribute__((address_space(3))) uint4 lds_row_t;
-        v4f lo = {lx, ly, lz, a.tape_time};
-        v4f hi = {lx + a.cs, ly + a.cs, lz + a.cs, a.tape_time};
-        mm = run_interval_tape_asm<MAXS>(tw, n, a.consts, (uint32_t)(uintptr_t)(lds_row_t*)scr,
-                                         lo, hi);
-        ilo = lo[0];
-        ihi = hi[0];
-    } else {
     float slo[MAXS], shi[MAXS];
     slo[0] = lx; shi[0] = lx + a.cs;
     slo[1] = ly; shi[1] = ly + a.cs;
@@ -295,7 +286,6 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     }
     ilo = slo[0];
     ihi = shi[0];
-    }
     // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
     const bool full = ihi < 0.0f;
     const bool empty = ilo > 0.0f;
@@ -411,7 +401,7 @@ __device__ __forceinline__ uint64_t child_tape_base(const uint32_t* counters, ui
 // Slot files of up to 12 slots are built for 8 wavefronts per SIMD (at most
 // 64 VGPRs and a scalar budget to match); larger ones keep the compiler's
 // choice.
-template <int MAXS, bool ASM>
+template <int MAXS>
 __global__ __launch_bounds__(kLevelBlock)
 __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval(LevelArgs a) {
     // min/max choice ballots of this wavefront's current chunk: mm_stride rows
@@ -438,7 +428,7 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval(L
         const uint32_t chunk = g * WPB + wib;
         if (chunk < total) {
             const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
-            interval_chunk<MAXS, ASM>(a, node, sub, child_base, lane, scr, br);
+            interval_chunk<MAXS>(a, node, sub, child_base, lane, scr, br);
         } else {
             br(ctr, 0, lane);
         }
@@ -1085,7 +1075,7 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval_b
         if (skip) continue;
         const uint32_t local = chunk - start;
         const uint32_t node = local / cpn, sub = local - node * cpn;
-        interval_chunk<MAXS, false>(a, node, sub, cb, lane, scr, WaveReserve{});
+        interval_chunk<MAXS>(a, node, sub, cb, lane, scr, WaveReserve{});
     }
 }
 
@@ -1584,12 +1574,12 @@ uint32_t resident_grid(Kern kern, uint32_t cu_count, uint32_t max_grid, size_t l
 
 // The occupancy of an interval kernel depends on its dynamic LDS (choice
 // rows); the last answer is kept per instantiation and thread.
-template <int M, bool ASM>
+template <int M>
 uint32_t interval_grid(uint32_t cu_count, uint32_t max_grid, size_t lds) {
     thread_local size_t last_lds = ~(size_t)0;
     thread_local uint32_t last_cu = 0, last_max = 0, g = 0;
     if (lds != last_lds || cu_count != last_cu || max_grid != last_max) {
-        g = resident_grid(k_level_interval<M, ASM>, cu_count, max_grid, lds);
+        g = resident_grid(k_level_interval<M>, cu_count, max_grid, lds);
         last_lds = lds;
         last_cu = cu_count;
         last_max = max_grid;
@@ -1612,41 +1602,19 @@ size_t interval_lds_bytes(uint32_t mm_stride) {
     return (size_t)(kLevelBlock / 64) * mm_stride * sizeof(uint4);
 }
 
-// The assembly interval interpreter (run_interval_tape_asm) is not
-// dispatched: its first run on the device, on the 1024^3 random CSG, ended in
-// a device fault whose cause has not been found (an instruction-level
-// emulation of its code matches the C++ rules on that tape), so every
-// interval level runs the C++ interpreter.
-constexpr bool kIntervalAsm = false;
-
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
-                                 bool asm_ok) {
+                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
     const size_t lds = interval_lds_bytes(a.mm_stride);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
-    if constexpr (kIntervalAsm) {
-        if (asm_ok && maxs == 8) {
-            k_level_interval<8, true><<<interval_grid<8, true>(cu_count, max_grid, lds),
-                                        kLevelBlock, lds, st>>>(a);
-            return hipGetLastError();
-        }
-        if (asm_ok && maxs == 16) {
-            k_level_interval<16, true><<<interval_grid<16, true>(cu_count, max_grid, lds),
-                                         kLevelBlock, lds, st>>>(a);
-            return hipGetLastError();
-        }
-    }
-    (void)asm_ok;
     // a 12-slot file frees 8 registers per lane over the 16-slot one, which
     // is what 8 wavefronts per SIMD need
     if (maxs == 16 && slot_count <= 12) {
-        k_level_interval<12, false><<<interval_grid<12, false>(cu_count, max_grid, lds),
-                                      kLevelBlock, lds, st>>>(a);
+        k_level_interval<12><<<interval_grid<12>(cu_count, max_grid, lds), kLevelBlock, lds,
+                               st>>>(a);
         return hipGetLastError();
     }
-    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M, false><<<interval_grid<M, false>(
-                                                               cu_count, max_grid, lds),
-                                                           kLevelBlock, lds, st>>>(a)));
+    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<interval_grid<M>(cu_count, max_grid, lds),
+                                                    kLevelBlock, lds, st>>>(a)));
     return hipGetLastError();
 }
 

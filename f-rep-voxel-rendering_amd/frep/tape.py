@@ -308,6 +308,9 @@ def compile_tape(bld: Builder, root: int) -> Tape:
         out = free_slot()
         slots[out] = node
         words.append(encode(op, out, a, b))
+    if bld.nodes[root][0] == "axis" and bld.nodes[root][1] != "x":
+        # a bare y, z or t root: the result must still land in slot 0
+        words.append(encode(OP_COPY, 0, _AXES.index(bld.nodes[root][1]), 0))
     if words and ((words[-1] >> 8) & 0xFF) != 0:
         raise AssertionError("result is not in slot 0")
     arr = np.array([len(words)] + words, dtype=np.uint32)

@@ -633,3 +633,14 @@ def test_render_miss_pixels_show_background(oracle):
     assert ((hits != MISS).sum()) > 20
     assert np.all(rgba.reshape(-1, 4)[hits == MISS][:, :3] == 0)
     assert np.all(np.isnan(normals[hits == MISS]))
+
+
+@pytest.mark.parametrize("axis", ["x", "y", "z", "t"])
+def test_bare_axis_root_evaluates_to_that_axis(oracle, axis):
+    # a tape whose result is an input must still leave it in slot 0
+    b = Builder()
+    root = b.axis(axis)
+    tp = compile_tape(b, root)
+    for p in [(1.0, 2.0, 3.0, 4.0), (-0.5, 0.25, -7.0, 0.125)]:
+        want = b.eval(root, *p)
+        assert oracle.eval_point(tp, *p) == want == p["xyzt".index(axis)]

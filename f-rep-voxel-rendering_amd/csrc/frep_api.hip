@@ -25,6 +25,38 @@
 
 using namespace frep;
 
+// Output staging of frep_render_noalloc: a fixed array in the library's code
+// object, one copy per GPU, held until the process ends.  In host code
+// `scratch` only names the symbol; its device address comes from
+// hipGetSymbolAddress (scratch_address), which is what kernels, copies and
+// fills receive.  Every allocation-free render on a GPU shares that GPU's
+// copy, so such calls that overlap on one GPU race.
+static __device__ __align__(256) unsigned char scratch[FREP_NOALLOC_STAGING_BYTES];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of `scratch` on `device` (current), looked up once per
+// device.
+hipError_t scratch_address(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[device] = p;
+    }
+    *out = g_scratch_ptr[device];
+    return hipSuccess;
+}
+
+}  // namespace
+
 namespace frep {
 
 uint32_t maxs_for(uint32_t slot_count) {
@@ -407,6 +439,7 @@ frep_status frep_init(int device, frep_ctx** out) {
     const char* ro = std::getenv("FREP_ROOT_OPS");
     c->root_ops = !(ro && ro[0] == '0');
     c->root_ops_required = ro && ro[0] == '2';
+    HIP_TRY(scratch_address(device, &c->noalloc_staging));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
     for (auto& st : c->lanes.st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -1519,49 +1552,106 @@ frep_status frep_camera_look(frep_camera* cam, const float initial_up[3], float 
     return FREP_OK;
 }
 
-frep_status frep_render(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam, uint8_t* rgba,
-                        uint64_t* hit_ids, float* normals) {
-    if (!ctx || !cam) return FREP_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    TreeObj* t = get_tree(ctx, h);
-    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
-    {
-        const frep_status s = check_camera(ctx, cam);
-        if (s != FREP_OK) return s;
-    }
-    HIP_TRY(hipSetDevice(ctx->device));
+}  // extern "C"
+
+namespace {
+
+// Staging layout of a render's outputs: rgba, then hit ids, then normals,
+// each 256-byte aligned.
+struct RenderStaging {
+    size_t rgba_b, hit_b, nrm_b, off_hit, off_nrm, total;
+};
+
+RenderStaging render_staging(const frep_camera* cam, bool rgba, bool hits, bool normals) {
+    RenderStaging r;
     const uint32_t spp = cam->spp ? cam->spp : 1;
     const size_t px = (size_t)cam->width * cam->height;
-    const size_t rgba_b = rgba ? px * 4 : 0;
-    const size_t hit_b = hit_ids ? px * spp * 8 : 0;
-    const size_t nrm_b = normals ? px * spp * 12 : 0;
-    const size_t off_hit = (rgba_b + 255) & ~(size_t)255;
-    const size_t off_nrm = (off_hit + hit_b + 255) & ~(size_t)255;
-    const size_t total = off_nrm + nrm_b + 256;
-    if (total > ctx->out_bytes) {
-        if (ctx->d_out) HIP_TRY(hipFree(ctx->d_out));
-        ctx->d_out = nullptr;
-        ctx->out_bytes = 0;
-        HIP_TRY(hipMalloc(&ctx->d_out, total));
-        ctx->out_bytes = total;
-    }
-    char* base = (char*)ctx->d_out;
+    r.rgba_b = rgba ? px * 4 : 0;
+    r.hit_b = hits ? px * spp * 8 : 0;
+    r.nrm_b = normals ? px * spp * 12 : 0;
+    r.off_hit = (r.rgba_b + 255) & ~(size_t)255;
+    r.off_nrm = (r.off_hit + r.hit_b + 255) & ~(size_t)255;
+    r.total = r.off_nrm + r.nrm_b + 256;
+    return r;
+}
+
+// Renders into the device staging at `base` (laid out by render_staging)
+// and copies the requested outputs to the caller's host arrays.
+frep_status render_into(frep_ctx* ctx, TreeObj* t, const frep_camera* cam, char* base,
+                        const RenderStaging& S, uint8_t* rgba, uint64_t* hit_ids,
+                        float* normals) {
     const RenderArgs a = render_args(t, cam, rgba ? (uint8_t*)base : nullptr,
-                                     hit_ids ? (unsigned long long*)(base + off_hit) : nullptr,
-                                     normals ? (float*)(base + off_nrm) : nullptr);
+                                     hit_ids ? (unsigned long long*)(base + S.off_hit) : nullptr,
+                                     normals ? (float*)(base + S.off_nrm) : nullptr);
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 2], ctx->stream));
     HIP_TRY(launch_render(ctx->stream, a, t->maxs));
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 3], ctx->stream));
-    if (rgba) HIP_TRY(hipMemcpyAsync(rgba, base, rgba_b, hipMemcpyDeviceToHost, ctx->stream));
+    if (rgba) HIP_TRY(hipMemcpyAsync(rgba, base, S.rgba_b, hipMemcpyDeviceToHost, ctx->stream));
     if (hit_ids)
-        HIP_TRY(hipMemcpyAsync(hit_ids, base + off_hit, hit_b, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(hit_ids, base + S.off_hit, S.hit_b, hipMemcpyDeviceToHost,
+                               ctx->stream));
     if (normals)
-        HIP_TRY(hipMemcpyAsync(normals, base + off_nrm, nrm_b, hipMemcpyDeviceToHost, ctx->stream));
+        HIP_TRY(hipMemcpyAsync(normals, base + S.off_nrm, S.nrm_b, hipMemcpyDeviceToHost,
+                               ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->timing)
         HIP_TRY(hipEventElapsedTime(&ctx->last.render_ms, ctx->ev[kMaxLevels + 2],
                                     ctx->ev[kMaxLevels + 3]));
     return FREP_OK;
+}
+
+frep_status render_checks(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam, TreeObj** out) {
+    TreeObj* t = get_tree(ctx, h);
+    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
+    const frep_status s = check_camera(ctx, cam);
+    if (s != FREP_OK) return s;
+    *out = t;
+    return FREP_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+frep_status frep_render(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam, uint8_t* rgba,
+                        uint64_t* hit_ids, float* normals) {
+    if (!ctx || !cam) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = nullptr;
+    {
+        const frep_status s = render_checks(ctx, h, cam, &t);
+        if (s != FREP_OK) return s;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    const RenderStaging S = render_staging(cam, rgba, hit_ids, normals);
+    if (S.total > ctx->out_bytes) {
+        if (ctx->d_out) HIP_TRY(hipFree(ctx->d_out));
+        ctx->d_out = nullptr;
+        ctx->out_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->d_out, S.total));
+        ctx->out_bytes = S.total;
+    }
+    return render_into(ctx, t, cam, (char*)ctx->d_out, S, rgba, hit_ids, normals);
+}
+
+frep_status frep_render_noalloc(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam,
+                                uint8_t* rgba, uint64_t* hit_ids, float* normals) {
+    if (!ctx || !cam) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = nullptr;
+    {
+        const frep_status s = render_checks(ctx, h, cam, &t);
+        if (s != FREP_OK) return s;
+    }
+    const RenderStaging S = render_staging(cam, rgba, hit_ids, normals);
+    if (S.total > (size_t)FREP_NOALLOC_STAGING_BYTES)
+        return fail(ctx, FREP_INVALID_ARG,
+                    "frep_render_noalloc: the requested outputs need " + std::to_string(S.total) +
+                        " bytes of staging, more than its fixed " +
+                        std::to_string((size_t)FREP_NOALLOC_STAGING_BYTES) + " bytes");
+    if (!ctx->noalloc_staging) return fail(ctx, FREP_DEVICE, "no staging address for this device");
+    HIP_TRY(hipSetDevice(ctx->device));
+    return render_into(ctx, t, cam, (char*)ctx->noalloc_staging, S, rgba, hit_ids, normals);
 }
 
 }  // extern "C"

@@ -210,6 +210,8 @@ struct frep_ctx {
     // render output staging
     void* d_out = nullptr;
     size_t out_bytes = 0;
+    // device address of the fixed staging array of frep_render_noalloc
+    void* noalloc_staging = nullptr;
     // staging of the synchronous batched voxelization
     frep::BatchStage stage;
     // tree buffers (device) and small pinned mirrors / request outputs

@@ -24,7 +24,7 @@ EXPORTED = [
     "frep_init", "frep_shutdown", "frep_last_error", "frep_set_timing", "frep_get_timing",
     "frep_device_info", "frep_tape_upload", "frep_tape_free", "frep_tree_create",
     "frep_voxelize", "frep_voxelize_batch", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
-    "frep_camera_look", "frep_render", "frep_eval_points", "frep_eval_intervals",
+    "frep_camera_look", "frep_render", "frep_render_noalloc", "frep_eval_points", "frep_eval_intervals",
     "frep_eval_gradients", "frep_math", "frep_root_schedule",
     "frep_server_create", "frep_server_destroy", "frep_server_last_error", "frep_server_scene",
     "frep_server_submit", "frep_server_wait", "frep_server_poll", "frep_server_stats_get",
@@ -126,6 +126,7 @@ def load():
         "frep_tree_free": (C.c_int, [vp, C.c_uint32]),
         "frep_camera_look": (C.c_int, [C.POINTER(Camera), f32p, C.c_float]),
         "frep_render": (C.c_int, [vp, C.c_uint32, C.POINTER(Camera), u8p, u64p, f32p]),
+        "frep_render_noalloc": (C.c_int, [vp, C.c_uint32, C.POINTER(Camera), u8p, u64p, f32p]),
         "frep_eval_points": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint32, C.c_float, f32p]),
         "frep_eval_intervals": (C.c_int, [vp, C.c_uint32, f32p, f32p, C.c_uint32, C.c_float,
                                           f32p, u8p]),
@@ -312,13 +313,15 @@ class Context:
         self._check(self.L.frep_tree_free(self.h, tree))
 
     # ---- render ----
-    def render(self, tree: int, cam: Camera, want=("rgba", "hits", "normals")):
+    def render(self, tree: int, cam: Camera, want=("rgba", "hits", "normals"), noalloc=False):
+        """frep_render, or with noalloc=True frep_render_noalloc (fixed staging)."""
         spp = max(1, cam.spp)
         npx = cam.width * cam.height
         rgba = np.zeros((cam.height, cam.width, 4), dtype=np.uint8) if "rgba" in want else None
         hits = np.zeros(npx * spp, dtype=np.uint64) if "hits" in want else None
         nrm = np.zeros((npx * spp, 3), dtype=np.float32) if "normals" in want else None
-        self._check(self.L.frep_render(
+        fn = self.L.frep_render_noalloc if noalloc else self.L.frep_render
+        self._check(fn(
             self.h, tree, C.byref(cam),
             _p(rgba, C.c_uint8) if rgba is not None else None,
             _p(hits, C.c_uint64) if hits is not None else None,

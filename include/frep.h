@@ -171,6 +171,21 @@ frep_status frep_camera_look(frep_camera* cam, const float initial_up[3], float 
 frep_status frep_render(frep_ctx* ctx, frep_tree_t tree, const frep_camera* cam,
                         uint8_t* rgba, uint64_t* hit_ids, float* normals);
 
+/* Allocation-free form of frep_render.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * arguments, kernels and outputs as frep_render, but the device staging of
+ * the outputs is a fixed array of FREP_NOALLOC_STAGING_BYTES held by the
+ * library on each GPU it runs on until the process ends, so the call makes
+ * no hipMalloc/hipFree.  Requests whose outputs need more staging than that
+ * (rgba 4 B per pixel, hit ids 8 B and normals 12 B per sample, each region
+ * rounded up to 256 B, plus 256 B) fail with FREP_INVALID_ARG.  All calls on
+ * one GPU share its array: calls that overlap on one GPU (from several
+ * threads or contexts) race. */
+#define FREP_NOALLOC_STAGING_BYTES (64u << 20)
+frep_status frep_render_noalloc(frep_ctx* ctx, frep_tree_t tree, const frep_camera* cam,
+                                uint8_t* rgba, uint64_t* hit_ids, float* normals);
+
 /* ---- batch tape evaluation (host pointers) ---- */
 frep_status frep_eval_points(frep_ctx* ctx, frep_tape_t tape, const float* xyz, uint32_t n,
                              float t, float* out);

@@ -3,14 +3,18 @@
 One step = one full voxelization (all tree levels, tape shortening included)
 of a 24-primitive random CSG tape over a 1024^3 grid (branching 16/8/8) on
 one MI355X, through the C ABI, with the tape already resident on the device.
-The metric is dense-equivalent voxel throughput: 1024^3 classified voxels per
-step / step time, in Gvoxel-evals/s.
+The value is executed voxel-evals per second: one interval evaluation per
+cell of every inner-level node plus one point evaluation per voxel of every
+last-level node, per step / step time.  The dense-equivalent rate (1024^3
+classified voxels per step / step time) is reported beside it as
+effective_gvoxel_per_s.
 
 Also reported: per-level kernel times (HIP events on the library's stream),
 the roofline fraction of the dominant kernel (algorithmic FP32 flops per
 launch / its mean duration / MI355X FP32 vector peak), a 1920x1080 render
 rate, and the reference's plain CPU evaluator (dense point sampling,
-restated in oracle/) timed on a bounded sample on this host.
+restated in oracle/) timed on bounded samples on this host, on all threads of
+its CPU share and on one thread.
 """
 from __future__ import annotations
 
@@ -28,6 +32,7 @@ sys.path.insert(0, os.path.join(ROOT, "f-rep-voxel-rendering_amd"))
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_PROFILE = os.path.join(ROOT, "profiles", "voxelize_csg24_1024_hbm.md")
+COUNTER_PROFILE = os.path.join(ROOT, "profiles", "voxelize_csg24_1024_valu_counters.md")
 WORKLOAD_TEXT = {
     "csg24_1024": "random CSG of 24 primitives (seed 2)",
     "spheres8_256": "union of 8 spheres (seed 1)",
@@ -51,29 +56,76 @@ def profiled_traffic(level):
     return None
 
 
-def cpu_baseline(tape, grid, budget_s=12.0):
-    """The reference's plain evaluator (SceneBuilder dense sampling, faithful
-    per-call allocation) on x-slabs of the same grid, all host threads."""
+def profiled_counters(level):
+    """FP32 VALU figures of a level kernel of the headline workload from the
+    committed rocprofv3 counter summary: VALU instructions, FP32 add/mul/
+    transcendental instructions per dispatch (x 64 lanes = FP32 lane ops),
+    and the counter-derived FP32 op rate over the traced mean duration."""
+    try:
+        text = open(COUNTER_PROFILE).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        if line.startswith("<!-- counters:"):
+            table = json.loads(line[len("<!-- counters:"):].rstrip("-> ").strip())
+            v = table.get(f"level{level}")
+            if v is not None:
+                v = dict(v)
+                v["source"] = os.path.relpath(COUNTER_PROFILE, ROOT)
+            return v
+    return None
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(tape, grid, budgets=(("faithful", 0, 10.0), ("tight", 0, 5.0),
+                                       ("faithful", 1, 5.0), ("tight", 1, 4.0))):
+    """The reference's plain evaluator (SceneBuilder dense sampling of every
+    fine voxel with Tape::Eval semantics) on x-slabs of the same grid.
+    "faithful" keeps the per-call slot-vector allocation of
+    vulkan/src/csg/tape.cpp:276, "tight" reuses one slot array.  Runs on the
+    host's full CPU share (at most 16 threads: the share of one GPU on these
+    boxes) and on 1 thread; each run times whole x-slabs for about its budget
+    in seconds.  The headline figure is the faithful run on all threads."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     O.build()
     F = 1
     for d in grid["dims"]:
         F *= d
-    threads = max(1, min(16, os.cpu_count() or 1))
-    slabs, t_used, x = 0, 0.0, 0
-    step = max(1, threads)
-    while t_used < budget_s and x < F:
-        t0 = time.perf_counter()
-        O.dense_occupancy(tape, grid, x, min(F, x + step), faithful=True, threads=threads)
-        t_used += time.perf_counter() - t0
-        slabs += min(F, x + step) - x
-        x += step
-    voxels = slabs * F * F
-    return {"value": voxels / t_used / 1e9, "unit": "Gvoxel-evals/s", "cores": threads,
-            "kind": "port",
-            "sample": f"{slabs} of {F} x-slabs ({voxels} voxels, {t_used:.1f} s), "
-                      f"dense Tape::Eval semantics with per-call slot allocation"}
+    n_threads = max(1, min(16, os.cpu_count() or 1))
+    runs = []
+    x = 0
+    for mode, single, budget_s in budgets:
+        threads = 1 if single else n_threads
+        slabs, t_used = 0, 0.0
+        while t_used < budget_s and slabs < F:
+            x0 = x % F
+            x1 = min(F, x0 + threads)
+            t0 = time.perf_counter()
+            O.dense_occupancy(tape, grid, x0, x1, faithful=(mode == "faithful"),
+                              threads=threads)
+            t_used += time.perf_counter() - t0
+            slabs += x1 - x0
+            x = x1
+        voxels = slabs * F * F
+        runs.append({"mode": mode, "threads": threads,
+                     "gvoxel_evals_per_s": round(voxels / t_used / 1e9, 5),
+                     "sample": f"{slabs} of {F} x-slabs, {voxels} voxels, {t_used:.1f} s"})
+    head = runs[0]
+    return {"value": head["gvoxel_evals_per_s"], "unit": "Gvoxel-evals/s",
+            "cores": head["threads"], "kind": "port",
+            "sample": f"{head['sample']}: dense Tape::Eval of the {tape.n}-op tape at every "
+                      f"voxel of the {F}^3 grid, per-call slot allocation",
+            "cpu_model": cpu_model(), "host_threads": os.cpu_count(), "runs": runs}
 
 
 def bench_batched(ctx, shapes, reps=5, warm=6):
@@ -270,10 +322,13 @@ def main():
     F = 1
     for d in grid["dims"]:
         F *= d
-    voxels = float(F) ** 3
-    value = voxels / (ms_per_step * 1e-3) / 1e9
-
     info = ctx.tree_info(tree)
+    # executed voxel-evals: every cell of every node of every level is one
+    # interval (inner levels) or point (last level) evaluation
+    evals = [int(info.node_count[L]) * int(grid["dims"][L]) ** 3 for L in range(nl)]
+    value = sum(evals) / (ms_per_step * 1e-3) / 1e9
+    dense_equivalent = float(F) ** 3 / (ms_per_step * 1e-3) / 1e9
+
     work = [{"flops": int(info.work_flops[L]), "op_evals": int(info.op_evals[L]),
              "kind": "point" if L == nl - 1 else "interval"} for L in range(nl)]
     dom = max(range(nl), key=lambda L: level_ms[L])
@@ -299,6 +354,13 @@ def main():
             "level_ms": [round(v, 4) for v in level_ms],
             "executed_op_evals": sum(w["op_evals"] for w in work),
         },
+        "value_basis": "executed voxel-evals per second: interval evaluations of every cell "
+                       "of every inner-level node plus point evaluations of every voxel of "
+                       "every last-level node",
+        "executed_voxel_evals_per_step": sum(evals),
+        "executed_voxel_evals_per_level": evals,
+        "effective_gvoxel_per_s": round(dense_equivalent, 3),
+        "effective_basis": f"dense-equivalent: {F}^3 voxels classified per step / step time",
         "roofline": {
             "bound": "valu",
             "kernel": f"level {dom} ({work[dom]['kind']} evaluation)",
@@ -310,8 +372,16 @@ def main():
             "traffic_source": "profiles/voxelize_csg24_1024_hbm.md (rocprofv3 FETCH_SIZE, "
                               "WRITE_SIZE per dispatch x copy-kernel calibration)",
             "algorithmic_flops_per_launch": work[dom]["flops"],
+            "per_level": [{"level": L, "kind": work[L]["kind"], "ms": round(level_ms[L], 4),
+                           "flops": work[L]["flops"],
+                           "tflops": round(work[L]["flops"] / (level_ms[L] * 1e-3) / 1e12, 4),
+                           "frac": round(work[L]["flops"] / (level_ms[L] * 1e-3) / 1e12
+                                         / FP32_PEAK_TFLOPS, 5)} for L in range(nl)],
         },
     }
+    counters = profiled_counters(dom) if args.workload == "csg24_1024" else None
+    if counters:
+        out["roofline"]["counters"] = counters
     if not args.no_render:
         fr = shapes.frame(args.workload)
         w, h = fr["width"], fr["height"]

@@ -684,9 +684,10 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
     t->consts_uid = tp->uid;
     // min/max choice rows live in the interval kernel's LDS
     j->mm_stride = std::max<uint32_t>(tp->n_mm, 1);
-    if (nl > 1 && j->mm_stride > kMaxMinMax)
+    if (nl > 1 && j->mm_stride > max_minmax(ctx->lds_per_block))
         return fail(ctx, FREP_UNSUPPORTED,
-                    "tape has more than " + std::to_string(kMaxMinMax) + " MIN/MAX instructions");
+                    "tape has more than " + std::to_string(max_minmax(ctx->lds_per_block)) +
+                        " MIN/MAX instructions");
     t->retries = 0;
     j->tp = tp;
     j->t = t;
@@ -1384,9 +1385,11 @@ frep_status frep_voxelize_batch(frep_ctx* ctx, uint32_t n, const frep_tape_t* ta
         if (!get_tree(ctx, trees[k])) return fail(ctx, FREP_INVALID_ARG, "unknown tree handle");
         const frep_status gs = check_grid(ctx, &grids[k]);
         if (gs != FREP_OK) return gs;
-        if (grids[k].n_levels > 1 && std::max<uint32_t>(get_tape(ctx, tapes[k])->n_mm, 1) > kMaxMinMax)
+        if (grids[k].n_levels > 1 && std::max<uint32_t>(get_tape(ctx, tapes[k])->n_mm, 1) >
+                                         max_minmax(ctx->lds_per_block))
             return fail(ctx, FREP_UNSUPPORTED,
-                        "tape has more than " + std::to_string(kMaxMinMax) + " MIN/MAX instructions");
+                        "tape has more than " + std::to_string(max_minmax(ctx->lds_per_block)) +
+                            " MIN/MAX instructions");
     }
     HIP_TRY(hipSetDevice(ctx->device));
     for (uint32_t k = 0; k < n; ++k) {

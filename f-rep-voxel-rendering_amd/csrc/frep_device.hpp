@@ -29,18 +29,19 @@ __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_re
 __device__ __forceinline__ float fmin_glm(float x, float y) { return y < x ? y : x; }
 __device__ __forceinline__ float fmax_glm(float x, float y) { return x < y ? y : x; }
 
-__device__ __forceinline__ float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+__host__ __device__ __forceinline__ float u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
 
 // ---------------------------------------------------------------------------
 // Transcendentals.  Cody & Waite (1980) argument reduction and Taylor
 // polynomials, using only correctly rounded operations and floor(), so the
-// CPU and GPU produce identical bits.
+// CPU and GPU produce identical bits (the tape compiler's constant folding
+// calls the same functions on the host).
 //   exp:  x = k ln2 + r, |r| <= ln2/2, two-part ln2, degree-7 polynomial,
 //         result scaled by 2^(k/2) and 2^(k - k/2) (two exact products).
 //   sin / cos: x = k pi/2 + r with a three-part pi/2 whose products with k
 //         are exact for |k| < 4096, degree-9 / degree-10 polynomials.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float dev_expf(float x) {
+__host__ __device__ __forceinline__ float dev_expf(float x) {
     if (x != x) return x;
     if (x > 88.72283935546875f) return __builtin_inff();
     if (x < -103.972084045410156f) return 0.0f;
@@ -62,7 +63,7 @@ __device__ __forceinline__ float dev_expf(float x) {
     return (p * s1) * s2;
 }
 
-__device__ __forceinline__ float sin_poly(float r) {
+__host__ __device__ __forceinline__ float sin_poly(float r) {
     const float z = r * r;
     float p = 2.75573188446287530661e-06f;
     p = -1.98412701138295233250e-04f + z * p;
@@ -71,7 +72,7 @@ __device__ __forceinline__ float sin_poly(float r) {
     return r + (r * z) * p;
 }
 
-__device__ __forceinline__ float cos_poly(float r) {
+__host__ __device__ __forceinline__ float cos_poly(float r) {
     const float z = r * r;
     float p = -2.75573199814971303567e-07f;
     p = 2.48015876422869041562e-05f + z * p;
@@ -81,7 +82,7 @@ __device__ __forceinline__ float cos_poly(float r) {
     return 1.0f + z * p;
 }
 
-__device__ __forceinline__ float reduce_half_pi(float x, int* q) {
+__host__ __device__ __forceinline__ float reduce_half_pi(float x, int* q) {
     const float kf = __builtin_floorf(x * 0.636619746685028076171875f + 0.5f);
     const float r = ((x - kf * 1.5703125f) - kf * 4.837512969970703125e-04f) -
                     kf * 7.549790126404332e-08f;
@@ -90,7 +91,7 @@ __device__ __forceinline__ float reduce_half_pi(float x, int* q) {
     return r;
 }
 
-__device__ __forceinline__ float dev_sinf(float x) {
+__host__ __device__ __forceinline__ float dev_sinf(float x) {
     if (!(__builtin_fabsf(x) <= 3.0e38f)) return x - x;
     int q;
     const float r = reduce_half_pi(x, &q);
@@ -98,7 +99,7 @@ __device__ __forceinline__ float dev_sinf(float x) {
     return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
 }
 
-__device__ __forceinline__ float dev_cosf(float x) {
+__host__ __device__ __forceinline__ float dev_cosf(float x) {
     if (!(__builtin_fabsf(x) <= 3.0e38f)) return x - x;
     int q;
     const float r = reduce_half_pi(x, &q);

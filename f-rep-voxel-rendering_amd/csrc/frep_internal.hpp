@@ -226,9 +226,15 @@ hipError_t launch_math(hipStream_t st, uint32_t fn, const float* x, uint32_t n, 
 constexpr uint32_t kLevelBlock = 256;
 
 // Min/max choice rows of the interval kernel live in LDS: 16 bytes per
-// MIN/MAX instruction per wavefront.  64 KiB per block caps a tape at 1024
-// MIN/MAX instructions.
-constexpr size_t kMaxChoiceLds = 64 * 1024;
-constexpr uint32_t kMaxMinMax = (uint32_t)(kMaxChoiceLds / 16 / (kLevelBlock / 64));
+// MIN/MAX instruction per wavefront, 4 wavefronts per block.  A block may
+// use up to the device's LDS per block (160 KiB on gfx950: 2544 MIN/MAX
+// instructions); rows past 64 KiB per block cost occupancy.
+// 1 KiB of the block's LDS is left for the kernels' static LDS (the
+// per-block reservation words).
+constexpr size_t kMaxChoiceLds = 159 * 1024;
+inline uint32_t max_minmax(size_t lds_per_block) {
+    const size_t b = lds_per_block - 1024 < kMaxChoiceLds ? lds_per_block - 1024 : kMaxChoiceLds;
+    return (uint32_t)(b / 16 / (kLevelBlock / 64));
+}
 
 }  // namespace frep

@@ -668,7 +668,7 @@ frep_status frep_server_scene(frep_server* srv, frep_tape_t tape, const frep_gri
         if (!tp) return srv_fail(srv, FREP_INVALID_ARG, "unknown tape handle");
         const frep_status gs = check_grid(ctx, grid);
         if (gs != FREP_OK) return srv_fail(srv, gs, ctx->err);
-        if (grid->n_levels > 1 && std::max<uint32_t>(tp->n_mm, 1) > kMaxMinMax)
+        if (grid->n_levels > 1 && std::max<uint32_t>(tp->n_mm, 1) > max_minmax(ctx->lds_per_block))
             return srv_fail(srv, FREP_UNSUPPORTED, "tape has too many MIN/MAX instructions");
     }
     std::lock_guard<std::mutex> lk(srv->mu);

@@ -25,7 +25,8 @@ EXPORTED = [
     "frep_device_info", "frep_tape_upload", "frep_tape_free", "frep_tree_create",
     "frep_voxelize", "frep_voxelize_batch", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
     "frep_camera_look", "frep_render", "frep_render_noalloc", "frep_eval_points", "frep_eval_intervals",
-    "frep_eval_gradients", "frep_math", "frep_root_schedule",
+    "frep_eval_gradients", "frep_math", "frep_root_schedule", "frep_compile",
+    "frep_compile_last_error",
     "frep_server_create", "frep_server_destroy", "frep_server_last_error", "frep_server_scene",
     "frep_server_submit", "frep_server_wait", "frep_server_poll", "frep_server_stats_get",
     "frep_server_reset_stats",
@@ -134,6 +135,9 @@ def load():
         "frep_math": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint32, f32p]),
         "frep_root_schedule": (C.c_int, [u32p, C.c_uint32, C.c_uint32, C.c_uint32, u32p, u32p,
                                          C.c_uint32]),
+        "frep_compile": (C.c_int, [vp, C.c_uint32, C.c_uint32, u32p, C.c_uint32, f32p,
+                                   C.c_uint32, vp]),
+        "frep_compile_last_error": (C.c_char_p, []),
         "frep_server_create": (C.c_int, [vp, C.POINTER(ServerConfig), C.POINTER(C.c_void_p)]),
         "frep_server_destroy": (C.c_int, [vp]),
         "frep_server_last_error": (C.c_char_p, [vp]),

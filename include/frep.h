@@ -199,6 +199,60 @@ frep_status frep_eval_gradients(frep_ctx* ctx, frep_tape_t tape, const float* xy
 /* fn: 0 sin, 1 cos, 2 exp -- the device implementations used by the tapes. */
 frep_status frep_math(frep_ctx* ctx, uint32_t fn, const float* x, uint32_t n, float* out);
 
+/* ---- tape compiler (host only, no device or context needed) ----
+ * Replaces csg::Tape::Tape(csg::Expr) (vulkan/src/csg/tape.cpp:14-21), which
+ * VoxelStorage::Init runs on the shape (vulkan/include/engine/voxel_storage.h:
+ * 35-49): ConstantFold, NormalForm + MergeDuplicates
+ * (vulkan/src/csg/simplifier.cpp:31-259), then topological order, constant
+ * pool, liveness and slot allocation (tape.cpp:23-228).  The expression DAG
+ * is a node list; node i's inputs a (and b) must be earlier nodes, and node
+ * identity is the list index (two equal constants at different indices are
+ * two nodes, as two csg::Expr(float) are).  Op codes follow csg::Operator
+ * (vulkan/include/csg/expression.h:14-22). */
+#define FREP_EXPR_X 0u
+#define FREP_EXPR_Y 1u
+#define FREP_EXPR_Z 2u
+#define FREP_EXPR_T 3u
+#define FREP_EXPR_CONST 4u  /* value */
+#define FREP_EXPR_SIN 5u    /* a */
+#define FREP_EXPR_COS 6u
+#define FREP_EXPR_EXP 7u
+#define FREP_EXPR_NEG 8u
+#define FREP_EXPR_SQRT 9u
+#define FREP_EXPR_ADD 10u   /* a, b */
+#define FREP_EXPR_SUB 11u
+#define FREP_EXPR_MUL 12u
+#define FREP_EXPR_DIV 13u
+#define FREP_EXPR_MIN 14u
+#define FREP_EXPR_MAX 15u
+
+typedef struct frep_expr_node {
+    uint32_t op;
+    uint32_t a, b;        /* input node indices (unused inputs ignored) */
+    float value;          /* CONST only */
+} frep_expr_node;
+
+typedef struct frep_compile_info {
+    uint32_t n_words;     /* tape words [n, instrs]: n + 1 */
+    uint32_t n_consts;    /* constant pool size */
+    uint32_t slot_count;  /* slots the tape uses (>= 4) */
+    uint32_t format;      /* FREP_TAPE_FORMAT_REF, or _WIDE past 256 constants */
+    uint32_t dag_nodes;   /* nodes given */
+    uint32_t tape_nodes;  /* nodes of the simplified DAG */
+} frep_compile_info;
+
+/* Compiles the DAG rooted at node `root`.  info is always filled; words
+ * (n_words) and consts (n_consts) are written when non-NULL, and
+ * FREP_INVALID_ARG is returned if their capacities are too small.  Inputs
+ * that are not a valid DAG, NaN constants and tapes past 256 slots are
+ * FREP_INVALID_ARG with the reason in frep_compile_last_error() (per
+ * thread).  The result uploads with frep_tape_upload(words, n_words,
+ * consts, n_consts, slot_count, format). */
+frep_status frep_compile(const frep_expr_node* nodes, uint32_t n_nodes, uint32_t root,
+                         uint32_t* words, uint32_t words_cap, float* consts,
+                         uint32_t consts_cap, frep_compile_info* info);
+const char* frep_compile_last_error(void);
+
 /* ---- level-0 schedule (host only, no device needed) ----
  * No counterpart in the reference: the reference evaluates the root tape one
  * instruction at a time (vulkan/shaders/tape/interval.comp:121-160).  This

@@ -682,6 +682,7 @@ frep_status prepare_job(frep_ctx* ctx, TapeObj* tp, TreeObj* t, const frep_grid*
         HIP_TRY(hipMemcpyAsync(t->d_consts, tp->d_consts, (size_t)nc * 4, hipMemcpyDeviceToDevice,
                                st));
     t->consts_uid = tp->uid;
+    t->n_consts = nc;
     // min/max choice rows live in the interval kernel's LDS
     j->mm_stride = std::max<uint32_t>(tp->n_mm, 1);
     if (nl > 1 && j->mm_stride > max_minmax(ctx->lds_per_block))
@@ -1465,12 +1466,14 @@ frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t h, frep_tree_info* out
     return FREP_OK;
 }
 
-frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t h, uint32_t* buf, uint64_t cap,
-                             uint64_t* n_words) {
-    if (!ctx || !n_words) return FREP_INVALID_ARG;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    TreeObj* t = get_tree(ctx, h);
-    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
+}  // extern "C"
+
+namespace {
+
+// The canonical blob of a built tree (layout in the oracle's header and the
+// integration notes): levels in (x, y, z) order, child links as canonical
+// indices, tapes by content.
+frep_status canonical_blob(frep_ctx* ctx, TreeObj* t, std::vector<uint32_t>& v) {
     HIP_TRY(hipSetDevice(ctx->device));
     const uint32_t nl = t->grid.n_levels;
     const uint64_t top = host_pool_top(t);
@@ -1502,7 +1505,6 @@ frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t h, uint32_t* buf, uint64
         rank[L].resize(cnt[L]);
         for (uint32_t k = 0; k < cnt[L]; ++k) rank[L][order[L][k]] = k;
     }
-    std::vector<uint32_t> v;
     v.push_back(0x54505246u);
     v.push_back(1u);
     v.push_back(nl);
@@ -1530,8 +1532,270 @@ frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t h, uint32_t* buf, uint64
             }
         }
     }
+    return FREP_OK;
+}
+
+// Tree file: a header with the grid, slot count and constant pool, then the
+// canonical blob.
+constexpr uint32_t kTreeFileMagic = 0x46505246u;   // 'FRPF'
+constexpr uint32_t kTreeBlobMagic = 0x54505246u;   // 'FRPT'
+constexpr uint32_t kTreeFileHeader = 18;
+
+inline uint32_t fbits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+inline float bitsf(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+}  // namespace
+
+extern "C" {
+
+frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t h, uint32_t* buf, uint64_t cap,
+                             uint64_t* n_words) {
+    if (!ctx || !n_words) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = get_tree(ctx, h);
+    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
+    std::vector<uint32_t> v;
+    {
+        const frep_status s = canonical_blob(ctx, t, v);
+        if (s != FREP_OK) return s;
+    }
     *n_words = v.size();
     if (buf && cap >= v.size()) std::memcpy(buf, v.data(), v.size() * 4);
+    return FREP_OK;
+}
+
+frep_status frep_tree_save(frep_ctx* ctx, frep_tree_t h, uint32_t* buf, uint64_t cap,
+                           uint64_t* n_words) {
+    if (!ctx || !n_words) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = get_tree(ctx, h);
+    if (!t || !t->built) return fail(ctx, FREP_INVALID_ARG, "unknown or unbuilt tree");
+    std::vector<uint32_t> v(kTreeFileHeader, 0u);
+    v[0] = kTreeFileMagic;
+    v[1] = 1u;
+    v[2] = t->grid.n_levels;
+    for (uint32_t L = 0; L < t->grid.n_levels; ++L) v[3 + L] = t->grid.dims[L];
+    for (int c = 0; c < 3; ++c) v[11 + c] = fbits(t->grid.low[c]);
+    v[14] = fbits(t->grid.world_size);
+    v[15] = fbits(t->grid.tape_time);
+    v[16] = t->slot_count;
+    std::vector<float> consts(t->consts_cap);
+    uint32_t nc = t->n_consts;
+    v[17] = nc;
+    if (nc) {
+        HIP_TRY(hipSetDevice(ctx->device));
+        HIP_TRY(hipMemcpy(consts.data(), t->d_consts, (size_t)nc * 4, hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < nc; ++i) v.push_back(fbits(consts[i]));
+    }
+    {
+        const frep_status s = canonical_blob(ctx, t, v);
+        if (s != FREP_OK) return s;
+    }
+    *n_words = v.size();
+    if (buf && cap >= v.size()) std::memcpy(buf, v.data(), v.size() * 4);
+    return FREP_OK;
+}
+
+frep_status frep_tree_load(frep_ctx* ctx, const uint32_t* buf, uint64_t n_words,
+                           frep_tree_t* out) {
+    if (!ctx || !buf || !out) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto bad = [&](const std::string& why) {
+        return fail(ctx, FREP_INVALID_ARG, "tree file: " + why);
+    };
+    if (n_words < kTreeFileHeader || buf[0] != kTreeFileMagic || buf[1] != 1u)
+        return bad("not a version-1 tree file");
+    frep_grid g{};
+    g.n_levels = buf[2];
+    if (g.n_levels < 1 || g.n_levels > kMaxLevels) return bad("level count");
+    for (uint32_t L = 0; L < g.n_levels; ++L) g.dims[L] = buf[3 + L];
+    for (int c = 0; c < 3; ++c) g.low[c] = bitsf(buf[11 + c]);
+    g.world_size = bitsf(buf[14]);
+    g.tape_time = bitsf(buf[15]);
+    {
+        const frep_status s = check_grid(ctx, &g);
+        if (s != FREP_OK) return s;
+    }
+    const uint32_t nl = g.n_levels;
+    const uint32_t slots = buf[16], nc = buf[17];
+    if (slots < 4 || slots > 256) return bad("slot count");
+    if (nc > 65536) return bad("constant count");
+    uint64_t pos = kTreeFileHeader + (uint64_t)nc;
+    if (pos + 3 + 2 * (uint64_t)nl > n_words) return bad("truncated");
+    std::vector<float> consts(nc);
+    for (uint32_t i = 0; i < nc; ++i) consts[i] = bitsf(buf[kTreeFileHeader + i]);
+    const uint32_t* b = buf;
+    if (b[pos] != kTreeBlobMagic || b[pos + 1] != 1u || b[pos + 2] != nl)
+        return bad("canonical blob header");
+    pos += 3;
+    for (uint32_t L = 0; L < nl; ++L)
+        if (b[pos + L] != g.dims[L]) return bad("blob dims differ from the grid");
+    std::vector<uint32_t> cnt(nl);
+    for (uint32_t L = 0; L < nl; ++L) cnt[L] = b[pos + nl + L];
+    pos += 2 * (uint64_t)nl;
+    if (cnt[0] != 1) return bad("level 0 must hold one node");
+    // records in the device layout, tapes appended level by level
+    std::vector<std::vector<uint32_t>> recs(nl);
+    std::vector<uint32_t> pool(kRootOff, 0u);
+    std::vector<uint64_t> words(nl, 0);
+    std::vector<std::vector<uint32_t>> coords(nl);
+    std::vector<std::vector<uint8_t>> seen(nl);
+    uint64_t base1 = 0;
+    bool asm_ok = !ctx->no_asm, reads_t = false;
+    uint32_t n_mm = 0;
+    for (uint32_t L = 0; L < nl; ++L) {
+        if (cnt[L] > kMaxLevelNodes) return bad("too many nodes in a level");
+        const uint32_t dim = g.dims[L], cells = dim * dim * dim, W = cells / 32;
+        const bool last = (L + 1 == nl);
+        const uint32_t ns = node_words(dim, last);
+        recs[L].assign((size_t)cnt[L] * ns, 0u);
+        coords[L].resize((size_t)cnt[L] * 3);
+        seen[L].assign(cnt[L], 0);
+        for (uint32_t k = 0; k < cnt[L]; ++k) {
+            if (pos + 4 > n_words) return bad("truncated");
+            uint32_t* r = recs[L].data() + (size_t)k * ns;
+            for (int c = 0; c < 3; ++c) coords[L][3 * k + c] = r[1 + c] = b[pos + c];
+            if (k > 0) {
+                const uint32_t* p = &coords[L][3 * (k - 1)];
+                const uint32_t* q = &coords[L][3 * k];
+                if (!(p[0] < q[0] || (p[0] == q[0] && (p[1] < q[1] || (p[1] == q[1] && p[2] < q[2])))))
+                    return bad("nodes out of canonical order");
+            }
+            pos += 3;
+            const uint32_t tk = b[pos];
+            if (pos + 1 + tk > n_words) return bad("truncated tape");
+            if (validate_tape(b + pos, tk + 1, nc, slots, FREP_TAPE_FORMAT_WIDE) != 0)
+                return bad("malformed tape");
+            r[0] = (uint32_t)pool.size();
+            pool.insert(pool.end(), b + pos, b + pos + tk + 1);
+            if (L == 0) {
+                while (pool.size() % 8) pool.push_back(0u);
+                base1 = pool.size();
+                bool t_written = false;
+                for (uint32_t i = 0; i < tk; ++i) {
+                    const uint32_t w = b[pos + 1 + i], op = w & 0xFFu;
+                    n_mm += (op == OP_MIN || op == OP_MAX);
+                    asm_ok = asm_ok && asm_op_ok(op);
+                    if (op != OP_CONST && !t_written) {
+                        const bool binary = op == OP_ADD || op == OP_SUB || op == OP_MUL ||
+                                            op == OP_DIV || op == OP_MIN || op == OP_MAX;
+                        if (((w >> 16) & 0xFFu) == 3u || (binary && (w >> 24) == 3u)) reads_t = true;
+                    }
+                    if (((w >> 8) & 0xFFu) == 3u) t_written = true;
+                }
+            } else {
+                words[L] += tk + 1;
+            }
+            pos += tk + 1;
+            const uint64_t need = (uint64_t)W * (last ? 1 : 2) + (last ? 0 : cells);
+            if (pos + need > n_words) return bad("truncated node");
+            std::memcpy(r + 4, b + pos, (size_t)W * 4);
+            pos += W;
+            if (!last) {
+                std::memcpy(r + 4 + W, b + pos, (size_t)W * 4);
+                pos += W;
+                for (uint32_t i = 0; i < cells; ++i) {
+                    const uint32_t ch = b[pos + i];
+                    const bool leaf = (r[4 + (i >> 5)] >> (i & 31)) & 1u;
+                    const bool inner = (r[4 + W + (i >> 5)] >> (i & 31)) & 1u;
+                    if (leaf && inner) return bad("cell both leaf and interior");
+                    if (inner != (ch != kNoChild)) return bad("interior bit and child link disagree");
+                    r[4 + 2 * W + i] = ch;
+                }
+                pos += cells;
+            }
+        }
+    }
+    if (pos != n_words) return bad("trailing words");
+    // tree structure: every node below the root has exactly one parent, at
+    // the cell its coordinates name
+    for (uint32_t L = 0; L + 1 < nl; ++L) {
+        const uint32_t dim = g.dims[L], cells = dim * dim * dim, W = cells / 32;
+        const uint32_t ns = node_words(dim, false);
+        for (uint32_t k = 0; k < cnt[L]; ++k) {
+            const uint32_t* r = recs[L].data() + (size_t)k * ns;
+            for (uint32_t i = 0; i < cells; ++i) {
+                const uint32_t ch = r[4 + 2 * W + i];
+                if (ch == kNoChild) continue;
+                if (ch >= cnt[L + 1] || seen[L + 1][ch]) return bad("child link");
+                seen[L + 1][ch] = 1;
+                const uint32_t cx = i / (dim * dim), cy = (i / dim) % dim, cz = i % dim;
+                const uint32_t* q = &coords[L + 1][3 * (size_t)ch];
+                if (q[0] != r[1] * dim + cx || q[1] != r[2] * dim + cy || q[2] != r[3] * dim + cz)
+                    return bad("child coordinates");
+            }
+        }
+        for (uint32_t k = 0; k < cnt[L + 1]; ++k)
+            if (!seen[L + 1][k]) return bad("node without a parent");
+    }
+    if (coords[0][0] || coords[0][1] || coords[0][2]) return bad("root coordinates");
+    if (nl > 1 && std::max<uint32_t>(n_mm, 1) > max_minmax(ctx->lds_per_block))
+        return fail(ctx, FREP_UNSUPPORTED, "tape has too many MIN/MAX instructions");
+    // device copy
+    HIP_TRY(hipSetDevice(ctx->device));
+    frep_tree_t hnd = 0;
+    {
+        const frep_status s = new_tree(ctx, &hnd);
+        if (s != FREP_OK) return s;
+    }
+    TreeObj* t = get_tree(ctx, hnd);
+    auto drop = [&](frep_status s) {
+        free_tree_buffers(ctx, t);
+        ctx->dpool.free(t->d_counters);
+        ctx->hpool.free(t->h_counters);
+        ctx->trees[hnd - 1].reset();
+        return s;
+    };
+    t->grid = g;
+    t->slot_count = slots;
+    t->maxs = maxs_for(slots);
+    t->asm_ok = asm_ok;
+    t->reads_t = reads_t;
+    float c = g.world_size;
+    for (uint32_t L = 0; L < nl; ++L) {
+        c /= (float)g.dims[L];
+        t->cs[L] = c;
+        t->ns[L] = node_words(g.dims[L], L + 1 == nl);
+        const frep_status s = ensure_level(ctx, t, L, std::max<uint32_t>(cnt[L], 1));
+        if (s != FREP_OK) return drop(s);
+        if (cnt[L])
+            if (hipMemcpy(t->d_nodes[L], recs[L].data(), recs[L].size() * 4,
+                          hipMemcpyHostToDevice) != hipSuccess)
+                return drop(fail(ctx, FREP_DEVICE, "tree upload"));
+    }
+    {
+        const frep_status s = ensure_pool(ctx, t, pool.size());
+        if (s != FREP_OK) return drop(s);
+    }
+    if (hipMemset(t->d_pool, 0, (pool.size() + kPoolPad) * 4) != hipSuccess ||
+        hipMemcpy(t->d_pool, pool.data(), pool.size() * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return drop(fail(ctx, FREP_DEVICE, "tape pool upload"));
+    void* p = nullptr;
+    if (ctx->dpool.alloc((size_t)(nc + kPoolPad) * 4, &p) != hipSuccess)
+        return drop(fail(ctx, FREP_OUT_OF_MEMORY, "constant pool"));
+    t->d_consts = static_cast<float*>(p);
+    t->consts_cap = nc + kPoolPad;
+    t->n_consts = nc;
+    t->consts_uid = 0;
+    if (nc && hipMemcpy(t->d_consts, consts.data(), (size_t)nc * 4, hipMemcpyHostToDevice) !=
+                  hipSuccess)
+        return drop(fail(ctx, FREP_DEVICE, "constant upload"));
+    // counters: node counts and tape words per level on the host mirror; the
+    // device block as a finished build leaves it (zero but the level-1 base)
+    std::memset(t->h_counters, 0, CNT_WORDS * 4);
+    t->h_counters[CNT_BASE1] = (uint32_t)base1;
+    for (uint32_t L = 1; L < nl; ++L) {
+        const uint64_t v = ((uint64_t)cnt[L] << kAllocShift) | words[L];
+        std::memcpy(t->h_counters + CNT_ALLOC + 2 * L, &v, 8);
+    }
+    std::vector<uint32_t> dc(CNT_WORDS, 0u);
+    dc[CNT_BASE1] = (uint32_t)base1;
+    if (hipMemcpy(t->d_counters, dc.data(), CNT_WORDS * 4, hipMemcpyHostToDevice) != hipSuccess)
+        return drop(fail(ctx, FREP_DEVICE, "counter upload"));
+    ++t->dev_gen;
+    t->ready_gen = 0;
+    t->built = true;
+    *out = hnd;
     return FREP_OK;
 }
 

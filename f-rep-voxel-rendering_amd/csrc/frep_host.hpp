@@ -58,6 +58,7 @@ struct TreeObj {
     uint64_t pool_cap = 0;             // usable words (allocation has kPoolPad more)
     float* d_consts = nullptr;
     uint32_t consts_cap = 0;
+    uint32_t n_consts = 0;             // constants of the tape d_consts holds
     uint32_t* d_counters = nullptr;
     uint32_t* h_counters = nullptr;    // pinned mirror
     uint32_t slot_count = 0, maxs = 0;

@@ -24,6 +24,7 @@ EXPORTED = [
     "frep_init", "frep_shutdown", "frep_last_error", "frep_set_timing", "frep_get_timing",
     "frep_device_info", "frep_tape_upload", "frep_tape_free", "frep_tree_create",
     "frep_voxelize", "frep_voxelize_batch", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
+    "frep_tree_save", "frep_tree_load",
     "frep_camera_look", "frep_render", "frep_render_noalloc", "frep_eval_points", "frep_eval_intervals",
     "frep_eval_gradients", "frep_math", "frep_root_schedule", "frep_compile",
     "frep_compile_last_error",
@@ -125,6 +126,8 @@ def load():
         "frep_tree_info_get": (C.c_int, [vp, C.c_uint32, C.POINTER(TreeInfo)]),
         "frep_tree_export": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint64, u64p]),
         "frep_tree_free": (C.c_int, [vp, C.c_uint32]),
+        "frep_tree_save": (C.c_int, [vp, C.c_uint32, u32p, C.c_uint64, u64p]),
+        "frep_tree_load": (C.c_int, [vp, u32p, C.c_uint64, u32p]),
         "frep_camera_look": (C.c_int, [C.POINTER(Camera), f32p, C.c_float]),
         "frep_render": (C.c_int, [vp, C.c_uint32, C.POINTER(Camera), u8p, u64p, f32p]),
         "frep_render_noalloc": (C.c_int, [vp, C.c_uint32, C.POINTER(Camera), u8p, u64p, f32p]),
@@ -304,6 +307,24 @@ class Context:
         info = TreeInfo()
         self._check(self.L.frep_tree_info_get(self.h, tree, C.byref(info)))
         return info
+
+    def save(self, tree: int, path=None) -> np.ndarray:
+        """frep_tree_save: the tree file as u32 words (also written to path)."""
+        n = C.c_uint64()
+        self._check(self.L.frep_tree_save(self.h, tree, None, 0, C.byref(n)))
+        buf = np.zeros(n.value, dtype=np.uint32)
+        self._check(self.L.frep_tree_save(self.h, tree, _p(buf, C.c_uint32), n.value, C.byref(n)))
+        if path is not None:
+            buf.astype("<u4").tofile(path)
+        return buf
+
+    def load(self, src) -> int:
+        """frep_tree_load from u32 words or a file path; returns the new tree."""
+        buf = np.fromfile(src, dtype="<u4") if isinstance(src, (str, bytes)) or \
+            hasattr(src, "__fspath__") else np.ascontiguousarray(src, dtype=np.uint32)
+        out = C.c_uint32()
+        self._check(self.L.frep_tree_load(self.h, _p(buf, C.c_uint32), len(buf), C.byref(out)))
+        return out.value
 
     def export(self, tree: int) -> np.ndarray:
         n = C.c_uint64()

@@ -112,3 +112,36 @@ def occupancy(levels: list[Level]) -> np.ndarray:
 
     rec(0, 0)
     return occ
+
+
+FILE_MAGIC = 0x46505246   # 'FRPF'
+FILE_HEADER = 18
+
+
+def file_words(grid: dict, consts, slot_count: int, blob) -> np.ndarray:
+    """A tree file (frep_tree_save's layout) from a grid, the tape's constant
+    pool and slot count, and a canonical blob."""
+    consts = np.asarray(consts, dtype=np.float32)
+    h = np.zeros(FILE_HEADER, dtype=np.uint32)
+    h[0], h[1], h[2] = FILE_MAGIC, 1, len(grid["dims"])
+    h[3:3 + len(grid["dims"])] = grid["dims"]
+    h[11:14] = np.asarray(grid["low"], dtype=np.float32).view(np.uint32)
+    h[14] = np.float32(grid["world_size"]).view(np.uint32)
+    h[15] = np.float32(grid.get("tape_time", 0.0)).view(np.uint32)
+    h[16], h[17] = slot_count, len(consts)
+    return np.concatenate([h, consts.view(np.uint32), np.asarray(blob, dtype=np.uint32)])
+
+
+def parse_file(words):
+    """(grid dict, constants, slot count, canonical blob) of a tree file."""
+    w = np.asarray(words, dtype=np.uint32)
+    if int(w[0]) != FILE_MAGIC or int(w[1]) != 1:
+        raise ValueError("not a tree file")
+    nl = int(w[2])
+    grid = {"dims": [int(d) for d in w[3:3 + nl]],
+            "low": [float(v) for v in w[11:14].view(np.float32)],
+            "world_size": float(w[14:15].view(np.float32)[0]),
+            "tape_time": float(w[15:16].view(np.float32)[0])}
+    nc = int(w[17])
+    consts = w[FILE_HEADER:FILE_HEADER + nc].view(np.float32).copy()
+    return grid, consts, int(w[16]), w[FILE_HEADER + nc:].copy()

@@ -160,6 +160,21 @@ frep_status frep_tree_info_get(frep_ctx* ctx, frep_tree_t tree, frep_tree_info* 
 frep_status frep_tree_export(frep_ctx* ctx, frep_tree_t tree, uint32_t* buf, uint64_t cap,
                              uint64_t* n_words);
 frep_status frep_tree_free(frep_ctx* ctx, frep_tree_t tree);
+/* Tree files (a resident tree written out and read back, e.g. to warm-start
+ * a server or keep fixtures).  The reference keeps trees only in device
+ * buffers whose node layout is its interface (vulkan/README.md:22-43,
+ * vulkan/include/engine/voxel_storage.h:63-88); it has no file form.
+ * Layout (u32 words): 'FRPF', 1, n_levels, dims[8], low[3] (f32), world_size
+ * (f32), tape_time (f32), slot_count, n_consts, n_consts f32 constants, then
+ * the canonical blob of frep_tree_export.  frep_tree_save writes it (size in
+ * *n_words; written when buf has room); frep_tree_load checks every field
+ * (grid, tapes, masks, child links, coordinates, canonical order) and
+ * creates a new built tree that renders, exports and re-voxelizes like the
+ * tree it came from. */
+frep_status frep_tree_save(frep_ctx* ctx, frep_tree_t tree, uint32_t* buf, uint64_t cap,
+                           uint64_t* n_words);
+frep_status frep_tree_load(frep_ctx* ctx, const uint32_t* buf, uint64_t n_words,
+                           frep_tree_t* out);
 
 /* ---- render ---- */
 /* Fills right/up/screen_world from forward, initial up and a horizontal fov

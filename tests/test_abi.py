@@ -71,3 +71,18 @@ def test_tape_compiler_reference_rules():
     assert ins[1][1] == 3
     assert encode(3, 1, 2, 3) == 3 | 1 << 8 | 2 << 16 | 3 << 24
     assert t.n == 2 * 14 + 1
+
+
+def test_tree_file_helpers_round_trip(oracle):
+    from frep import tree as T
+    tape = shapes.spheres_union(8, 1)
+    g = shapes.make_grid([8, 4], tape_time=0.25)
+    blob = oracle.Tree(tape, g).canonical()
+    w = T.file_words(g, tape.consts, tape.slot_count, blob)
+    assert int(w[0]) == 0x46505246 and int(w[2]) == 2
+    g2, c2, s2, b2 = T.parse_file(w)
+    assert g2["dims"] == [8, 4] and g2["tape_time"] == 0.25 and g2["world_size"] == 2.0
+    assert s2 == tape.slot_count and np.array_equal(b2, blob)
+    assert np.array_equal(c2, np.asarray(tape.consts, np.float32))
+    levels = T.parse(b2)
+    assert [len(lv.tapes) for lv in levels] == oracle.Tree(tape, g).counts()

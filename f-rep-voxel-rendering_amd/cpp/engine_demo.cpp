@@ -2,12 +2,14 @@
 // (vulkan/src/application.cpp:60-93: Voxelize, then Trace) written against
 // the C++ engine classes of include/frep_engine.hpp.  Voxelizes one tape,
 // traces one frame and writes the canonical tree blob, the rgba8 image and
-// the per-pixel hit ids.
+// the per-pixel hit ids; then traces 4 frames into a 4-layer render target
+// and writes the presented image.
 //
 // usage: engine_demo TAPE.bin d0,d1,... WIDTH HEIGHT OUT_PREFIX
 //   TAPE.bin: u32 n_words, n_words u32 words ([n, instrs]), u32 n_consts,
 //             n_consts f32 constants, u32 slot_count (little endian)
-//   writes OUT_PREFIX.tree (u32 blob), OUT_PREFIX.rgba, OUT_PREFIX.hits
+//   writes OUT_PREFIX.tree (u32 blob), OUT_PREFIX.rgba, OUT_PREFIX.hits,
+//   OUT_PREFIX.present (rgba8)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -90,6 +92,16 @@ int main(int argc, char** argv) {
         raytracer.Trace(&camera, time);
         voxelizer.PrintStats();
 
+        // the reference's frame loop with a layered target: one jittered
+        // sample per frame into the next layer, then the presented mean
+        frep_engine::RenderTarget target;
+        target.Init(&device, width, height, 4);
+        frep_engine::Raytracer layered;
+        layered.Init(&device, &target, &voxels);
+        for (int frame = 0; frame < 4; ++frame) layered.Trace(&camera, time);
+        const std::vector<uint8_t> presented = target.Present();
+        target.Cleanup();
+
         uint64_t n = 0;
         frep_engine::check(device.ctx, frep_tree_export(device.ctx, voxels.treeHandle, nullptr, 0, &n),
                            "frep_tree_export");
@@ -98,7 +110,8 @@ int main(int argc, char** argv) {
                            frep_tree_export(device.ctx, voxels.treeHandle, blob.data(), n, &n),
                            "frep_tree_export");
         if (!write_file(prefix + ".tree", blob) || !write_file(prefix + ".rgba", raytracer.Image()) ||
-            !write_file(prefix + ".hits", raytracer.HitIds())) {
+            !write_file(prefix + ".hits", raytracer.HitIds()) ||
+            !write_file(prefix + ".present", presented)) {
             std::fprintf(stderr, "cannot write outputs under %s\n", prefix.c_str());
             return 1;
         }

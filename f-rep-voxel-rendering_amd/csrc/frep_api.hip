@@ -466,6 +466,8 @@ void frep_shutdown(frep_ctx* ctx) {
         ctx->hpool.free(t->h_counters);
     }
     if (ctx->d_out) (void)hipFree(ctx->d_out);
+    for (auto& tg : ctx->targets)
+        if (tg) ctx->dpool.free(tg->d);
     free_stage(&ctx->stage);
     for (auto& kv : ctx->tune)
         if (kv.second.exec) (void)hipGraphExecDestroy(kv.second.exec);
@@ -1899,6 +1901,90 @@ frep_status frep_render(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam, ui
         ctx->out_bytes = S.total;
     }
     return render_into(ctx, t, cam, (char*)ctx->d_out, S, rgba, hit_ids, normals);
+}
+
+frep_status frep_target_create(frep_ctx* ctx, uint32_t width, uint32_t height, uint32_t layers,
+                               frep_target_t* out) {
+    if (!ctx || !out) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (width == 0 || height == 0 || width > 16384 || height > 16384 || layers == 0 ||
+        layers > 64)
+        return fail(ctx, FREP_INVALID_ARG, "target size: 1..16384 pixels a side, 1..64 layers");
+    HIP_TRY(hipSetDevice(ctx->device));
+    std::unique_ptr<TargetObj> tg(new TargetObj());
+    tg->width = width;
+    tg->height = height;
+    tg->layers = layers;
+    const size_t bytes = (size_t)width * height * layers * 4;
+    HIP_TRY(ctx->dpool.alloc(bytes, &tg->d));
+    HIP_TRY(hipMemsetAsync(tg->d, 0, bytes, ctx->stream));   // RenderTarget::ZeroOutImage
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->targets.push_back(std::move(tg));
+    *out = (frep_target_t)ctx->targets.size();
+    return FREP_OK;
+}
+
+frep_status frep_target_free(frep_ctx* ctx, frep_target_t h) {
+    if (!ctx) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (h == 0 || h > ctx->targets.size() || !ctx->targets[h - 1])
+        return fail(ctx, FREP_INVALID_ARG, "unknown target");
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    ctx->dpool.free(ctx->targets[h - 1]->d);
+    ctx->targets[h - 1].reset();
+    return FREP_OK;
+}
+
+frep_status frep_render_layer(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam,
+                              frep_target_t target, uint32_t layer) {
+    if (!ctx || !cam) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    TreeObj* t = nullptr;
+    {
+        const frep_status s = render_checks(ctx, h, cam, &t);
+        if (s != FREP_OK) return s;
+    }
+    if (target == 0 || target > ctx->targets.size() || !ctx->targets[target - 1])
+        return fail(ctx, FREP_INVALID_ARG, "unknown target");
+    const TargetObj* tg = ctx->targets[target - 1].get();
+    if (cam->width != tg->width || cam->height != tg->height)
+        return fail(ctx, FREP_INVALID_ARG, "camera size differs from the target's");
+    if ((cam->spp ? cam->spp : 1) != 1)
+        return fail(ctx, FREP_INVALID_ARG, "a target layer holds one sample per pixel (spp 1)");
+    if (layer >= tg->layers) return fail(ctx, FREP_INVALID_ARG, "layer out of range");
+    HIP_TRY(hipSetDevice(ctx->device));
+    uint8_t* dst = (uint8_t*)tg->d + (size_t)layer * tg->width * tg->height * 4;
+    const RenderArgs a = render_args(t, cam, dst, nullptr, nullptr);
+    if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 2], ctx->stream));
+    HIP_TRY(launch_render(ctx->stream, a, t->maxs));
+    if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 3], ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if (ctx->timing)
+        HIP_TRY(hipEventElapsedTime(&ctx->last.render_ms, ctx->ev[kMaxLevels + 2],
+                                    ctx->ev[kMaxLevels + 3]));
+    return FREP_OK;
+}
+
+frep_status frep_target_resolve(frep_ctx* ctx, frep_target_t target, uint8_t* rgba) {
+    if (!ctx || !rgba) return FREP_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (target == 0 || target > ctx->targets.size() || !ctx->targets[target - 1])
+        return fail(ctx, FREP_INVALID_ARG, "unknown target");
+    const TargetObj* tg = ctx->targets[target - 1].get();
+    HIP_TRY(hipSetDevice(ctx->device));
+    const size_t px = (size_t)tg->width * tg->height;
+    if (px * 4 + 256 > ctx->out_bytes) {
+        if (ctx->d_out) HIP_TRY(hipFree(ctx->d_out));
+        ctx->d_out = nullptr;
+        ctx->out_bytes = 0;
+        HIP_TRY(hipMalloc(&ctx->d_out, px * 4 + 256));
+        ctx->out_bytes = px * 4 + 256;
+    }
+    HIP_TRY(launch_resolve(ctx->stream, tg->d, (uint32_t)px, tg->layers, ctx->d_out));
+    HIP_TRY(hipMemcpyAsync(rgba, ctx->d_out, px * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return FREP_OK;
 }
 
 frep_status frep_render_noalloc(frep_ctx* ctx, frep_tree_t h, const frep_camera* cam,

@@ -47,6 +47,14 @@ struct TapeObj {
     RootSched root[kRootVariants];
 };
 
+// Layered render target (the reference's RenderTarget image array,
+// vulkan/include/engine/render_target.h:13-19): `layers` rgba8 images of
+// width x height, layer-major, in device memory.
+struct TargetObj {
+    uint32_t width = 0, height = 0, layers = 0;
+    void* d = nullptr;
+};
+
 struct TreeObj {
     frep_grid grid{};
     bool built = false;
@@ -215,6 +223,8 @@ struct frep_ctx {
     void* noalloc_staging = nullptr;
     // staging of the synchronous batched voxelization
     frep::BatchStage stage;
+    // layered render targets
+    std::vector<std::unique_ptr<frep::TargetObj>> targets;
     // tree buffers (device) and small pinned mirrors / request outputs
     frep::BlockPool dpool{false};
     frep::BlockPool hpool{true};

@@ -212,6 +212,9 @@ hipError_t launch_tree_stats(hipStream_t st, const uint32_t* nodes, uint32_t ns,
                              const uint32_t* pool, uint64_t pool_words, bool interval,
                              unsigned long long* out);
 hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs);
+// Mean of n_layers rgba8 layers of `pixels` pixels each into out (rgba8).
+hipError_t launch_resolve(hipStream_t st, const void* layers, uint32_t pixels, uint32_t n_layers,
+                          void* out);
 hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
                               const float* xyz, uint32_t n, float t, float* out, uint32_t maxs,
                               bool asm_ok);

@@ -1862,6 +1862,38 @@ hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs) {
     return hipGetLastError();
 }
 
+// Presentation of a layered render target (vulkan/shaders/renderer/
+// triangle.frag:14-23): the mean over the layers of each pixel's rgb as
+// unorm values, alpha 1, written back as rgba8 rounded to nearest.  Layers
+// are summed in order in float, as the fragment shader's loop does.  One
+// thread per pixel; 4 bytes read per layer and 4 written per pixel.
+__global__ __launch_bounds__(256) void k_resolve(const uchar4* layers, uint32_t pixels,
+                                                 uint32_t n_layers, uchar4* out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= pixels) return;
+    float r = 0.0f, g = 0.0f, b = 0.0f;
+    for (uint32_t l = 0; l < n_layers; ++l) {
+        const uchar4 q = layers[(size_t)l * pixels + i];
+        r = r + (float)q.x / 255.0f;
+        g = g + (float)q.y / 255.0f;
+        b = b + (float)q.z / 255.0f;
+    }
+    const float n = (float)n_layers;
+    auto u8 = [](float v) {
+        v = v < 0.0f ? 0.0f : (v > 1.0f ? 1.0f : v);
+        return (unsigned char)(v * 255.0f + 0.5f);
+    };
+    out[i] = make_uchar4(u8(r / n), u8(g / n), u8(b / n), u8(n / n));
+}
+
+hipError_t launch_resolve(hipStream_t st, const void* layers, uint32_t pixels, uint32_t n_layers,
+                          void* out) {
+    if (pixels == 0) return hipSuccess;
+    k_resolve<<<(pixels + 255) / 256, 256, 0, st>>>((const uchar4*)layers, pixels, n_layers,
+                                                    (uchar4*)out);
+    return hipGetLastError();
+}
+
 hipError_t launch_eval_points(hipStream_t st, const uint32_t* tape, const float* consts,
                               const float* xyz, uint32_t n, float t, float* out, uint32_t maxs,
                               bool asm_ok) {

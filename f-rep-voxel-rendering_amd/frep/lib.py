@@ -25,7 +25,8 @@ EXPORTED = [
     "frep_device_info", "frep_tape_upload", "frep_tape_free", "frep_tree_create",
     "frep_voxelize", "frep_voxelize_batch", "frep_tree_info_get", "frep_tree_export", "frep_tree_free",
     "frep_tree_save", "frep_tree_load",
-    "frep_camera_look", "frep_render", "frep_render_noalloc", "frep_eval_points", "frep_eval_intervals",
+    "frep_camera_look", "frep_render", "frep_render_noalloc", "frep_target_create",
+    "frep_target_free", "frep_render_layer", "frep_target_resolve", "frep_eval_points", "frep_eval_intervals",
     "frep_eval_gradients", "frep_math", "frep_root_schedule", "frep_compile",
     "frep_compile_last_error",
     "frep_server_create", "frep_server_destroy", "frep_server_last_error", "frep_server_scene",
@@ -131,6 +132,11 @@ def load():
         "frep_camera_look": (C.c_int, [C.POINTER(Camera), f32p, C.c_float]),
         "frep_render": (C.c_int, [vp, C.c_uint32, C.POINTER(Camera), u8p, u64p, f32p]),
         "frep_render_noalloc": (C.c_int, [vp, C.c_uint32, C.POINTER(Camera), u8p, u64p, f32p]),
+        "frep_target_create": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, u32p]),
+        "frep_target_free": (C.c_int, [vp, C.c_uint32]),
+        "frep_render_layer": (C.c_int, [vp, C.c_uint32, C.POINTER(Camera), C.c_uint32,
+                                        C.c_uint32]),
+        "frep_target_resolve": (C.c_int, [vp, C.c_uint32, u8p]),
         "frep_eval_points": (C.c_int, [vp, C.c_uint32, f32p, C.c_uint32, C.c_float, f32p]),
         "frep_eval_intervals": (C.c_int, [vp, C.c_uint32, f32p, f32p, C.c_uint32, C.c_float,
                                           f32p, u8p]),
@@ -352,6 +358,26 @@ class Context:
             _p(hits, C.c_uint64) if hits is not None else None,
             _p(nrm, C.c_float) if nrm is not None else None))
         return rgba, hits, nrm
+
+    # ---- layered render target ----
+    def target_create(self, width: int, height: int, layers: int) -> int:
+        out = C.c_uint32()
+        self._check(self.L.frep_target_create(self.h, width, height, layers, C.byref(out)))
+        self._targets = getattr(self, "_targets", {})
+        self._targets[out.value] = (width, height)
+        return out.value
+
+    def target_free(self, target: int):
+        self._check(self.L.frep_target_free(self.h, target))
+
+    def render_layer(self, tree: int, cam: Camera, target: int, layer: int):
+        self._check(self.L.frep_render_layer(self.h, tree, C.byref(cam), target, layer))
+
+    def target_resolve(self, target: int) -> np.ndarray:
+        w, h = self._targets[target]
+        out = np.zeros((h, w, 4), dtype=np.uint8)
+        self._check(self.L.frep_target_resolve(self.h, target, _p(out, C.c_uint8)))
+        return out
 
     # ---- batch evaluation ----
     def eval_points(self, tape: int, xyz, t=0.0) -> np.ndarray:

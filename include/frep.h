@@ -186,6 +186,25 @@ frep_status frep_camera_look(frep_camera* cam, const float initial_up[3], float 
 frep_status frep_render(frep_ctx* ctx, frep_tree_t tree, const frep_camera* cam,
                         uint8_t* rgba, uint64_t* hit_ids, float* normals);
 
+/* ---- layered render target and its presentation ----
+ * The reference renders one jittered sample per frame into layer
+ * (frame % temporalSampleCount) of its RenderTarget, an array of rgba8
+ * images (vulkan/include/engine/render_target.h:13-19,
+ * vulkan/src/engine/raytracer.cpp:159-162), and presents the mean of the
+ * layers (vulkan/shaders/renderer/triangle.frag:14-23).  A target is that
+ * image array on the device, zeroed at creation (RenderTarget::ZeroOutImage).
+ * frep_render_layer renders cam (spp 1, the target's size) into one layer;
+ * frep_target_resolve writes the fragment shader's result as rgba8: per
+ * pixel the mean over the layers of rgb/255, alpha 1, rounded to nearest
+ * (before any swapchain colour conversion, which is outside this library). */
+typedef uint32_t frep_target_t;
+frep_status frep_target_create(frep_ctx* ctx, uint32_t width, uint32_t height, uint32_t layers,
+                               frep_target_t* out);
+frep_status frep_target_free(frep_ctx* ctx, frep_target_t target);
+frep_status frep_render_layer(frep_ctx* ctx, frep_tree_t tree, const frep_camera* cam,
+                              frep_target_t target, uint32_t layer);
+frep_status frep_target_resolve(frep_ctx* ctx, frep_target_t target, uint8_t* rgba);
+
 /* Allocation-free form of frep_render.  No counterpart in the reference: it
  * is for callers that cannot allocate device memory (code under HIP graph
  * capture, tight serving loops, harnesses that forbid hipMalloc).  Same

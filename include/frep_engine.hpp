@@ -7,6 +7,8 @@
  *   Voxelizer     vulkan/include/engine/voxelizer.h:12-24
  *   Raytracer     vulkan/include/engine/raytracer.h:14-28
  *   Camera        vulkan/include/engine/camera.h:6-28, vulkan/src/engine/camera.cpp:16-27
+ *   RenderTarget  vulkan/include/engine/render_target.h:9-30 (+ the presentation
+ *                 pass's layer mean, vulkan/shaders/renderer/triangle.frag:14-23)
  *
  * so that the reference's driver (vulkan/src/application.cpp:60-93)
  *
@@ -207,6 +209,38 @@ class Camera {
     }
 };
 
+// Layered render target (render_target.h:9-30): temporalSampleCount rgba8
+// images; Present() is the fragment shader's mean of the layers
+// (vulkan/shaders/renderer/triangle.frag:14-23), as rgba8.
+class RenderTarget {
+  public:
+    uint32_t width = 0, height = 0;
+    uint32_t temporalSampleCount = 1;
+    frep_target_t handle = 0;
+
+    void Init(Device* device, uint32_t w, uint32_t h, uint32_t samples) {
+        m_device = device;
+        width = w;
+        height = h;
+        temporalSampleCount = samples;
+        check(device->ctx, frep_target_create(device->ctx, w, h, samples, &handle),
+              "frep_target_create");
+    }
+    void Cleanup() {
+        if (handle) frep_target_free(m_device->ctx, handle);
+        handle = 0;
+    }
+    std::vector<uint8_t> Present() const {
+        std::vector<uint8_t> img((size_t)width * height * 4);
+        check(m_device->ctx, frep_target_resolve(m_device->ctx, handle, img.data()),
+              "frep_target_resolve");
+        return img;
+    }
+
+  private:
+    Device* m_device = nullptr;
+};
+
 class Raytracer {
   public:
     uint32_t samplesPerPixel = 1;
@@ -220,6 +254,14 @@ class Raytracer {
         m_voxels = voxels;
         m_width = width;
         m_height = height;
+    }
+    // The reference's form (raytracer.h:18-21): every Trace writes one
+    // jittered sample per pixel into the target's next layer
+    // (raytracer.cpp:159-162), the jitter seeded by the frame count.
+    void Init(Device* device, RenderTarget* target, VoxelStorage* voxels) {
+        Init(device, target->width, target->height, voxels);
+        m_target = target;
+        m_targetImgLayer = 0;
     }
     void Cleanup() { m_voxels = nullptr; }
 
@@ -238,9 +280,9 @@ class Raytracer {
         }
         cam.width = m_width;
         cam.height = m_height;
-        cam.spp = samplesPerPixel;
-        cam.jitter = jitter ? 1u : 0u;
-        cam.seed = seed;
+        cam.spp = m_target ? 1u : samplesPerPixel;
+        cam.jitter = (jitter || m_target) ? 1u : 0u;
+        cam.seed = m_target ? seed + m_frame : seed;
         for (int k = 0; k < 4; ++k) cam.background[k] = m_background[k];
         // the reference's lights (raytracer.cpp:187-192) and ambient term
         const float l0[3] = {-1.0f, -1.0f, -0.2f}, l1[3] = {1.0f, -1.0f, -0.2f};
@@ -252,6 +294,14 @@ class Raytracer {
         for (int k = 0; k < 4; ++k) cam.ambient[k] = k < 3 ? 0.2f : 1.0f;
         const float up0[3] = {camera->initialUp[0], camera->initialUp[1], camera->initialUp[2]};
         check(c, frep_camera_look(&cam, up0, camera->fovDeg), "frep_camera_look");
+        if (m_target) {
+            check(c, frep_render_layer(c, m_voxels->treeHandle, &cam, m_target->handle,
+                                       m_targetImgLayer),
+                  "frep_render_layer");
+            m_targetImgLayer = (m_targetImgLayer + 1) % m_target->temporalSampleCount;
+            ++m_frame;
+            return;
+        }
         const size_t px = (size_t)m_width * m_height;
         m_image.resize(px * 4);
         m_hits.resize(keepHitIds ? px * cam.spp : 0);
@@ -274,6 +324,8 @@ class Raytracer {
     }
     Device* m_device = nullptr;
     VoxelStorage* m_voxels = nullptr;
+    RenderTarget* m_target = nullptr;
+    uint32_t m_targetImgLayer = 0, m_frame = 0;
     uint32_t m_width = 0, m_height = 0;
     std::array<float, 4> m_background{0.0f, 0.0f, 0.0f, 1.0f};
     std::vector<uint8_t> m_image;

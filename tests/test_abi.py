@@ -86,3 +86,13 @@ def test_tree_file_helpers_round_trip(oracle):
     assert np.array_equal(c2, np.asarray(tape.consts, np.float32))
     levels = T.parse(b2)
     assert [len(lv.tapes) for lv in levels] == oracle.Tree(tape, g).counts()
+
+
+def test_png_round_trip(tmp_path):
+    from frep import image
+    a = np.random.default_rng(3).integers(0, 256, (9, 13, 4)).astype(np.uint8)
+    image.write_png(tmp_path / "a.png", a)
+    assert np.array_equal(image.read_png(tmp_path / "a.png"), a)
+    image.write_ppm(tmp_path / "a.ppm", a)
+    data = open(tmp_path / "a.ppm", "rb").read()
+    assert data.startswith(b"P6 13 9 255\n") and len(data) == len(b"P6 13 9 255\n") + 13 * 9 * 3

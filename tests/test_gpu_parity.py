@@ -781,4 +781,13 @@ def test_cpp_engine_classes_match_the_c_abi(ctx, tmp_path):
     assert np.array_equal(got_tree, ctx.export(tree))
     assert np.array_equal(got_hits, hits)
     assert np.array_equal(got_rgba, rgba)
+    # RenderTarget + layered Raytracer: frames 0..3 jittered with seeds 0..3
+    got_present = np.fromfile(prefix + ".present", dtype=np.uint8).reshape(h, w, 4)
+    tg = ctx.target_create(w, h, 4)
+    cam.jitter = 1
+    for frame in range(4):
+        cam.seed = frame
+        ctx.render_layer(tree, cam, tg, frame)
+    assert np.array_equal(got_present, ctx.target_resolve(tg))
+    ctx.target_free(tg)
     ctx.tree_free(tree)

@@ -58,7 +58,10 @@ struct Scene {
     uint32_t pins = 0;             // in-flight requests reading the tree
     uint64_t bytes = 0;            // device bytes charged to the budget
     uint64_t last_use = 0;
+    float built_time = 0.0f;       // tape time the resident tree was built at
 };
+
+inline bool same_time(float a, float b) { return std::memcmp(&a, &b, 4) == 0; }
 
 // Output layout of one request: rgba, hit ids, normals at 256-byte aligned
 // offsets of one block (the same layout in the slot's device buffer, at
@@ -268,19 +271,29 @@ frep_status enqueue_renders(frep_server* srv, Slot& S, frep_scene_t only) {
     return FREP_OK;
 }
 
-// Admission: moves requests from the queue into slot S (FIFO, skipping those
-// whose scene a batch in flight is building, and those whose scene does not
-// fit the budget yet).  Caller holds srv->mu.  Returns the number admitted.
+// Admission: moves requests from the queue into slot S in FIFO order.  A
+// request renders its scene's tree at its tape time (its own with
+// FREP_REQUEST_AT_TIME, else the scene's): it joins a build of that time in
+// this batch, or renders the resident tree if it was built at that time, or
+// asks for a build -- which must wait while a batch in flight builds or
+// reads the tree, or while the tree does not fit the budget.  A request left
+// waiting holds back the later requests of its scene, so each scene's
+// requests are admitted in submission order.  Caller holds srv->mu.
+// Returns the number admitted.
 size_t admit(frep_server* srv, Slot& S) {
     frep_ctx* ctx = srv->ctx;
     S.build.clear();
     S.jobs.clear();
     S.reqs.clear();
     S.out_need = 0;
-    auto ours = [&](frep_scene_t sid) {
+    auto job_of = [&](frep_scene_t sid) -> const SlotJob* {
         for (const SlotJob& j : S.build)
-            if (j.scene == sid) return true;
-        return false;
+            if (j.scene == sid) return &j;
+        return nullptr;
+    };
+    std::vector<frep_scene_t> held;      // scenes with an earlier request left waiting
+    auto is_held = [&](frep_scene_t sid) {
+        return std::find(held.begin(), held.end(), sid) != held.end();
     };
     for (auto it = srv->queue.begin(); it != srv->queue.end();) {
         if (S.reqs.size() >= srv->cfg.max_batch) break;
@@ -288,27 +301,55 @@ size_t admit(frep_server* srv, Slot& S) {
         Ticket& tk = srv->tickets[tid];
         const frep_scene_t sid = tk.req.scene;
         Scene& sc = srv->scenes[sid - 1];
-        const bool mine = ours(sid);
-        if (sc.building && !mine) {   // being built by a batch in flight
+        if (is_held(sid)) {
             ++it;
             continue;
         }
-        if (!sc.resident && !mine) {
+        const float te = (tk.req.flags & FREP_REQUEST_AT_TIME) ? tk.req.tape_time
+                                                               : sc.grid.tape_time;
+        const SlotJob* mj = job_of(sid);
+        bool build = false;
+        if (mj) {
+            if (!same_time(mj->grid.tape_time, te)) {   // another time: a later batch
+                held.push_back(sid);
+                ++it;
+                continue;
+            }
+        } else if (sc.building) {                        // being built by a batch in flight
+            held.push_back(sid);
+            ++it;
+            continue;
+        } else if (!(sc.resident && same_time(sc.built_time, te))) {
+            if (sc.resident && sc.pins > 0) {            // requests in flight read the tree
+                held.push_back(sid);
+                ++it;
+                continue;
+            }
+            build = true;
+        }
+        if (build) {
             SlotJob j{};
             j.scene = sid;
             j.tape = sc.tape;
             j.grid = sc.grid;
+            j.grid.tape_time = te;
             auto h = srv->hints.find(dims_key(sc.grid));
             j.has_hint = h != srv->hints.end();
             if (j.has_hint) j.hint = h->second;
             {
                 std::lock_guard<std::mutex> lk(ctx->mu);
-                j.est = estimate_job_bytes(get_tape(ctx, sc.tape), &sc.grid,
+                j.est = estimate_job_bytes(get_tape(ctx, sc.tape), &j.grid,
                                            j.has_hint ? &j.hint : nullptr);
             }
-            while (srv->resident_bytes + j.est > srv->cfg.device_budget_bytes && evict_one(srv)) {
+            // a resident tree rebuilt at another time gives back its bytes;
+            // marked as building so that eviction leaves it alone
+            const uint64_t credit = sc.resident ? std::min(sc.bytes, srv->resident_bytes) : 0;
+            sc.building = true;
+            while (srv->resident_bytes - credit + j.est > srv->cfg.device_budget_bytes &&
+                   evict_one(srv)) {
             }
-            if (srv->resident_bytes + j.est > srv->cfg.device_budget_bytes) {
+            if (srv->resident_bytes - credit + j.est > srv->cfg.device_budget_bytes) {
+                sc.building = false;
                 // does not fit now; if nothing is in flight it never will
                 bool any_busy = false;
                 for (uint32_t k = 0; k < srv->n_slots; ++k) any_busy = any_busy || srv->slots[k].busy;
@@ -319,12 +360,19 @@ size_t admit(frep_server* srv, Slot& S) {
                     srv->done_cv.notify_all();
                     continue;
                 }
+                held.push_back(sid);
                 ++it;
                 continue;
+            }
+            if (sc.resident) {
+                srv->resident_bytes -= credit;
+                sc.bytes = 0;
+                sc.resident = false;
             }
             if (!sc.tree) {
                 std::lock_guard<std::mutex> lk(ctx->mu);
                 if (new_tree(ctx, &sc.tree) != FREP_OK) {
+                    sc.building = false;
                     complete(srv, tid, FREP_OUT_OF_MEMORY, 0, 0);
                     it = srv->queue.erase(it);
                     srv->done_cv.notify_all();
@@ -484,6 +532,7 @@ void finalize(frep_server* srv, Slot& S) {
         }
         t->built = true;
         sc.resident = true;
+        sc.built_time = j.grid.tape_time;
         {
             std::lock_guard<std::mutex> lk(ctx->mu);
             sc.bytes = tree_device_bytes(t);

@@ -78,9 +78,13 @@ class ServerConfig(C.Structure):
 
 
 class Request(C.Structure):
-    _fields_ = [("scene", C.c_uint32), ("pad", C.c_uint32), ("camera", Camera),
+    _fields_ = [("scene", C.c_uint32), ("flags", C.c_uint32), ("camera", Camera),
+                ("tape_time", C.c_float),
                 ("rgba", C.POINTER(C.c_uint8)), ("hit_ids", C.POINTER(C.c_uint64)),
                 ("normals", C.POINTER(C.c_float))]
+
+
+REQUEST_AT_TIME = 1
 
 
 class Result(C.Structure):
@@ -477,7 +481,9 @@ class Server:
         self._check(self.L.frep_server_scene(self.h, tape, C.byref(g), C.byref(out)))
         return out.value
 
-    def submit(self, scene: int, cam: Camera, want=("rgba",)) -> int:
+    def submit(self, scene: int, cam: Camera, want=("rgba",), tape_time=None) -> int:
+        """Queues a render of `scene`; with tape_time, of its tree built at
+        that time (FREP_REQUEST_AT_TIME)."""
         spp = max(1, cam.spp)
         npx = cam.width * cam.height
         rgba = np.zeros((cam.height, cam.width, 4), dtype=np.uint8) if "rgba" in want else None
@@ -486,6 +492,9 @@ class Server:
         req = Request()
         req.scene = scene
         req.camera = cam
+        if tape_time is not None:
+            req.flags = REQUEST_AT_TIME
+            req.tape_time = tape_time
         req.rgba = _p(rgba, C.c_uint8) if rgba is not None else None
         req.hit_ids = _p(hits, C.c_uint64) if hits is not None else None
         req.normals = _p(nrm, C.c_float) if nrm is not None else None

@@ -312,7 +312,10 @@ frep_status frep_root_schedule(const uint32_t* words, uint32_t n_words, uint32_t
  * batches overlap on the device.  Scenes are built only while the resident
  * trees fit the device budget; idle trees are evicted least recently used
  * first and rebuilt on demand.  Each request's outputs are what
- * frep_voxelize + frep_render give for its scene and camera.  The tapes of
+ * frep_voxelize + frep_render give for its scene (at its tape time) and
+ * camera.  A request at a tape time other than the one the scene's resident
+ * tree was built at rebuilds the tree once no request in flight reads it;
+ * the requests of one scene are admitted in submission order.  The tapes of
  * registered scenes must outlive the server. */
 typedef struct frep_server frep_server;
 typedef uint32_t frep_scene_t;
@@ -323,10 +326,17 @@ typedef struct frep_server_config {
     uint32_t max_inflight;          /* batches on the device at once, 1..4; 0 = 2 */
 } frep_server_config;
 
+/* Request flag: render the scene's tree built at the request's tape_time
+ * (the reference re-voxelizes every frame at the current time when
+ * voxelizeRealTime is set, vulkan/src/application.cpp:68-82).  Without it
+ * a request uses the tape time the scene was registered with. */
+#define FREP_REQUEST_AT_TIME 1u
+
 typedef struct frep_request {
     frep_scene_t scene;
-    uint32_t pad;
+    uint32_t flags;                 /* FREP_REQUEST_AT_TIME or 0 */
     frep_camera camera;
+    float tape_time;                /* used with FREP_REQUEST_AT_TIME */
     uint8_t* rgba;                  /* host outputs as in frep_render (any may be */
     uint64_t* hit_ids;              /* NULL); written before the request completes */
     float* normals;

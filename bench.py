@@ -36,7 +36,8 @@ COUNTER_PROFILE = os.path.join(ROOT, "profiles", "voxelize_csg24_1024_valu_count
 WORKLOAD_TEXT = {
     "csg24_1024": "random CSG of 24 primitives (seed 2)",
     "spheres8_256": "union of 8 spheres (seed 1)",
-    "gyroid_4096": "gyroid lattice (k = 20) thickened, clipped by a box, smooth-unioned with 32 spheres",
+    "gyroid_4096": "gyroid lattice (k = 20) thickened, clipped by a box, smooth-unioned with "
+                   "416 spheres (balanced smooth-union tree)",
 }
 
 

@@ -479,6 +479,56 @@ inline uint32_t tape_n_mm(const uint32_t* tape) {
 }
 }  // namespace
 
+namespace {
+/* One node of level L: its leaf mask, and for inner levels its interior mask
+ * and, in cell order, the children of its ambiguous cells (coords + the
+ * shortened tape).  child[] is left unset: the caller numbers children. */
+void eval_node(orc_node& nd, uint32_t L, const orc_grid* grid, const float* consts,
+               const std::vector<float>& cell_size, std::vector<orc_node>* kids,
+               std::vector<uint32_t>* kid_cells) {
+    const uint32_t dim = grid->dims[L];
+    const uint32_t cells = dim * dim * dim;
+    const float cs = cell_size[L];
+    const bool last = (L + 1 == grid->n_levels);
+    nd.leaf.assign(cells / 32, 0);
+    if (!last) {
+        nd.interior.assign(cells / 32, 0);
+        nd.child.assign(cells, 0xFFFFFFFFu);
+    }
+    const uint32_t n_mm = tape_n_mm(nd.tape.data());
+    std::vector<uint8_t> ch(n_mm + 1);
+    std::vector<uint32_t> st(nd.tape.size());
+    for (uint32_t idx = 0; idx < cells; ++idx) {
+        const uint32_t cx = idx / (dim * dim), cy = (idx / dim) % dim, cz = idx % dim;
+        const uint32_t gc[3] = {nd.coords[0] * dim + cx, nd.coords[1] * dim + cy,
+                                nd.coords[2] * dim + cz};
+        const float pos[3] = {grid->low[0] + (float)gc[0] * cs,
+                              grid->low[1] + (float)gc[1] * cs,
+                              grid->low[2] + (float)gc[2] * cs};
+        const uint32_t q = idx >> 5, bit = 1u << (idx & 31u);
+        if (last) {
+            const float d = orc_eval_point(nd.tape.data(), consts, pos[0], pos[1],
+                                           pos[2], grid->tape_time);
+            if (d < 0.0f) nd.leaf[q] |= bit;
+            continue;
+        }
+        const float hi[3] = {pos[0] + cs, pos[1] + cs, pos[2] + cs};
+        float itv[2];
+        orc_eval_interval(nd.tape.data(), consts, pos, hi, grid->tape_time, itv, ch.data());
+        if (itv[0] > 0) continue;                   /* empty */
+        if (itv[1] < 0) { nd.leaf[q] |= bit; continue; }   /* full */
+        nd.interior[q] |= bit;
+        orc_node c;
+        c.coords[0] = gc[0]; c.coords[1] = gc[1]; c.coords[2] = gc[2];
+        int reused = 0;
+        const uint32_t k = orc_shorten(nd.tape.data(), ch.data(), st.data(), &reused);
+        c.tape.assign(st.begin(), st.begin() + k + 1);
+        kids->push_back(std::move(c));
+        kid_cells->push_back(idx);
+    }
+}
+}  // namespace
+
 extern "C" orc_tree* orc_voxelize(const uint32_t* tape, const float* consts,
                                   const orc_grid* grid) {
     orc_tree* tr = new orc_tree();
@@ -497,53 +547,17 @@ extern "C" orc_tree* orc_voxelize(const uint32_t* tape, const float* consts,
     root.tape.assign(tape, tape + tape[0] + 1);
     tr->levels[0].push_back(std::move(root));
     for (uint32_t L = 0; L < nl; ++L) {
-        const uint32_t dim = grid->dims[L];
-        const uint32_t cells = dim * dim * dim;
-        const float cs = tr->cell_size[L];
-        const bool last = (L + 1 == nl);
         std::vector<orc_node>& nodes = tr->levels[L];
         std::vector<orc_node> next;
+        std::vector<uint32_t> cells_of;
         for (size_t j = 0; j < nodes.size(); ++j) {
-            orc_node& nd = nodes[j];
-            nd.leaf.assign(cells / 32, 0);
-            if (!last) {
-                nd.interior.assign(cells / 32, 0);
-                nd.child.assign(cells, 0xFFFFFFFFu);
-            }
-            const uint32_t n_mm = tape_n_mm(nd.tape.data());
-            std::vector<uint8_t> ch(n_mm + 1);
-            std::vector<uint32_t> st(nd.tape.size());
-            for (uint32_t idx = 0; idx < cells; ++idx) {
-                const uint32_t cx = idx / (dim * dim), cy = (idx / dim) % dim, cz = idx % dim;
-                const uint32_t gc[3] = {nd.coords[0] * dim + cx, nd.coords[1] * dim + cy,
-                                        nd.coords[2] * dim + cz};
-                const float pos[3] = {grid->low[0] + (float)gc[0] * cs,
-                                      grid->low[1] + (float)gc[1] * cs,
-                                      grid->low[2] + (float)gc[2] * cs};
-                const uint32_t q = idx >> 5, bit = 1u << (idx & 31u);
-                if (last) {
-                    const float d = orc_eval_point(nd.tape.data(), consts, pos[0], pos[1],
-                                                   pos[2], grid->tape_time);
-                    if (d < 0.0f) nd.leaf[q] |= bit;
-                    continue;
-                }
-                const float hi[3] = {pos[0] + cs, pos[1] + cs, pos[2] + cs};
-                float itv[2];
-                orc_eval_interval(nd.tape.data(), consts, pos, hi, grid->tape_time, itv,
-                                  ch.data());
-                if (itv[0] > 0) continue;                   /* empty */
-                if (itv[1] < 0) { nd.leaf[q] |= bit; continue; }   /* full */
-                nd.interior[q] |= bit;
-                orc_node c;
-                c.coords[0] = gc[0]; c.coords[1] = gc[1]; c.coords[2] = gc[2];
-                int reused = 0;
-                const uint32_t k = orc_shorten(nd.tape.data(), ch.data(), st.data(), &reused);
-                c.tape.assign(st.begin(), st.begin() + k + 1);
-                nd.child[idx] = (uint32_t)next.size();
-                next.push_back(std::move(c));
-            }
+            const size_t first = next.size();
+            cells_of.clear();
+            eval_node(nodes[j], L, grid, consts, tr->cell_size, &next, &cells_of);
+            for (size_t c = 0; c < cells_of.size(); ++c)
+                nodes[j].child[cells_of[c]] = (uint32_t)(first + c);
         }
-        if (!last) tr->levels[L + 1] = std::move(next);
+        if (L + 1 < nl) tr->levels[L + 1] = std::move(next);
     }
     return tr;
 }
@@ -595,6 +609,264 @@ extern "C" uint64_t orc_tree_canonical(const orc_tree* tree, uint32_t* buf, uint
     }
     if (buf && cap >= v.size()) std::memcpy(buf, v.data(), v.size() * 4);
     return v.size();
+}
+
+/* ---- trees too large to build whole on the CPU -----------------------
+ *
+ * A 4096^3 tree holds millions of last-level nodes; the oracle evaluates its
+ * first levels in full and a sample of the rest.  Blob records are located by
+ * one walk over the blob (records are variable-length: k tape words). */
+
+namespace {
+struct BlobIndex {
+    uint32_t nl = 0;
+    std::vector<uint32_t> dims, counts;
+    std::vector<std::vector<uint64_t>> rec;    /* word offset of each record */
+};
+
+/* Returns false if the blob is not a well-formed canonical tree blob. */
+bool index_blob(const uint32_t* b, uint64_t n, BlobIndex* ix) {
+    if (n < 3 || b[0] != 0x54505246u || b[1] != 1u) return false;
+    const uint32_t nl = b[2];
+    if (nl < 1 || nl > 8 || n < 3 + 2ull * nl) return false;
+    ix->nl = nl;
+    ix->dims.assign(b + 3, b + 3 + nl);
+    ix->counts.assign(b + 3 + nl, b + 3 + 2 * nl);
+    ix->rec.assign(nl, {});
+    uint64_t pos = 3 + 2ull * nl;
+    for (uint32_t L = 0; L < nl; ++L) {
+        const uint64_t cells = (uint64_t)ix->dims[L] * ix->dims[L] * ix->dims[L];
+        const bool last = (L + 1 == nl);
+        ix->rec[L].resize(ix->counts[L]);
+        for (uint32_t j = 0; j < ix->counts[L]; ++j) {
+            if (pos + 4 > n) return false;
+            ix->rec[L][j] = pos;
+            pos += 4 + (uint64_t)b[pos + 3];
+            pos += cells / 32 + (last ? 0 : cells / 32 + cells);
+            if (pos > n) return false;
+        }
+    }
+    return pos == n;
+}
+
+bool coords_less(const orc_node& a, const orc_node& b) {
+    if (a.coords[0] != b.coords[0]) return a.coords[0] < b.coords[0];
+    if (a.coords[1] != b.coords[1]) return a.coords[1] < b.coords[1];
+    return a.coords[2] < b.coords[2];
+}
+
+template <class F>
+void parallel_for(size_t n, uint32_t threads, F fn) {
+    const size_t T = std::max<size_t>(1, std::min<size_t>(threads ? threads : 1, n));
+    if (T == 1) {
+        for (size_t i = 0; i < n; ++i) fn(i);
+        return;
+    }
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < T; ++t)
+        pool.emplace_back([&, t] {
+            for (size_t i = t; i < n; i += T) fn(i);
+        });
+    for (auto& th : pool) th.join();
+}
+}  // namespace
+
+extern "C" int64_t orc_check_blob(const uint32_t* tape, const float* consts,
+                                  const orc_grid* grid, const uint32_t* blob,
+                                  uint64_t n_words, uint32_t n_full, uint32_t sample_every,
+                                  uint32_t dense_every, uint32_t threads, uint64_t* report) {
+    for (int i = 0; i < 24; ++i) report[i] = 0;
+    report[4] = report[5] = ~0ull;
+    uint64_t bad = 0;
+    auto fail = [&](uint32_t L, uint64_t r, uint64_t kind) {
+        if (bad == 0) { report[4] = L; report[5] = r; report[6] = kind; }
+        ++bad;
+    };
+    BlobIndex ix;
+    if (!index_blob(blob, n_words, &ix)) { report[6] = 1; return -1; }
+    const uint32_t nl = grid->n_levels;
+    if (ix.nl != nl) { report[6] = 1; return -1; }
+    for (uint32_t L = 0; L < nl; ++L)
+        if (ix.dims[L] != grid->dims[L]) { report[6] = 1; return -1; }
+    const std::vector<float> cs = cell_sizes(*grid);
+    if (sample_every == 0) sample_every = 1;
+    if (dense_every == 0) dense_every = 1;
+    auto coords_at = [&](uint32_t L, uint64_t r) { return blob + ix.rec[L][r]; };
+    /* record of level L with the given coords (records are in coords order) */
+    auto find = [&](uint32_t L, const uint32_t* c) -> uint64_t {
+        uint64_t lo = 0, hi = ix.rec[L].size();
+        while (lo < hi) {
+            const uint64_t mid = (lo + hi) / 2;
+            const uint32_t* m = coords_at(L, mid);
+            const bool less = m[0] != c[0] ? m[0] < c[0] : m[1] != c[1] ? m[1] < c[1] : m[2] < c[2];
+            if (less) lo = mid + 1; else hi = mid;
+        }
+        if (lo < ix.rec[L].size()) {
+            const uint32_t* m = coords_at(L, lo);
+            if (m[0] == c[0] && m[1] == c[1] && m[2] == c[2]) return lo;
+        }
+        return ~0ull;
+    };
+
+    /* `level` holds oracle nodes of level L (coords + tape): every node of
+     * the level when `complete` (then in canonical order), else the
+     * children of the nodes evaluated at level L-1 */
+    std::vector<orc_node> level(1);
+    level[0].coords[0] = level[0].coords[1] = level[0].coords[2] = 0;
+    level[0].tape.assign(tape, tape + tape[0] + 1);
+    bool complete = true;
+    uint64_t dense_seen = 0;
+    for (uint32_t L = 0; L < nl && !level.empty(); ++L) {
+        const bool last = (L + 1 == nl);
+        const bool full = (L < n_full);
+        const uint32_t dim = grid->dims[L];
+        const uint64_t cells = (uint64_t)dim * dim * dim, W = cells / 32;
+        std::vector<uint64_t> rec(level.size());
+        if (complete) {
+            report[8 + L] = level.size();
+            if (ix.counts[L] != level.size()) { fail(L, ~0ull, 2); break; }
+            for (size_t j = 0; j < level.size(); ++j) rec[j] = j;
+        } else {
+            for (size_t j = 0; j < level.size(); ++j) rec[j] = find(L, level[j].coords);
+        }
+        /* coords and tape of every node */
+        std::vector<char> ok(level.size(), 1);
+        for (size_t j = 0; j < level.size(); ++j) {
+            const orc_node& nd = level[j];
+            if (rec[j] == ~0ull) { fail(L, j, 3); ok[j] = 0; continue; }
+            const uint32_t* p = coords_at(L, rec[j]);
+            if (p[0] != nd.coords[0] || p[1] != nd.coords[1] || p[2] != nd.coords[2]) {
+                fail(L, rec[j], 3);
+                ok[j] = 0;
+                continue;
+            }
+            const uint32_t k = p[3];
+            if (k + 1 != nd.tape.size() || std::memcmp(p + 3, nd.tape.data(), 4ull * (k + 1)) != 0) {
+                fail(L, rec[j], 4);
+                ok[j] = 0;
+            }
+        }
+        /* which nodes are evaluated */
+        std::vector<size_t> todo;
+        for (size_t j = 0; j < level.size(); ++j)
+            if (ok[j] && (full || j % sample_every == 0)) todo.push_back(j);
+        std::vector<std::vector<orc_node>> kids(todo.size());
+        std::vector<std::vector<uint32_t>> kid_cells(todo.size());
+        std::vector<uint64_t> dense_bad(todo.size(), 0);
+        std::vector<char> dense(todo.size(), 0);
+        if (last && !full)
+            for (size_t t = 0; t < todo.size(); ++t) dense[t] = (dense_seen++ % dense_every) == 0;
+        parallel_for(todo.size(), threads, [&](size_t t) {
+            orc_node& nd = level[todo[t]];
+            eval_node(nd, L, grid, consts, cs, &kids[t], &kid_cells[t]);
+            if (!dense[t]) return;
+            /* the leaf bits against point evaluation of the full tape */
+            for (uint32_t idx = 0; idx < cells; ++idx) {
+                const uint32_t cx = idx / (dim * dim), cy = (idx / dim) % dim, cz = idx % dim;
+                const float px = grid->low[0] + (float)(nd.coords[0] * dim + cx) * cs[L];
+                const float py = grid->low[1] + (float)(nd.coords[1] * dim + cy) * cs[L];
+                const float pz = grid->low[2] + (float)(nd.coords[2] * dim + cz) * cs[L];
+                const bool in = orc_eval_point(tape, consts, px, py, pz, grid->tape_time) < 0.0f;
+                dense_bad[t] += in != (((nd.leaf[idx >> 5] >> (idx & 31u)) & 1u) != 0);
+            }
+        });
+        for (size_t t = 0; t < todo.size(); ++t) {
+            report[3] += dense_bad[t];
+            if (dense[t]) report[2] += cells;
+        }
+        report[full ? 0 : 1] += todo.size();
+        /* the next level: every child (in canonical order) below a full
+         * level, the children of the evaluated nodes below a sampled one */
+        std::vector<orc_node> next;
+        if (!last) {
+            std::vector<std::pair<size_t, uint32_t>> src;   /* (todo index, cell) */
+            for (size_t t = 0; t < todo.size(); ++t)
+                for (size_t c = 0; c < kids[t].size(); ++c) {
+                    next.push_back(std::move(kids[t][c]));
+                    src.emplace_back(t, kid_cells[t][c]);
+                }
+            std::vector<uint32_t> order(next.size());
+            for (uint32_t i = 0; i < order.size(); ++i) order[i] = i;
+            std::sort(order.begin(), order.end(),
+                      [&](uint32_t a, uint32_t b) { return coords_less(next[a], next[b]); });
+            std::vector<orc_node> sorted(next.size());
+            for (uint32_t r = 0; r < order.size(); ++r) {
+                sorted[r] = std::move(next[order[r]]);
+                const auto& sc = src[order[r]];
+                /* a full level knows every child's rank; below a sampled
+                 * level the rank is read from the blob and its record's
+                 * coords are checked at the next level */
+                if (complete && full) level[todo[sc.first]].child[sc.second] = r;
+            }
+            next = std::move(sorted);
+        }
+        /* masks and child entries of the evaluated nodes */
+        for (size_t t = 0; t < todo.size(); ++t) {
+            const size_t j = todo[t];
+            const orc_node& nd = level[j];
+            const uint32_t* p = blob + ix.rec[L][rec[j]] + 4 + blob[ix.rec[L][rec[j]] + 3];
+            if (std::memcmp(p, nd.leaf.data(), 4 * W) != 0) { fail(L, rec[j], 5); continue; }
+            if (last) continue;
+            if (std::memcmp(p + W, nd.interior.data(), 4 * W) != 0) { fail(L, rec[j], 6); continue; }
+            const uint32_t* ch = p + 2 * W;
+            if (complete && full) {
+                if (std::memcmp(ch, nd.child.data(), 4 * cells) != 0) fail(L, rec[j], 7);
+                continue;
+            }
+            /* entries: none where the interior bit is clear; where it is set,
+             * a record of level L+1 with the child's coords */
+            bool good = true;
+            for (uint32_t idx = 0; idx < cells && good; ++idx) {
+                const bool in = (nd.interior[idx >> 5] >> (idx & 31u)) & 1u;
+                if (!in) { good = ch[idx] == 0xFFFFFFFFu; continue; }
+                const uint32_t cx = idx / (dim * dim), cy = (idx / dim) % dim, cz = idx % dim;
+                const uint32_t want[3] = {nd.coords[0] * dim + cx, nd.coords[1] * dim + cy,
+                                          nd.coords[2] * dim + cz};
+                good = ch[idx] < ix.counts[L + 1] &&
+                       std::memcmp(coords_at(L + 1, ch[idx]), want, 12) == 0;
+            }
+            if (!good) fail(L, rec[j], 7);
+        }
+        if (!full) complete = false;
+        level = std::move(next);
+    }
+    report[7] = bad;
+    return (int64_t)(bad + report[3]);
+}
+
+extern "C" orc_tree* orc_tree_from_blob(const uint32_t* blob, uint64_t n_words,
+                                        const float* consts, uint32_t n_consts,
+                                        const orc_grid* grid) {
+    BlobIndex ix;
+    if (!index_blob(blob, n_words, &ix) || ix.nl != grid->n_levels) return nullptr;
+    orc_tree* tr = new orc_tree();
+    tr->grid = *grid;
+    tr->consts.assign(consts, consts + n_consts);
+    tr->cell_size = cell_sizes(*grid);
+    tr->levels.resize(ix.nl);
+    for (uint32_t L = 0; L < ix.nl; ++L) {
+        const uint64_t cells = (uint64_t)ix.dims[L] * ix.dims[L] * ix.dims[L], W = cells / 32;
+        const bool last = (L + 1 == ix.nl);
+        auto& nodes = tr->levels[L];
+        nodes.resize(ix.counts[L]);
+        for (uint32_t j = 0; j < ix.counts[L]; ++j) {
+            const uint32_t* p = blob + ix.rec[L][j];
+            orc_node& nd = nodes[j];
+            nd.coords[0] = p[0]; nd.coords[1] = p[1]; nd.coords[2] = p[2];
+            nd.tape.assign(p + 3, p + 4 + p[3]);
+            p += 4 + p[3];
+            nd.leaf.assign(p, p + W);
+            if (last) continue;
+            nd.interior.assign(p + W, p + 2 * W);
+            nd.child.assign(p + 2 * W, p + 2 * W + cells);
+            for (uint32_t c : nd.child)
+                if (c != 0xFFFFFFFFu && c >= ix.counts[L + 1]) {
+                    delete tr;
+                    return nullptr;
+                }
+        }
+    }
+    return tr;
 }
 
 namespace {
@@ -848,12 +1120,19 @@ extern "C" void orc_camera_basis(orc_camera* cam, const float* initial_up, float
 
 extern "C" void orc_render(const orc_tree* tree, const orc_camera* cam,
                            uint64_t* hit_ids, float* normals, uint8_t* rgba) {
+    orc_render_rows(tree, cam, 0, 1, hit_ids, normals, rgba);
+}
+
+extern "C" void orc_render_rows(const orc_tree* tree, const orc_camera* cam,
+                                uint32_t row_begin, uint32_t row_step,
+                                uint64_t* hit_ids, float* normals, uint8_t* rgba) {
     const F3 pos = F3{cam->pos[0], cam->pos[1], cam->pos[2]};
     const F3 fwd = F3{cam->forward[0], cam->forward[1], cam->forward[2]};
     const F3 right = F3{cam->right[0], cam->right[1], cam->right[2]};
     const F3 up = F3{cam->up[0], cam->up[1], cam->up[2]};
     const uint32_t W = cam->width, H = cam->height, spp = cam->spp ? cam->spp : 1;
-    for (uint32_t y = 0; y < H; ++y)
+    if (row_step == 0) row_step = 1;
+    for (uint32_t y = row_begin; y < H; y += row_step)
         for (uint32_t x = 0; x < W; ++x) {
             float acc[4] = {0, 0, 0, 0};
             for (uint32_t s = 0; s < spp; ++s) {

@@ -127,6 +127,44 @@ typedef struct orc_camera {
 void orc_render(const orc_tree* tree, const orc_camera* cam,
                 uint64_t* hit_ids, float* normals, uint8_t* rgba);
 
+/* orc_render over rows row_begin, row_begin + row_step, ... only (outputs
+ * are full-frame buffers; other rows are left untouched). */
+void orc_render_rows(const orc_tree* tree, const orc_camera* cam,
+                     uint32_t row_begin, uint32_t row_step,
+                     uint64_t* hit_ids, float* normals, uint8_t* rgba);
+
+/* ---- trees too large to build whole on the CPU ---- */
+
+/* Checks a canonical tree blob against the oracle without building the
+ * whole tree.  Levels 0..n_full-1 are evaluated in full (threads workers):
+ * node counts, and every record's coords, tape, leaf and interior masks and
+ * child ranks compared.  Below them the check descends by sampling: at level
+ * n_full every sample_every-th record (canonical order) is evaluated, and at
+ * each deeper level every sample_every-th child of the nodes evaluated one
+ * level up.  For every node the oracle reaches, the blob must hold a record
+ * with its coords and tape (found by coords); for every node evaluated, its
+ * leaf and interior masks must match and each child entry must name a record
+ * with the child's coords.  At the last level, every dense_every-th evaluated
+ * node's leaf bits are also checked against point evaluation of the FULL
+ * tape at each voxel's low corner (dense sampling; independent of
+ * shortening).  report (24 u64): [0] records evaluated at full levels,
+ * [1] records evaluated by sampling, [2] voxels dense-checked, [3] dense
+ * mismatches, [4] level and [5] record of the first mismatch, [6] its kind
+ * (1 malformed, 2 node count, 3 coords, 4 tape, 5 leaf, 6 interior, 7 child
+ * entries), [7] mismatching records, [8 + L] the oracle's node count of
+ * level L for L <= n_full (0 where unknown).  Returns the number of
+ * mismatches (records + dense voxels), or -1 for a malformed blob. */
+int64_t orc_check_blob(const uint32_t* tape, const float* consts, const orc_grid* grid,
+                       const uint32_t* blob, uint64_t n_words, uint32_t n_full,
+                       uint32_t sample_every, uint32_t dense_every, uint32_t threads,
+                       uint64_t* report);
+
+/* A tree read from a canonical blob (e.g. one the GPU library exported), so
+ * that the oracle's raytracer can render it.  NULL if malformed. */
+orc_tree* orc_tree_from_blob(const uint32_t* blob, uint64_t n_words,
+                             const float* consts, uint32_t n_consts,
+                             const orc_grid* grid);
+
 /* Camera basis as the reference computes it (camera.cpp:16-27,
  * raytracer.cpp:177-181): fills right/up/screen_world from forward,
  * initial up and a horizontal fov in degrees. */

@@ -81,6 +81,14 @@ def lib():
                                           C.c_uint32, C.c_int, C.c_int, u8p]
         L.orc_render.restype = None
         L.orc_render.argtypes = [C.c_void_p, C.POINTER(Camera), C.POINTER(C.c_uint64), f32p, u8p]
+        L.orc_render_rows.restype = None
+        L.orc_render_rows.argtypes = [C.c_void_p, C.POINTER(Camera), C.c_uint32, C.c_uint32,
+                                      C.POINTER(C.c_uint64), f32p, u8p]
+        L.orc_check_blob.restype = C.c_int64
+        L.orc_check_blob.argtypes = [u32p, f32p, C.POINTER(Grid), u32p, C.c_uint64, C.c_uint32,
+                                     C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]
+        L.orc_tree_from_blob.restype = C.c_void_p
+        L.orc_tree_from_blob.argtypes = [u32p, C.c_uint64, f32p, C.c_uint32, C.POINTER(Grid)]
         _lib = L
     return _lib
 
@@ -169,12 +177,21 @@ def eval_gradient(tape, x, y, z, t=0.0):
 class Tree:
     """Owning wrapper of an oracle tree."""
 
-    def __init__(self, tape, grid: dict):
+    def __init__(self, tape, grid: dict, blob=None):
+        """The oracle's voxelization of `tape` over `grid`, or with `blob` the
+        tree a canonical blob describes (for the oracle's raytracer)."""
         self._w, wp = _u32(tape.words)
         self._c, cp = _f32(tape.consts)
         self.grid = grid
         self._g = make_grid(grid)
-        self.h = lib().orc_voxelize(wp, cp, C.byref(self._g))
+        if blob is None:
+            self.h = lib().orc_voxelize(wp, cp, C.byref(self._g))
+        else:
+            b, bp = _u32(blob)
+            self.h = lib().orc_tree_from_blob(bp, b.size, cp, len(tape.consts),
+                                              C.byref(self._g))
+            if not self.h:
+                raise ValueError("malformed tree blob")
 
     def __del__(self):
         if getattr(self, "h", None):
@@ -211,6 +228,48 @@ class Tree:
                          normals.ctypes.data_as(C.POINTER(C.c_float)),
                          rgba.ctypes.data_as(C.POINTER(C.c_uint8)))
         return hits, normals.reshape(-1, 3), rgba.reshape(cam.height, cam.width, 4)
+
+
+    def render_rows(self, cam: Camera, row_begin: int, row_step: int):
+        """render() over rows row_begin, row_begin + row_step, ... only;
+        returns full-frame arrays with just those rows written."""
+        n = cam.width * cam.height * max(1, cam.spp)
+        hits = np.zeros(n, dtype=np.uint64)
+        normals = np.zeros(n * 3, dtype=np.float32)
+        rgba = np.zeros(cam.width * cam.height * 4, dtype=np.uint8)
+        lib().orc_render_rows(self.h, C.byref(cam), row_begin, row_step,
+                              hits.ctypes.data_as(C.POINTER(C.c_uint64)),
+                              normals.ctypes.data_as(C.POINTER(C.c_float)),
+                              rgba.ctypes.data_as(C.POINTER(C.c_uint8)))
+        return hits, normals.reshape(-1, 3), rgba.reshape(cam.height, cam.width, 4)
+
+
+REPORT_KINDS = {1: "malformed", 2: "node count", 3: "coords", 4: "tape", 5: "leaf mask",
+                6: "interior mask", 7: "child ranks"}
+
+
+def check_blob(tape, grid: dict, blob, n_full: int, sample_every: int = 1,
+               dense_every: int = 1, threads: int = 1):
+    """orc_check_blob: levels 0..n_full-1 of a canonical blob compared in
+    full with the oracle's, then a sampled descent (every sample_every-th
+    node per level, found in the blob by coords), with every dense_every-th
+    sampled last-level node also checked against dense sampling of the full
+    tape.  Returns (mismatches, report dict)."""
+    w, wp = _u32(tape.words)
+    c, cp = _f32(tape.consts)
+    g = make_grid(grid)
+    b, bp = _u32(blob)
+    rep = (C.c_uint64 * 24)()
+    bad = lib().orc_check_blob(wp, cp, C.byref(g), bp, b.size, n_full, sample_every,
+                               dense_every, threads, rep)
+    nl = len(grid["dims"])
+    r = {"full_records": rep[0], "sampled_records": rep[1], "dense_voxels": rep[2],
+         "dense_mismatches": rep[3], "bad_records": rep[7],
+         "counts": [int(rep[8 + L]) for L in range(nl)]}
+    if rep[6]:
+        r["first"] = {"level": int(rep[4]), "record": int(rep[5]),
+                      "kind": REPORT_KINDS.get(int(rep[6]), int(rep[6]))}
+    return int(bad), r
 
 
 def dense_occupancy(tape, grid: dict, x_begin=0, x_end=None, faithful=False, threads=1):

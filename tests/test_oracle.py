@@ -333,6 +333,101 @@ def test_tree_canonical_blob_parses_and_matches_flatten(oracle):
                 assert levels[L + 1].tapes[c][0] <= levels[L].tapes[j][0]
 
 
+def _record_fields(blob):
+    """Word offsets of each record's fields: {(level, j): (coords, tape,
+    leaf, interior, child)} (interior/child None at the last level)."""
+    nl = int(blob[2])
+    dims = [int(v) for v in blob[3:3 + nl]]
+    counts = [int(v) for v in blob[3 + nl:3 + 2 * nl]]
+    pos, out = 3 + 2 * nl, {}
+    for L in range(nl):
+        cells = dims[L] ** 3
+        last = L == nl - 1
+        for j in range(counts[L]):
+            k = int(blob[pos + 3])
+            leaf = pos + 4 + k
+            out[(L, j)] = (pos, pos + 3, leaf, None if last else leaf + cells // 32,
+                           None if last else leaf + cells // 16)
+            pos = leaf + cells // 32 + (0 if last else cells // 32 + cells)
+    return out
+
+
+@pytest.mark.parametrize("n_full,sample,dense", [(0, 1, 1), (1, 3, 2), (2, 1, 1), (2, 4, 3),
+                                                 (3, 1, 1)])
+def test_check_blob_accepts_the_oracles_tree(oracle, n_full, sample, dense):
+    """The sampled checker that validates 4096^3 trees passes the oracle's
+    own canonical blob at every depth of full evaluation and sampling rate,
+    and reports the oracle's node counts for the levels it evaluated."""
+    tape, grid = shapes.random_csg(24, 2), shapes.make_grid([8, 8, 4])
+    t = oracle.Tree(tape, grid)
+    bad, rep = oracle.check_blob(tape, grid, t.canonical(), n_full, sample, dense, threads=4)
+    assert bad == 0 and rep["bad_records"] == 0 and "first" not in rep
+    known = min(n_full, 2) + 1
+    assert rep["counts"][:known] == t.counts()[:known]
+    if n_full < 3:
+        assert rep["sampled_records"] > 0 and rep["dense_voxels"] > 0
+
+
+def test_check_blob_finds_planted_mistakes(oracle):
+    """One flipped bit in any field of any level is reported, with its kind
+    and record, whether that level is checked in full or by sampling."""
+    tape, grid = shapes.random_csg(24, 2), shapes.make_grid([8, 8, 4])
+    blob = oracle.Tree(tape, grid).canonical()
+    fields = _record_fields(blob)
+    rng = np.random.default_rng(3)
+    kinds = {0: "coords", 1: "tape", 2: "leaf mask", 3: "interior mask", 4: "child ranks"}
+    for n_full in (1, 3):
+        for L in (1, 2):
+            n = int(blob[3 + 3 + L])
+            for f, kind in kinds.items():
+                if fields[(L, 0)][f] is None:
+                    continue
+                j = int(rng.integers(0, n))
+                off = fields[(L, j)][f]
+                if f == 1:
+                    off += 1 + int(rng.integers(0, int(blob[off])))      # an instruction word
+                elif f == 4:
+                    off += int(np.flatnonzero(blob[off:off + grid["dims"][L] ** 3]
+                                              != 0xFFFFFFFF)[0])     # an existing child
+                elif f == 2 or f == 3:
+                    off += int(rng.integers(0, grid["dims"][L] ** 3 // 32))
+                b2 = blob.copy()
+                b2[off] ^= np.uint32(1 << (0 if f in (0, 4) else int(rng.integers(0, 24))))
+                bad, rep = oracle.check_blob(tape, grid, b2, n_full, 1, 1, threads=4)
+                assert bad > 0, (n_full, L, kind)
+                if f == 0:          # also seen from the parent's child entry
+                    continue
+                first = rep["first"]
+                assert first["level"] == L, (n_full, L, kind, first)
+                if not (f == 4 and n_full == 1):
+                    assert first["kind"] == kind and first["record"] == j, (n_full, L, kind, first)
+    # a changed node count, and a truncated blob
+    b2 = blob.copy()
+    b2[3 + 3 + 1] += 1
+    assert oracle.check_blob(tape, grid, b2, 1)[0] != 0
+    assert oracle.check_blob(tape, grid, blob[:-1], 1)[0] == -1
+
+
+def test_tree_from_blob_renders_rows_like_the_tree(oracle):
+    """A tree read back from its canonical blob is the same tree, and the
+    row-subset render equals the full render on those rows."""
+    tape, grid = shapes.random_csg(24, 2), shapes.make_grid([8, 8, 4])
+    t = oracle.Tree(tape, grid)
+    blob = t.canonical()
+    t2 = oracle.Tree(tape, grid, blob=blob)
+    assert np.array_equal(t2.canonical(), blob)
+    cam = oracle.make_camera(shapes.orbit_basis(0.7, 0.3, 3.0, 160, 96), 160, 96, spp=3,
+                             jitter=True, seed=9)
+    h, n, rgba = t.render(cam)
+    h2, n2, rgba2 = t2.render_rows(cam, 2, 5)
+    rows = np.arange(2, 96, 5)
+    assert np.array_equal(h.reshape(96, 160, 3)[rows], h2.reshape(96, 160, 3)[rows])
+    assert np.array_equal(rgba[rows], rgba2[rows])
+    assert (h.reshape(96, 160, 3)[rows] != np.uint64(2 ** 64 - 1)).sum() > 100
+    with pytest.raises(ValueError):
+        oracle.Tree(tape, grid, blob=blob[:-3])
+
+
 def test_node_counts_brute_force(oracle):
     # level-1 node count == number of root cells whose interval is ambiguous
     tape = shapes.spheres_union(8, 1)

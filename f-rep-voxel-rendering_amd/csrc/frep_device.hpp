@@ -41,12 +41,16 @@ __host__ __device__ __forceinline__ float u2f(uint32_t u) { return __builtin_bit
 //   sin / cos: x = k pi/2 + r with a three-part pi/2 whose products with k
 //         are exact for |k| < 4096, degree-9 / degree-10 polynomials.
 // ---------------------------------------------------------------------------
+//
+// Each function computes its main path on an in-range stand-in argument and
+// selects the special-case result at the end, with no data-dependent
+// branch: inside a tape interpreter's loop a divergent branch would make the
+// compiler restructure the whole uniform dispatch around it.
 __host__ __device__ __forceinline__ float dev_expf(float x) {
-    if (x != x) return x;
-    if (x > 88.72283935546875f) return __builtin_inff();
-    if (x < -103.972084045410156f) return 0.0f;
-    const float kf = __builtin_floorf(x * 1.44269502162933349609375f + 0.5f);
-    const float r = (x - kf * 0.693145751953125f) - kf * 1.428606765330187e-06f;
+    const bool big = x > 88.72283935546875f, tiny = x < -103.972084045410156f;
+    const float xs = (x != x || big || tiny) ? 0.0f : x;
+    const float kf = __builtin_floorf(xs * 1.44269502162933349609375f + 0.5f);
+    const float r = (xs - kf * 0.693145751953125f) - kf * 1.428606765330187e-06f;
     float p = 1.98412701138295233250e-04f;
     p = 1.38888892251998186111e-03f + r * p;
     p = 8.33333376795053482056e-03f + r * p;
@@ -60,7 +64,8 @@ __host__ __device__ __forceinline__ float dev_expf(float x) {
     const int k2 = k - k1;
     const float s1 = u2f((uint32_t)(k1 + 127) << 23);
     const float s2 = u2f((uint32_t)(k2 + 127) << 23);
-    return (p * s1) * s2;
+    const float e = (p * s1) * s2;
+    return x != x ? x : (big ? __builtin_inff() : (tiny ? 0.0f : e));
 }
 
 __host__ __device__ __forceinline__ float sin_poly(float r) {
@@ -92,19 +97,25 @@ __host__ __device__ __forceinline__ float reduce_half_pi(float x, int* q) {
 }
 
 __host__ __device__ __forceinline__ float dev_sinf(float x) {
-    if (!(__builtin_fabsf(x) <= 3.0e38f)) return x - x;
+    const bool fin = __builtin_fabsf(x) <= 3.0e38f;
     int q;
-    const float r = reduce_half_pi(x, &q);
+    const float r = reduce_half_pi(fin ? x : 0.0f, &q);
     const float s = sin_poly(r), c = cos_poly(r);
-    return q == 0 ? s : (q == 1 ? c : (q == 2 ? -s : -c));
+    // q = 0..3: s, c, -s, -c (bit selects, which stay branch-free)
+    const float u = (q & 1) ? c : s;
+    const float v = (q & 2) ? -u : u;
+    return fin ? v : x - x;
 }
 
 __host__ __device__ __forceinline__ float dev_cosf(float x) {
-    if (!(__builtin_fabsf(x) <= 3.0e38f)) return x - x;
+    const bool fin = __builtin_fabsf(x) <= 3.0e38f;
     int q;
-    const float r = reduce_half_pi(x, &q);
+    const float r = reduce_half_pi(fin ? x : 0.0f, &q);
     const float s = sin_poly(r), c = cos_poly(r);
-    return q == 0 ? c : (q == 1 ? -s : (q == 2 ? -c : s));
+    // q = 0..3: c, -s, -c, s
+    const float u = (q & 1) ? s : c;
+    const float v = ((q + 1) & 2) ? -u : u;
+    return fin ? v : x - x;
 }
 
 // ---------------------------------------------------------------------------
@@ -305,27 +316,34 @@ __device__ __forceinline__ Itv interval_op(uint32_t op, Itv a, Itv b, float c) {
     Itv r;
     switch (op) {
         case OP_CONST: r = Itv{c, c}; break;
-        case OP_COS:
-            r.lo = contains_multiple(a.lo - kPi, a.hi - kPi, two_pi)
-                       ? -1.0f : fmin_glm(dev_cosf(a.lo), dev_cosf(a.hi));
-            r.hi = contains_multiple(a.lo, a.hi, two_pi)
-                       ? 1.0f : fmax_glm(dev_cosf(a.lo), dev_cosf(a.hi));
+        // values first, then selects: no data-dependent branch (see dev_expf)
+        case OP_COS: {
+            const float cl = dev_cosf(a.lo), ch = dev_cosf(a.hi);
+            const bool m_lo = contains_multiple(a.lo - kPi, a.hi - kPi, two_pi);
+            const bool m_hi = contains_multiple(a.lo, a.hi, two_pi);
+            r.lo = m_lo ? -1.0f : fmin_glm(cl, ch);
+            r.hi = m_hi ? 1.0f : fmax_glm(cl, ch);
             break;
-        case OP_SIN:
-            r.lo = contains_multiple(a.lo + half_pi, a.hi + half_pi, two_pi)
-                       ? -1.0f : fmin_glm(dev_sinf(a.lo), dev_sinf(a.hi));
-            r.hi = contains_multiple(a.lo - half_pi, a.hi - half_pi, two_pi)
-                       ? 1.0f : fmax_glm(dev_sinf(a.lo), dev_sinf(a.hi));
+        }
+        case OP_SIN: {
+            const float sl = dev_sinf(a.lo), sh = dev_sinf(a.hi);
+            const bool m_lo = contains_multiple(a.lo + half_pi, a.hi + half_pi, two_pi);
+            const bool m_hi = contains_multiple(a.lo - half_pi, a.hi - half_pi, two_pi);
+            r.lo = m_lo ? -1.0f : fmin_glm(sl, sh);
+            r.hi = m_hi ? 1.0f : fmax_glm(sl, sh);
             break;
+        }
         case OP_ADD: r = Itv{a.lo + b.lo, a.hi + b.hi}; break;
         case OP_SUB: r = Itv{a.lo - b.hi, a.hi - b.lo}; break;
         case OP_MUL: r = imul(a, b); break;
-        case OP_DIV:
-            if ((b.lo == 0 && b.hi == 0) || (b.lo < 0 && b.hi > 0)) r = Itv{-inf, inf};
-            else if (b.lo == 0) r = imul(a, Itv{1.0f / b.hi, inf});
-            else if (b.hi == 0) r = imul(a, Itv{-inf, 1.0f / b.lo});
-            else r = imul(a, Itv{1.0f / b.hi, 1.0f / b.lo});
+        case OP_DIV: {
+            const bool whole = (b.lo == 0 && b.hi == 0) || (b.lo < 0 && b.hi > 0);
+            const float rh = 1.0f / b.hi, rl = 1.0f / b.lo;
+            const Itv q = b.lo == 0 ? Itv{rh, inf} : (b.hi == 0 ? Itv{-inf, rl} : Itv{rh, rl});
+            const Itv m = imul(a, q);
+            r = whole ? Itv{-inf, inf} : m;
             break;
+        }
         case OP_MIN: r = Itv{fmin_glm(a.lo, b.lo), fmin_glm(a.hi, b.hi)}; break;
         case OP_MAX: r = Itv{fmax_glm(a.lo, b.lo), fmax_glm(a.hi, b.hi)}; break;
         case OP_EXP: r = Itv{dev_expf(a.lo), dev_expf(a.hi)}; break;

@@ -215,66 +215,54 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
             // index); masking keeps every register-file access in range
             const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
             const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu), ib = slot_in<MAXS>(w >> 24);
-            const Itv A{slo[ia], shi[ia]};
-            FREP_FENCE();
-            Itv r{0.0f, 0.0f};
+            // One uniform compare per test, the ops inner levels run most
+            // first (CONST, COPY, ADD/SUB/MUL make up ~90% of their words);
             // op codes: 0 CONST 1 SIN 2 COS 3 ADD 4 SUB 5 MUL 6 DIV 7 MIN
             //           8 MAX 9 EXP 10 NEG 11 SQRT 12 COPY
-            if (op & 8u) {
-                if (op & 4u) {
-                    r = A;
-                } else if (op & 2u) {
-                    if (op & 1u)
-                        r = Itv{__builtin_sqrtf(fmax_glm(0.0f, A.lo)),
-                                __builtin_sqrtf(fmax_glm(0.0f, A.hi))};
-                    else
-                        r = Itv{-A.hi, -A.lo};
-                } else if (op & 1u) {
-                    r = Itv{dev_expf(A.lo), dev_expf(A.hi)};
-                } else {
-                    const Itv B{slo[ib], shi[ib]};
-                    FREP_FENCE();
-                    r = Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
-                    const uint32_t ch = mm_choice(OP_MAX, A, B);
-                    const uint64_t F = __ballot((ch & MM_FIRST) != 0);
-                    const uint64_t S = __ballot((ch & MM_SECOND) != 0);
-                    scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
-                                         (uint32_t)(S >> 32));
-                    ++mm;
-                }
-            } else if (op & 4u) {
-                const Itv B{slo[ib], shi[ib]};
-                FREP_FENCE();
-                if (op & 2u) {
-                    if (op & 1u) {
-                        r = Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)};
-                        const uint32_t ch = mm_choice(OP_MIN, A, B);
-                        const uint64_t F = __ballot((ch & MM_FIRST) != 0);
-                        const uint64_t S = __ballot((ch & MM_SECOND) != 0);
-                        scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
-                                             (uint32_t)(S >> 32));
-                        ++mm;
-                    } else {
-                        r = interval_op(OP_DIV, A, B, 0.0f);
-                    }
-                } else if (op & 1u) {
-                    r = imul(A, B);
-                } else {
-                    r = Itv{A.lo - B.hi, A.hi - B.lo};
-                }
-            } else if (op & 2u) {
-                if (op & 1u) {
-                    const Itv B{slo[ib], shi[ib]};
-                    FREP_FENCE();
-                    r = Itv{A.lo + B.lo, A.hi + B.hi};
-                } else {
-                    r = interval_op(OP_COS, A, A, 0.0f);
-                }
-            } else if (op & 1u) {
-                r = interval_op(OP_SIN, A, A, 0.0f);
-            } else {
+            Itv r;
+            if (op == OP_CONST) {
                 const float c = rdlanef(fc0, j);
                 r = Itv{c, c};
+            } else if (op == OP_COPY) {
+                if (o == ia) continue;
+                r = Itv{slo[ia], shi[ia]};
+                FREP_FENCE();
+            } else if (op - OP_ADD <= OP_MUL - OP_ADD) {
+                const Itv A{slo[ia], shi[ia]};
+                FREP_FENCE();
+                const Itv B{slo[ib], shi[ib]};
+                FREP_FENCE();
+                if (op == OP_MUL) r = imul(A, B);
+                else if (op == OP_ADD) r = Itv{A.lo + B.lo, A.hi + B.hi};
+                else r = Itv{A.lo - B.hi, A.hi - B.lo};
+            } else if (op == OP_MIN || op == OP_MAX) {
+                const Itv A{slo[ia], shi[ia]};
+                FREP_FENCE();
+                const Itv B{slo[ib], shi[ib]};
+                FREP_FENCE();
+                r = op == OP_MIN ? Itv{fmin_glm(A.lo, B.lo), fmin_glm(A.hi, B.hi)}
+                                 : Itv{fmax_glm(A.lo, B.lo), fmax_glm(A.hi, B.hi)};
+                const uint32_t ch = mm_choice(op, A, B);
+                const uint64_t F = __ballot((ch & MM_FIRST) != 0);
+                const uint64_t S = __ballot((ch & MM_SECOND) != 0);
+                scr[mm] = make_uint4((uint32_t)F, (uint32_t)(F >> 32), (uint32_t)S,
+                                     (uint32_t)(S >> 32));
+                ++mm;
+            } else if (op == OP_NEG) {
+                const Itv A{slo[ia], shi[ia]};
+                FREP_FENCE();
+                r = Itv{-A.hi, -A.lo};
+            } else if (op == OP_SQRT) {
+                const Itv A{slo[ia], shi[ia]};
+                FREP_FENCE();
+                r = Itv{__builtin_sqrtf(fmax_glm(0.0f, A.lo)),
+                        __builtin_sqrtf(fmax_glm(0.0f, A.hi))};
+            } else {                                   // DIV, SIN, COS, EXP
+                const Itv A{slo[ia], shi[ia]};
+                FREP_FENCE();
+                const Itv B{slo[ib], shi[ib]};
+                FREP_FENCE();
+                r = interval_op(op, A, B, 0.0f);
             }
             FREP_FENCE();
             slo[o] = r.lo;

@@ -323,10 +323,8 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
                 for (int32_t i = e - 1; i >= lo; --i) {
                     const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
                     const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
-                    const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu);
-                    const uint32_t ib = slot_in<MAXS>(w >> 24);
                     const bool is_mm = (op == OP_MIN || op == OP_MAX);
-                    uint32_t ch = 0;
+                    uint32_t ch = MM_BOTH;
                     if (is_mm) {
                         const uint4 v = row;
                         --jm;
@@ -336,23 +334,31 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
                         ch = (uint32_t)((F >> lane) & 1u) |
                              ((uint32_t)((S >> lane) & 1u) << 1);
                     }
-                    if ((act[o >> 5] >> (o & 31u)) & 1u) {
-                        act[o >> 5] &= ~(1u << (o & 31u));  // output first (shorten.comp:179)
-                        uint32_t sel;
-                        if (op == OP_CONST) sel = MM_NONE;
-                        else if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
-                            sel = MM_BOTH;
-                        else if (is_mm) sel = ch;
-                        else sel = MM_FIRST;
-                        if (sel & MM_FIRST) act[ia >> 5] |= 1u << (ia & 31u);
-                        if (sel & MM_SECOND) act[ib >> 5] |= 1u << (ib & 31u);
-                        uint32_t ow = w;
-                        if (is_mm && ch == MM_FIRST) ow = (w & ~0xFFu) | OP_COPY;
-                        else if (is_mm && ch == MM_SECOND)
-                            ow = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
-                        out[n - kept] = ow;
-                        ++kept;
-                    }
+                    const uint32_t obit = 1u << (o & 31u);
+                    const bool live = (act[o >> 5] & obit) != 0;
+                    // a word no lane keeps changes nothing (common once the
+                    // cells of a chunk agree on their choices)
+                    if (__ballot(live) == 0) continue;
+                    const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu);
+                    const uint32_t ib = slot_in<MAXS>(w >> 24);
+                    // operands a kept word reads: none for CONST, both for
+                    // ADD/SUB/MUL/DIV, the chosen ones for MIN/MAX, else A
+                    const uint32_t sel = op == OP_CONST ? MM_NONE
+                                         : (op - OP_ADD <= OP_DIV - OP_ADD) ? MM_BOTH
+                                         : (is_mm ? ch : MM_FIRST);
+                    // lane masks instead of branches: clear the output slot
+                    // first, then set the operand slots (shorten.comp:179)
+                    const uint32_t keep = live ? ~0u : 0u;
+                    act[o >> 5] &= ~(obit & keep);
+                    act[ia >> 5] |= ((sel & MM_FIRST) ? (1u << (ia & 31u)) : 0u) & keep;
+                    act[ib >> 5] |= ((sel & MM_SECOND) ? (1u << (ib & 31u)) : 0u) & keep;
+                    const uint32_t copy_a = (w & ~0xFFu) | OP_COPY;
+                    const uint32_t copy_b = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
+                    const uint32_t ow = !is_mm ? w
+                                        : (ch == MM_FIRST ? copy_a
+                                                          : (ch == MM_SECOND ? copy_b : w));
+                    if (live) out[n - kept] = ow;
+                    kept += live ? 1u : 0u;
                 }
                 bw0 = bw1;
                 bw1 = bw2;

@@ -166,6 +166,19 @@ __device__ __forceinline__ uint32_t slot_in(uint32_t x) {
     else return x < (uint32_t)MAXS ? x : (uint32_t)(MAXS - 1);
 }
 
+// Slot field of a tape word for the interval levels' register files:
+// masked to the power of two at or above MAXS, which folds into the field
+// extract (one s_bfe, where a clamp costs an extra s_min per operand).
+// Every slot of an uploaded tape is below its slot count (validated on
+// upload, and the shortened tapes the levels write keep their parents' slot
+// fields), and a tape runs on a file of at least that many slots, so the
+// mask never changes a field; it only bounds the index.
+template <int MAXS>
+__device__ __forceinline__ uint32_t slot_field(uint32_t x) {
+    constexpr uint32_t span = MAXS <= 1 ? 1u : (1u << (32 - __builtin_clz((uint32_t)MAXS - 1u)));
+    return x & (span - 1u);
+}
+
 // Evaluates a tape at K <= 8 points per lane.
 //
 // Register file: each point has its own slot array.  The compiler keeps every

@@ -213,8 +213,8 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
             const uint32_t w = rdlane(fw0, j);
             // operand fields are slot indices except for CONST (pool
             // index); masking keeps every register-file access in range
-            const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
-            const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu), ib = slot_in<MAXS>(w >> 24);
+            const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
+            const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu), ib = slot_field<MAXS>(w >> 24);
             // One uniform compare per test, the ops inner levels run most
             // first (CONST, COPY, ADD/SUB/MUL make up ~90% of their words);
             // op codes: 0 CONST 1 SIN 2 COS 3 ADD 4 SUB 5 MUL 6 DIV 7 MIN
@@ -322,7 +322,7 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
                 const int32_t lo = s > 0 ? s : 0;
                 for (int32_t i = e - 1; i >= lo; --i) {
                     const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
-                    const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
+                    const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
                     const bool is_mm = (op == OP_MIN || op == OP_MAX);
                     uint32_t ch = MM_BOTH;
                     if (is_mm) {
@@ -339,8 +339,8 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
                     // a word no lane keeps changes nothing (common once the
                     // cells of a chunk agree on their choices)
                     if (__ballot(live) == 0) continue;
-                    const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu);
-                    const uint32_t ib = slot_in<MAXS>(w >> 24);
+                    const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu);
+                    const uint32_t ib = slot_field<MAXS>(w >> 24);
                     // operands a kept word reads: none for CONST, both for
                     // ADD/SUB/MUL/DIV, the chosen ones for MIN/MAX, else A
                     const uint32_t sel = op == OP_CONST ? MM_NONE
@@ -900,8 +900,11 @@ __device__ __forceinline__ void level_point_body(const LevelArgs& a) {
         // of padding past its end) and length, and c3's header.
         uint32_t c = gwave;
         if (c >= total) return;
+        // node of a chunk: one chunk per node for 8-wide last levels (the
+        // common case) skips the integer division
+        auto node_of = [&](uint32_t ch) { return cpw == 1 ? ch : ch / cpw; };
         auto header = [&](uint32_t ch, uint32_t h[4]) {
-            ctape_t r = as_const(a.nodes + (size_t)(ch / cpw) * a.ns);
+            ctape_t r = as_const(a.nodes + (size_t)node_of(ch) * a.ns);
             h[0] = r[0]; h[1] = r[1]; h[2] = r[2]; h[3] = r[3];
         };
         uint32_t h0[4], h1[4], h2[4];
@@ -916,10 +919,10 @@ __device__ __forceinline__ void level_point_body(const LevelArgs& a) {
         for (;;) {
             const uint32_t c1 = c + nwaves, c2 = c1 + nwaves, c3 = c2 + nwaves;
             const uint32_t* pf_len = a.pool + (c2 < total ? h2[0] : h0[0]);
-            const uint32_t* pf_hdr = a.nodes + (size_t)((c3 < total ? c3 : c) / cpw) * a.ns;
+            const uint32_t* pf_hdr = a.nodes + (size_t)node_of(c3 < total ? c3 : c) * a.ns;
             uint32_t w2, n2, h3[4];
             float c0n;
-            const uint32_t node = c / cpw;
+            const uint32_t node = node_of(c);
             point_chunk_asm<MAXS, true>(a, node, c - node * cpw, lane, h0[0], h0[1], h0[2], h0[3],
                                         w0, c0, pf_len + 1, pf_len, pf_hdr, &w2, &n2, h3, n0, w1,
                                         n1, &c0n);

@@ -11,6 +11,7 @@
 #include "frep_internal.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <utility>
 
@@ -384,6 +385,265 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     __builtin_amdgcn_s_setprio(0);
 }
 
+// The same rules with two cells per lane: a wavefront owns 128 consecutive
+// mask bits (two 64-bit groups, lane l evaluating bits 128*sub + l and
+// 128*sub + 64 + l).  Tape words are decoded, dispatched and their windows
+// and constants loaded once for both cells, and a wavefront's header and
+// window loads are waited on once per 128 cells; each MIN/MAX writes one
+// choice row per group.  Everything else -- classification, reservation,
+// shortening, child records -- is interval_chunk's, per group.
+template <int MAXS, class Reserve>
+__device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t node, uint32_t sub,
+                                                uint64_t child_base, uint32_t lane, uint4* scr,
+                                                const Reserve& reserve) {
+    constexpr int AW = (MAXS + 31) / 32;
+    const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
+    uint32_t* rec = a.nodes + (size_t)node * a.ns;
+    ctape_t hdr = as_const(rec);
+    const uint32_t toff = hdr[0], ncx = hdr[1], ncy = hdr[2], ncz = hdr[3];
+    const uint32_t idx0 = sub * 128 + lane, idx1 = idx0 + 64;
+    uint32_t cx, cy, cz;
+    cell_of(idx0, dim, a.dim_shift, &cx, &cy, &cz);
+    const uint32_t gx0 = ncx * dim + cx, gy0 = ncy * dim + cy, gz0 = ncz * dim + cz;
+    cell_of(idx1, dim, a.dim_shift, &cx, &cy, &cz);
+    const uint32_t gx1 = ncx * dim + cx, gy1 = ncy * dim + cy, gz1 = ncz * dim + cz;
+    const uint32_t n = as_const(a.pool + toff)[0];
+    const uint32_t* tw = a.pool + toff + 1;
+    if (n >= 96) __builtin_amdgcn_s_setprio(3);
+    else if (n >= 64) __builtin_amdgcn_s_setprio(2);
+    else if (n >= 32) __builtin_amdgcn_s_setprio(1);
+
+    // ---- forward pass: interval evaluation (interval.comp:121-160) ----
+    uint32_t mm = 0;
+    float slo0[MAXS], shi0[MAXS], slo1[MAXS], shi1[MAXS];
+    {
+        // main.comp:113-116, 124-126: box [pos, pos + cell_size]
+        const float lx0 = a.low[0] + (float)gx0 * a.cs, lx1 = a.low[0] + (float)gx1 * a.cs;
+        const float ly0 = a.low[1] + (float)gy0 * a.cs, ly1 = a.low[1] + (float)gy1 * a.cs;
+        const float lz0 = a.low[2] + (float)gz0 * a.cs, lz1 = a.low[2] + (float)gz1 * a.cs;
+        slo0[0] = lx0; shi0[0] = lx0 + a.cs; slo1[0] = lx1; shi1[0] = lx1 + a.cs;
+        slo0[1] = ly0; shi0[1] = ly0 + a.cs; slo1[1] = ly1; shi1[1] = ly1 + a.cs;
+        slo0[2] = lz0; shi0[2] = lz0 + a.cs; slo1[2] = lz1; shi1[2] = lz1 + a.cs;
+        slo0[3] = shi0[3] = slo1[3] = shi1[3] = a.tape_time;
+    }
+    uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
+    uint32_t fw2 = window_load(tw, 128, n, lane);
+    float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
+    for (uint32_t s = 0; s < n; s += 64) {
+        const uint32_t m = n - s < 64 ? n - s : 64;
+        for (uint32_t j = 0; j < m; ++j) {
+            const uint32_t w = rdlane(fw0, j);
+            const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
+            const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu), ib = slot_field<MAXS>(w >> 24);
+            Itv r0, r1;
+            if (op == OP_CONST) {
+                const float c = rdlanef(fc0, j);
+                r0 = r1 = Itv{c, c};
+            } else if (op == OP_COPY) {
+                if (o == ia) continue;
+                r0 = Itv{slo0[ia], shi0[ia]};
+                r1 = Itv{slo1[ia], shi1[ia]};
+                FREP_FENCE();
+            } else if (op - OP_ADD <= OP_MUL - OP_ADD) {
+                const Itv A0{slo0[ia], shi0[ia]}, A1{slo1[ia], shi1[ia]};
+                FREP_FENCE();
+                const Itv B0{slo0[ib], shi0[ib]}, B1{slo1[ib], shi1[ib]};
+                FREP_FENCE();
+                if (op == OP_MUL) {
+                    r0 = imul(A0, B0);
+                    r1 = imul(A1, B1);
+                } else if (op == OP_ADD) {
+                    r0 = Itv{A0.lo + B0.lo, A0.hi + B0.hi};
+                    r1 = Itv{A1.lo + B1.lo, A1.hi + B1.hi};
+                } else {
+                    r0 = Itv{A0.lo - B0.hi, A0.hi - B0.lo};
+                    r1 = Itv{A1.lo - B1.hi, A1.hi - B1.lo};
+                }
+            } else if (op == OP_MIN || op == OP_MAX) {
+                const Itv A0{slo0[ia], shi0[ia]}, A1{slo1[ia], shi1[ia]};
+                FREP_FENCE();
+                const Itv B0{slo0[ib], shi0[ib]}, B1{slo1[ib], shi1[ib]};
+                FREP_FENCE();
+                if (op == OP_MIN) {
+                    r0 = Itv{fmin_glm(A0.lo, B0.lo), fmin_glm(A0.hi, B0.hi)};
+                    r1 = Itv{fmin_glm(A1.lo, B1.lo), fmin_glm(A1.hi, B1.hi)};
+                } else {
+                    r0 = Itv{fmax_glm(A0.lo, B0.lo), fmax_glm(A0.hi, B0.hi)};
+                    r1 = Itv{fmax_glm(A1.lo, B1.lo), fmax_glm(A1.hi, B1.hi)};
+                }
+                const uint32_t ch0 = mm_choice(op, A0, B0), ch1 = mm_choice(op, A1, B1);
+                const uint64_t F0 = __ballot((ch0 & MM_FIRST) != 0);
+                const uint64_t S0 = __ballot((ch0 & MM_SECOND) != 0);
+                const uint64_t F1 = __ballot((ch1 & MM_FIRST) != 0);
+                const uint64_t S1 = __ballot((ch1 & MM_SECOND) != 0);
+                scr[2 * mm] = make_uint4((uint32_t)F0, (uint32_t)(F0 >> 32), (uint32_t)S0,
+                                         (uint32_t)(S0 >> 32));
+                scr[2 * mm + 1] = make_uint4((uint32_t)F1, (uint32_t)(F1 >> 32), (uint32_t)S1,
+                                             (uint32_t)(S1 >> 32));
+                ++mm;
+            } else if (op == OP_NEG) {
+                const Itv A0{slo0[ia], shi0[ia]}, A1{slo1[ia], shi1[ia]};
+                FREP_FENCE();
+                r0 = Itv{-A0.hi, -A0.lo};
+                r1 = Itv{-A1.hi, -A1.lo};
+            } else if (op == OP_SQRT) {
+                const Itv A0{slo0[ia], shi0[ia]}, A1{slo1[ia], shi1[ia]};
+                FREP_FENCE();
+                r0 = Itv{__builtin_sqrtf(fmax_glm(0.0f, A0.lo)),
+                         __builtin_sqrtf(fmax_glm(0.0f, A0.hi))};
+                r1 = Itv{__builtin_sqrtf(fmax_glm(0.0f, A1.lo)),
+                         __builtin_sqrtf(fmax_glm(0.0f, A1.hi))};
+            } else {                                   // DIV, SIN, COS, EXP
+                const Itv A0{slo0[ia], shi0[ia]}, A1{slo1[ia], shi1[ia]};
+                FREP_FENCE();
+                const Itv B0{slo0[ib], shi0[ib]}, B1{slo1[ib], shi1[ib]};
+                FREP_FENCE();
+                r0 = interval_op(op, A0, B0, 0.0f);
+                r1 = interval_op(op, A1, B1, 0.0f);
+            }
+            FREP_FENCE();
+            slo0[o] = r0.lo;
+            shi0[o] = r0.hi;
+            slo1[o] = r1.lo;
+            shi1[o] = r1.hi;
+        }
+        fw0 = fw1; fc0 = fc1;
+        fw1 = fw2; fc1 = window_consts(fw2, a.consts);
+        fw2 = window_load(tw, (int32_t)s + 192, n, lane);
+    }
+    // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
+    const bool full0 = shi0[0] < 0.0f, empty0 = slo0[0] > 0.0f, amb0 = !full0 && !empty0;
+    const bool full1 = shi1[0] < 0.0f, empty1 = slo1[0] > 0.0f, amb1 = !full1 && !empty1;
+    const uint64_t Bfull0 = __ballot(full0), Bamb0 = __ballot(amb0);
+    const uint64_t Bfull1 = __ballot(full1), Bamb1 = __ballot(amb1);
+    store_mask64(rec + 4 + sub * 4, Bfull0, lane);
+    store_mask64(rec + 4 + sub * 4 + 2, Bfull1, lane);
+    store_mask64(rec + 4 + W + sub * 4, Bamb0, lane);
+    store_mask64(rec + 4 + W + sub * 4 + 2, Bamb1, lane);
+
+    uint32_t child0 = kNoChild, child1 = kNoChild;
+    // one packed reservation for both groups: group 0's children first
+    const uint32_t namb0 = (uint32_t)__popcll(Bamb0), namb = namb0 + (uint32_t)__popcll(Bamb1);
+    const uint32_t stride = n + 1;
+    const uint64_t old =
+        reserve((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
+                ((uint64_t)namb << kAllocShift) | ((uint64_t)namb * stride), lane);
+    if (namb != 0) {
+        const uint32_t rank0 = lane_rank(Bamb0), rank1 = namb0 + lane_rank(Bamb1);
+        const uint32_t cbase = alloc_nodes(old);
+        const uint64_t pbase = child_base + alloc_words(old);
+        const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
+        const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
+        if (!(nodes_ok && pool_ok)) {
+            if (lane == 0)
+                atomicOr(a.counters + CNT_OVERFLOW,
+                         ((nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL)) << (2 * a.level));
+        } else {
+            // ---- backward pass: liveness + compaction (shorten.comp:164-223) ----
+            const uint64_t base0 = pbase + (uint64_t)rank0 * stride;
+            const uint64_t base1 = pbase + (uint64_t)rank1 * stride;
+            uint32_t* out0 = a.pool + base0;
+            uint32_t* out1 = a.pool + base1;
+            uint32_t act0[AW], act1[AW];
+#pragma unroll
+            for (int k = 0; k < AW; ++k) act0[k] = act1[k] = 0;
+            act0[0] = amb0 ? 1u : 0u;                      // slot 0 is the result
+            act1[0] = amb1 ? 1u : 0u;
+            uint32_t kept0 = 0, kept1 = 0;
+            int32_t jm = (int32_t)mm - 1;
+            uint4 row0 = scr[2 * (jm > 0 ? jm : 0)], row1 = scr[2 * (jm > 0 ? jm : 0) + 1];
+            int32_t e = (int32_t)n;
+            uint32_t bw0 = window_load(tw, e - 64, n, lane);
+            uint32_t bw1 = window_load(tw, e - 128, n, lane);
+            uint32_t bw2 = window_load(tw, e - 192, n, lane);
+            while (e > 0) {
+                const int32_t s = e - 64;
+                const int32_t lo = s > 0 ? s : 0;
+                for (int32_t i = e - 1; i >= lo; --i) {
+                    const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
+                    const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
+                    const bool is_mm = (op == OP_MIN || op == OP_MAX);
+                    uint32_t ch0 = MM_BOTH, ch1 = MM_BOTH;
+                    if (is_mm) {
+                        const uint4 v0 = row0, v1 = row1;
+                        --jm;
+                        row0 = scr[2 * (jm > 0 ? jm : 0)];
+                        row1 = scr[2 * (jm > 0 ? jm : 0) + 1];
+                        ch0 = (uint32_t)(((((uint64_t)v0.y << 32) | v0.x) >> lane) & 1u) |
+                              ((uint32_t)(((((uint64_t)v0.w << 32) | v0.z) >> lane) & 1u) << 1);
+                        ch1 = (uint32_t)(((((uint64_t)v1.y << 32) | v1.x) >> lane) & 1u) |
+                              ((uint32_t)(((((uint64_t)v1.w << 32) | v1.z) >> lane) & 1u) << 1);
+                    }
+                    const uint32_t obit = 1u << (o & 31u);
+                    const bool live0 = (act0[o >> 5] & obit) != 0;
+                    const bool live1 = (act1[o >> 5] & obit) != 0;
+                    if (__ballot(live0 || live1) == 0) continue;
+                    const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu);
+                    const uint32_t ib = slot_field<MAXS>(w >> 24);
+                    const uint32_t base_sel = op == OP_CONST ? MM_NONE
+                                              : (op - OP_ADD <= OP_DIV - OP_ADD) ? MM_BOTH
+                                                                                 : MM_FIRST;
+                    const uint32_t sel0 = is_mm ? ch0 : base_sel;
+                    const uint32_t sel1 = is_mm ? ch1 : base_sel;
+                    const uint32_t abit = 1u << (ia & 31u), bbit = 1u << (ib & 31u);
+                    const uint32_t keep0 = live0 ? ~0u : 0u, keep1 = live1 ? ~0u : 0u;
+                    act0[o >> 5] &= ~(obit & keep0);
+                    act1[o >> 5] &= ~(obit & keep1);
+                    act0[ia >> 5] |= ((sel0 & MM_FIRST) ? abit : 0u) & keep0;
+                    act1[ia >> 5] |= ((sel1 & MM_FIRST) ? abit : 0u) & keep1;
+                    act0[ib >> 5] |= ((sel0 & MM_SECOND) ? bbit : 0u) & keep0;
+                    act1[ib >> 5] |= ((sel1 & MM_SECOND) ? bbit : 0u) & keep1;
+                    const uint32_t copy_a = (w & ~0xFFu) | OP_COPY;
+                    const uint32_t copy_b = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
+                    const uint32_t ow0 = !is_mm ? w
+                                         : (ch0 == MM_FIRST ? copy_a
+                                                            : (ch0 == MM_SECOND ? copy_b : w));
+                    const uint32_t ow1 = !is_mm ? w
+                                         : (ch1 == MM_FIRST ? copy_a
+                                                            : (ch1 == MM_SECOND ? copy_b : w));
+                    if (live0) out0[n - kept0] = ow0;
+                    if (live1) out1[n - kept1] = ow1;
+                    kept0 += live0 ? 1u : 0u;
+                    kept1 += live1 ? 1u : 0u;
+                }
+                bw0 = bw1;
+                bw1 = bw2;
+                bw2 = window_load(tw, e - 256, n, lane);
+                e -= 64;
+            }
+            if (amb0) {
+                uint32_t final_off = toff;
+                if (kept0 < n) {
+                    out0[n - kept0] = kept0;
+                    final_off = (uint32_t)(base0 + n - kept0);
+                }
+                child0 = cbase + rank0;
+                uint32_t* crec = a.child_nodes + (size_t)child0 * a.child_ns;
+                crec[0] = final_off;
+                crec[1] = gx0;
+                crec[2] = gy0;
+                crec[3] = gz0;
+            }
+            if (amb1) {
+                uint32_t final_off = toff;
+                if (kept1 < n) {
+                    out1[n - kept1] = kept1;
+                    final_off = (uint32_t)(base1 + n - kept1);
+                }
+                child1 = cbase + rank1;
+                uint32_t* crec = a.child_nodes + (size_t)child1 * a.child_ns;
+                crec[0] = final_off;
+                crec[1] = gx1;
+                crec[2] = gy1;
+                crec[3] = gz1;
+            }
+        }
+    }
+    rec[4 + 2 * W + idx0] = child0;
+    rec[4 + 2 * W + idx1] = child1;
+    __builtin_amdgcn_s_setprio(0);
+}
+
 // Node count of `level` and first pool word of level+1's tapes (tapes of
 // level+1 start after those of levels 1..level).
 __device__ __forceinline__ uint64_t child_tape_base(const uint32_t* counters, uint32_t level) {
@@ -395,9 +655,9 @@ __device__ __forceinline__ uint64_t child_tape_base(const uint32_t* counters, ui
 // Slot files of up to 12 slots are built for 8 wavefronts per SIMD (at most
 // 64 VGPRs and a scalar budget to match); larger ones keep the compiler's
 // choice.
-template <int MAXS>
+template <int MAXS, int KC = 1>
 __global__ __launch_bounds__(kLevelBlock)
-__attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval(LevelArgs a) {
+__attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 8 : 1)))) void k_level_interval(LevelArgs a) {
     // min/max choice ballots of this wavefront's current chunk: mm_stride rows
     // of {FIRST lanes, SECOND lanes} (two 64-bit masks) per wavefront
     extern __shared__ uint4 lds_choice[];
@@ -408,9 +668,9 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval(L
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
     uint64_t child_base = child_tape_base(a.counters, a.level);
     child_base = ((uint64_t)uni((uint32_t)(child_base >> 32)) << 32) | uni((uint32_t)child_base);
-    const uint32_t cpn = a.dim * a.dim * a.dim / 64;
+    const uint32_t cpn = a.dim * a.dim * a.dim / (64 * KC);
     const uint32_t total = n_nodes * cpn;
-    uint4* scr = lds_choice + (size_t)wib * a.mm_stride;
+    uint4* scr = lds_choice + (size_t)wib * a.mm_stride * KC;
     // rounds of kLevelBlock/64 consecutive chunks per block, one reservation
     // atomic per round; a wavefront past the end still takes part
     constexpr uint32_t WPB = kLevelBlock / 64;
@@ -422,7 +682,8 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval(L
         const uint32_t chunk = g * WPB + wib;
         if (chunk < total) {
             const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
-            interval_chunk<MAXS>(a, node, sub, child_base, lane, scr, br);
+            if constexpr (KC == 2) interval_chunk2<MAXS>(a, node, sub, child_base, lane, scr, br);
+            else interval_chunk<MAXS>(a, node, sub, child_base, lane, scr, br);
         } else {
             br(ctr, 0, lane);
         }
@@ -1571,12 +1832,12 @@ uint32_t resident_grid(Kern kern, uint32_t cu_count, uint32_t max_grid, size_t l
 
 // The occupancy of an interval kernel depends on its dynamic LDS (choice
 // rows); the last answer is kept per instantiation and thread.
-template <int M>
+template <int M, int KC = 1>
 uint32_t interval_grid(uint32_t cu_count, uint32_t max_grid, size_t lds) {
     thread_local size_t last_lds = ~(size_t)0;
     thread_local uint32_t last_cu = 0, last_max = 0, g = 0;
     if (lds != last_lds || cu_count != last_cu || max_grid != last_max) {
-        g = resident_grid(k_level_interval<M>, cu_count, max_grid, lds);
+        g = resident_grid(k_level_interval<M, KC>, cu_count, max_grid, lds);
         last_lds = lds;
         last_cu = cu_count;
         last_max = max_grid;
@@ -1595,6 +1856,15 @@ uint32_t point_grid(uint32_t cu_count, uint32_t max_grid) {
     return g;
 }
 
+// FREP_INTERVAL_K1=1 keeps one cell per lane on every interval level.
+bool two_cells_per_lane() {
+    static const bool on = [] {
+        const char* e = std::getenv("FREP_INTERVAL_K1");
+        return !(e && e[0] == '1');
+    }();
+    return on;
+}
+
 size_t interval_lds_bytes(uint32_t mm_stride) {
     return (size_t)(kLevelBlock / 64) * mm_stride * sizeof(uint4);
 }
@@ -1603,6 +1873,21 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
                                  uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
     const size_t lds = interval_lds_bytes(a.mm_stride);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
+    // two cells per lane (one choice row per group, so twice the LDS) for
+    // files of up to 16 slots on nodes of a multiple of 128 cells
+    const uint32_t cells = a.dim * a.dim * a.dim;
+    if (maxs <= 16 && cells % 128 == 0 && 2 * lds <= kMaxChoiceLds && two_cells_per_lane()) {
+        if (maxs == 8)
+            k_level_interval<8, 2><<<interval_grid<8, 2>(cu_count, max_grid, 2 * lds),
+                                     kLevelBlock, 2 * lds, st>>>(a);
+        else if (slot_count <= 12)
+            k_level_interval<12, 2><<<interval_grid<12, 2>(cu_count, max_grid, 2 * lds),
+                                      kLevelBlock, 2 * lds, st>>>(a);
+        else
+            k_level_interval<16, 2><<<interval_grid<16, 2>(cu_count, max_grid, 2 * lds),
+                                      kLevelBlock, 2 * lds, st>>>(a);
+        return hipGetLastError();
+    }
     // a 12-slot file frees 8 registers per lane over the 16-slot one, which
     // is what 8 wavefronts per SIMD need
     if (maxs == 16 && slot_count <= 12) {

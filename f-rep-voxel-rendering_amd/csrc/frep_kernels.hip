@@ -1856,13 +1856,11 @@ uint32_t point_grid(uint32_t cu_count, uint32_t max_grid) {
     return g;
 }
 
-// FREP_INTERVAL_K1=1 keeps one cell per lane on every interval level.
+// FREP_INTERVAL_K1=1 keeps one cell per lane on every interval level
+// (read at each launch, so a test can compare both forms in one process).
 bool two_cells_per_lane() {
-    static const bool on = [] {
-        const char* e = std::getenv("FREP_INTERVAL_K1");
-        return !(e && e[0] == '1');
-    }();
-    return on;
+    const char* e = std::getenv("FREP_INTERVAL_K1");
+    return !(e && e[0] == '1');
 }
 
 size_t interval_lds_bytes(uint32_t mm_stride) {

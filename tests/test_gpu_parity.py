@@ -737,6 +737,34 @@ def test_asm_and_cpp_interpreters_agree(oracle):
         plain.close()
 
 
+@pytest.mark.parametrize("name,dims,t", [("csg24", [16, 8, 8], 0.0), ("spheres8", [8, 8, 4], 0.0),
+                                          ("trig", [8, 8, 4], 0.5), ("gyroid", [8, 8, 4], 0.0),
+                                          ("csg24", [8, 8, 8, 4], 0.0)])
+def test_interval_one_and_two_cells_per_lane_agree(ctx, oracle, name, dims, t):
+    """Interval levels evaluate two cells per lane wherever a node holds a
+    multiple of 128 cells and the slot file is at most 16 (one choice row
+    per 64-cell group); FREP_INTERVAL_K1=1 forces one cell per lane.  Both
+    give the oracle's tree, in 3- and 4-level trees, with sin/cos/exp/div
+    and time-dependent tapes."""
+    import os
+    tape = shapes.gyroid_blend() if name == "gyroid" else TAPES[name]()
+    grid = shapes.make_grid(dims, tape_time=t)
+    th = ctx.upload(tape)
+    outs = []
+    for k1 in ("1", None):
+        if k1:
+            os.environ["FREP_INTERVAL_K1"] = k1
+        try:
+            tr = ctx.voxelize(th, grid)
+        finally:
+            os.environ.pop("FREP_INTERVAL_K1", None)
+        outs.append(ctx.export(tr))
+        ctx.tree_free(tr)
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[1], oracle.Tree(tape, grid).canonical())
+    ctx.tape_free(th)
+
+
 def test_cpp_engine_classes_match_the_c_abi(ctx, tmp_path):
     """The C++ engine classes (include/frep_engine.hpp), driven like the
     reference's Application::Draw by cpp/engine_demo, give the same tree,

@@ -1272,6 +1272,10 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         }
         LevelArgs la[kMaxLevels];
         for (uint32_t L = 0; L < nl; ++L) la[L] = level_args(t, L, j.mm_stride);
+        // node counts of the tree's previous build (zero for a new tree):
+        // a hint for the interval levels' work shape only
+        uint32_t nodes_hint[kMaxLevels] = {};
+        for (uint32_t L = 0; L < nl; ++L) nodes_hint[L] = host_nodes(t, L);
         la[nl - 1].finish = 1;
         la[nl - 1].host_counters = t->h_counters;
         // the buffers are as k_init_tree leaves them for this tape when the
@@ -1306,7 +1310,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
                     HIP_TRY(launch_root_interval(ctx->stream, ra));
                 else
                     HIP_TRY(launch_level_interval(ctx->stream, la[L], t->maxs, t->slot_count,
-                                                  ctx->cu_count, ctx->level_grid));
+                                                  ctx->cu_count, ctx->level_grid, nodes_hint[L]));
                 if (events) HIP_TRY(hipEventRecord(ctx->ev[2 + L], ctx->stream));
             }
             return FREP_OK;

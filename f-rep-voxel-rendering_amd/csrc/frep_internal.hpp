@@ -175,8 +175,11 @@ hipError_t launch_init_tree(hipStream_t st, uint32_t* counters, uint32_t* nodes0
 // Level kernels run a persistent grid of min(max_grid, cu_count x resident
 // blocks per CU) blocks; max_grid bounds the per-wave scratch rows.
 // slot_count: the tape's slot count (at most maxs), which picks the slot file.
+// nodes_hint: the level's node count in an earlier build (0 when unknown);
+// a level of few nodes keeps one cell per lane, for more wavefronts.
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid);
+                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
+                                 uint32_t nodes_hint = 0);
 // asm_ok: every op of the tape is in the assembly interpreter's set.
 // slots: the tape's slot count (at most maxs), which picks the slot file.
 hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,

@@ -1868,13 +1868,19 @@ size_t interval_lds_bytes(uint32_t mm_stride) {
 }
 
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
-                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
+                                 uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
+                                 uint32_t nodes_hint) {
     const size_t lds = interval_lds_bytes(a.mm_stride);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
     // two cells per lane (one choice row per group, so twice the LDS) for
-    // files of up to 16 slots on nodes of a multiple of 128 cells
+    // files of up to 16 slots on nodes of a multiple of 128 cells, unless
+    // the level is known to be small: below 16 one-cell chunks per CU, the
+    // wavefronts would not fill the SIMDs (a level of 122 nodes of 512
+    // cells ran 50 us with one cell per lane, 55.5 us with two)
     const uint32_t cells = a.dim * a.dim * a.dim;
-    if (maxs <= 16 && cells % 128 == 0 && 2 * lds <= kMaxChoiceLds && two_cells_per_lane()) {
+    const bool big = nodes_hint == 0 || (uint64_t)nodes_hint * (cells / 64) >= 16ull * cu_count;
+    if (maxs <= 16 && cells % 128 == 0 && 2 * lds <= kMaxChoiceLds && big &&
+        two_cells_per_lane()) {
         if (maxs == 8)
             k_level_interval<8, 2><<<interval_grid<8, 2>(cu_count, max_grid, 2 * lds),
                                      kLevelBlock, 2 * lds, st>>>(a);

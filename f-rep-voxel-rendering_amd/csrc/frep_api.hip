@@ -439,6 +439,10 @@ frep_status frep_init(int device, frep_ctx** out) {
     const char* ro = std::getenv("FREP_ROOT_OPS");
     c->root_ops = !(ro && ro[0] == '0');
     c->root_ops_required = ro && ro[0] == '2';
+    // the plain last level's wavefronts lower their issue priority as they
+    // progress, so a SIMD's wavefronts finish together
+    const char* pb = std::getenv("FREP_POINT_BALANCE");
+    c->point_balance = !(pb && pb[0] == '0');
     HIP_TRY(scratch_address(device, &c->noalloc_staging));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
@@ -1278,6 +1282,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         for (uint32_t L = 0; L < nl; ++L) nodes_hint[L] = host_nodes(t, L);
         la[nl - 1].finish = 1;
         la[nl - 1].host_counters = t->h_counters;
+        la[nl - 1].balance = ctx->point_balance ? 1u : 0u;
         // the buffers are as k_init_tree leaves them for this tape when the
         // last plain build of it into them completed (finish_tree)
         const bool skip_init = t->ready_gen == t->dev_gen && t->ready_tape == tp->uid;

@@ -527,23 +527,15 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
         [nm1] "=&s"(nm1), [cls] "=&s"(cls), [tiny] "=&s"(tiny), [sgn] "=&s"(sgn),             \
         [p0] "=&s"(p0), [p1] "=&s"(p1), [p2] "=&s"(p2), [p3] "=&s"(p3)
 
-// Prefetch around the interpreter (run_point_tape_asm8_pf): issued before
+// Prefetch around the interpreter (run_point_tape_asm8_pf2): issued before
 // the body, waited after it, so nothing is in flight when the block ends.
 //   pfw  lane l's word of 64 at pfa (the next tape's window 0)
 //   pfn  the word at pfb (the next tape's length)
 //   pfh  four words at pfc (the header of the node after that)
-#define FREP_ASM_PF_ISSUE                                                                     \
-    "v_mbcnt_lo_u32_b32 %[pft], -1, 0\n"                                                      \
-    "v_mbcnt_hi_u32_b32 %[pft], -1, %[pft]\n"                                                 \
-    "v_lshlrev_b32_e32 %[pft], 2, %[pft]\n"                                                   \
-    "global_load_dword %[pfw], %[pft], %[pfa]\n"                                              \
-    "s_load_dword %[pfn], %[pfb], 0x0\n"                                                      \
-    "s_load_dwordx4 %[pfh], %[pfc], 0x0\n"
+//   pfk  the constants of the next node, from its window pw1 (lane l's word
+//        of a tape of pn1 words): the constant a CONST word names for lanes
+//        below pn1, the pool's first constant otherwise
 #define FREP_ASM_PF_WAIT "s_waitcnt vmcnt(0) lgkmcnt(0)\n"
-// Two nodes deep (run_point_tape_asm8_pf2): also the constants of the next
-// node, from its window pw1 (lane l's word of a tape of pn1 words): the
-// constant a CONST word names for lanes below pn1, the pool's first
-// constant otherwise, into pfk.
 #define FREP_ASM_PF2_ISSUE                                                                    \
     "v_mbcnt_lo_u32_b32 %[pft], -1, 0\n"                                                      \
     "v_mbcnt_hi_u32_b32 %[pft], -1, %[pft]\n"                                                 \
@@ -622,58 +614,9 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
 
 // The same with the tape's length n given, loading while the tape runs:
 // pf_win[lane] into *w1, pf_len[0] into *n1 and pf_hdr[0..3] into h2 (all
-// readable addresses).
-template <int MAXS>
-__device__ __forceinline__ v8f run_point_tape_asm8_pf(const uint32_t* tape, const float* consts,
-                                                   float t, v8f x, v8f y, v8f z, uint32_t w0,
-                                                   float c0, uint32_t n, const uint32_t* pf_win,
-                                                   const uint32_t* pf_len, const uint32_t* pf_hdr,
-                                                   uint32_t* w1, uint32_t* n1, uint32_t h2[4],
-                                                   uint32_t use_t = 1) {
-    static_assert(MAXS == 8 || MAXS == 10 || MAXS == 16, "assembly path covers 8, 10 or 16 slots");
-    const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
-    FREP_ASM_POINT_DECLS;
-    typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-    uint32_t pfw, pft, pfn;
-    u32x4 pfh;
-    const uint64_t pfa = (uint64_t)pf_win, pfb = (uint64_t)pf_len, pfc = (uint64_t)pf_hdr;
-    if constexpr (MAXS == 16) {
-        asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v160}"(w0), "+{v163}"(c0),
-                       FREP_ASM_POINT_SCALARS,
-                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
-                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
-                       [lg] "n"(4)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100, FREP_ASM_CLOBBER_101_159,
-                       FREP_ASM_CLOBBER_W16, "vcc", "memory");
-    } else if constexpr (MAXS == 10) {
-        asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v112}"(w0), "+{v115}"(c0),
-                       FREP_ASM_POINT_SCALARS,
-                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
-                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
-                       [lg] "n"(4)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_100,
-                       FREP_ASM_CLOBBER_101_111, FREP_ASM_CLOBBER_W10, "vcc", "memory");
-    } else {
-        asm volatile(FREP_ASM_PF_ISSUE FREP_ASM_POINT_BODY FREP_ASM_PF_WAIT
-                     : "+{v[16:23]}"(x), "+{v[24:31]}"(y), "+{v[32:39]}"(z), "+{v96}"(w0), "+{v99}"(c0),
-                       FREP_ASM_POINT_SCALARS,
-                       [pfw] "=&v"(pfw), [pft] "=&v"(pft), [pfn] "=&s"(pfn), [pfh] "=&s"(pfh)
-                     : [pfa] "s"(pfa), [pfb] "s"(pfb), [pfc] "s"(pfc), [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
-                       [lg] "n"(3)
-                     : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
-                       "vcc", "memory");
-    }
-    *w1 = pfw;
-    *n1 = pfn;
-    h2[0] = pfh.x; h2[1] = pfh.y; h2[2] = pfh.z; h2[3] = pfh.w;
-    return x;
-}
-
-// Two nodes deep: as run_point_tape_asm8_pf, and also the constants of the
-// next node (its window nx_w, lane l's word of a tape of nx_n words) into
-// *nx_c, as point_window0 gathers them.
+// readable addresses), and the constants of the next node (its window nx_w,
+// lane l's word of a tape of nx_n words) into *nx_c, as point_window0
+// gathers them.
 template <int MAXS>
 __device__ __forceinline__ v8f run_point_tape_asm8_pf2(const uint32_t* tape, const float* consts,
                                                    float t, v8f x, v8f y, v8f z, uint32_t w0,

@@ -80,6 +80,8 @@ struct LevelArgs {
     uint32_t* host_counters;  // pinned host mirror of the counters (finish)
     uint32_t reads_t;         // the tape reads the input t: the assembly interpreter fills
                               // slot 3 with tape_time (otherwise the tape writes it first)
+    uint32_t balance;         // point level, plain kernel: a wavefront lowers its issue
+                              // priority (3 -> 0) as it works through its share
 };
 // Word CNT_DONE of a host mirror written by finish_tree.
 constexpr uint32_t kCountersWritten = 0xC0DE0001u;

@@ -1177,6 +1177,14 @@ __device__ __forceinline__ void level_point_body(const LevelArgs& a) {
         point_window0(a.pool + h0[0], a.consts, lane, &w0, &c0, &n0);
         uint32_t w1 = a.pool[h1[0] + 1 + lane];
         uint32_t n1 = c + nwaves < total ? as_const(a.pool + h1[0])[0] : 0u;
+        // A SIMD issues from its oldest wavefront first, so with equal shares
+        // its wavefronts would finish one after another and the last ones run
+        // alone.  With balance on, a wavefront steps its priority down from 3
+        // as it completes each quarter of its share and the others catch up.
+        const uint32_t q =
+            a.balance ? uni(max(1u, (total + 4u * nwaves - 1u) / (4u * nwaves))) : 0u;
+        if (q) __builtin_amdgcn_s_setprio(3);
+        uint32_t done = 0;
         for (;;) {
             const uint32_t c1 = c + nwaves, c2 = c1 + nwaves, c3 = c2 + nwaves;
             const uint32_t* pf_len = a.pool + (c2 < total ? h2[0] : h0[0]);
@@ -1188,6 +1196,12 @@ __device__ __forceinline__ void level_point_body(const LevelArgs& a) {
                                         w0, c0, pf_len + 1, pf_len, pf_hdr, &w2, &n2, h3, n0, w1,
                                         n1, &c0n);
             if (c1 >= total) break;
+            ++done;
+            if (q) {
+                if (done == q) __builtin_amdgcn_s_setprio(2);
+                else if (done == 2u * q) __builtin_amdgcn_s_setprio(1);
+                else if (done == 3u * q) __builtin_amdgcn_s_setprio(0);
+            }
             // the assembly's scalar outputs are wave-uniform: say so, so that
             // they stay in SGPRs for the next block's scalar inputs
             w0 = w1;
@@ -1200,6 +1214,7 @@ __device__ __forceinline__ void level_point_body(const LevelArgs& a) {
             h1[0] = h2[0]; h1[1] = h2[1]; h1[2] = h2[2]; h1[3] = h2[3];
             h2[0] = uni(h3[0]); h2[1] = uni(h3[1]); h2[2] = uni(h3[2]); h2[3] = uni(h3[3]);
         }
+        __builtin_amdgcn_s_setprio(0);
     } else {
         for (uint32_t chunk = gwave; chunk < total; chunk += nwaves) {
             const uint32_t node = chunk / cpw, sub = chunk - node * cpw;

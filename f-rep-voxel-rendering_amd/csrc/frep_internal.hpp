@@ -82,6 +82,8 @@ struct LevelArgs {
                               // slot 3 with tape_time (otherwise the tape writes it first)
     uint32_t balance;         // point level, plain kernel: a wavefront lowers its issue
                               // priority (3 -> 0) as it works through its share
+    uint32_t log_cap;         // interval level: shortening-log entries per wavefront (set at
+                              // launch from the LDS the launch gives it)
 };
 // Word CNT_DONE of a host mirror written by finish_tree.
 constexpr uint32_t kCountersWritten = 0xC0DE0001u;
@@ -135,6 +137,7 @@ struct BatchArgs {
     const LevelArgs* shapes;  // device array
     uint32_t n_shapes;
     uint32_t mm_stride;       // LDS choice rows per wavefront (max over shapes)
+    uint32_t log_cap;         // shortening-log entries per wavefront (set at launch)
 };
 constexpr uint32_t kBatchMax = 64;
 

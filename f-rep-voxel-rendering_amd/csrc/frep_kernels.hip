@@ -29,6 +29,16 @@ __device__ __forceinline__ uint32_t lane_rank(uint64_t m) {
                                      __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
+// Inclusive prefix sum of v over the lanes of the wavefront.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t u = (uint32_t)__shfl_up((int)v, (unsigned)d);
+        if (lane >= (uint32_t)d) v += u;
+    }
+    return v;
+}
+
 // Counter words read by the kernels (level node counts, tape bases, the
 // overflow bits of earlier levels) are final before the reading launch
 // starts: earlier launches on the stream wrote them, and the launch
@@ -154,6 +164,290 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 }
 
 // ---------------------------------------------------------------------------
+// Tape shortening of a wavefront's ambiguous cells (shorten.comp:164-223),
+// for G groups of 64 cells (lane l owns cell l of each group).
+//
+// One walk over the parent tape from its end serves every cell: a cell
+// keeps word i iff the word's output slot is live there (the output slot
+// is cleared, then the operands its min/max choice reads are set).  The
+// walk counts each cell's kept words, sorts the cells of a group into
+// classes that decide the same at every word (the same kept words and the
+// same min/max rewrites, so the same shortened tape: cells that decided the
+// same so far have the same live slots, so a class splits only at a live
+// MIN/MAX whose choices differ), and logs in LDS, for
+// every word some cell keeps, its index, its min/max row and the ballots of
+// the cells keeping it.  The first cell of each class owns the class's
+// tape.  One reservation (per wavefront, or per block) takes the child ids
+// and exactly the words of those tapes ([k, k words] each; none where
+// nothing was removed: the child then reuses the parent's tape,
+// shorten.comp:188-191).  Each owner's tape is written from the log by
+// ballot and prefix-sum compaction, 64 log entries at a time.  A log that
+// outgrew its LDS is replaced by a second walk in which the owners write
+// their kept words from the end of their region.
+// ---------------------------------------------------------------------------
+
+// A kept word as the shortened tape holds it: a MIN/MAX with one chosen
+// operand becomes a COPY of that operand (shorten.comp:199-214).
+__device__ __forceinline__ uint32_t shortened_word(uint32_t w, bool is_mm, uint32_t ch,
+                                                   uint32_t o, uint32_t ib) {
+    const uint32_t copy_a = (w & ~0xFFu) | OP_COPY;
+    const uint32_t copy_b = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
+    return !is_mm ? w : (ch == MM_FIRST ? copy_a : (ch == MM_SECOND ? copy_b : w));
+}
+
+// Backward walk over the parent tape tw[0..n) of a wavefront whose min/max
+// choice rows are scr[m * G + g] (m-th MIN/MAX, group g; mm of them):
+// on_word(i, m, is_mm, live, L, ch, ow) for every word some cell keeps,
+// with, per group, this lane's cell's liveness, the ballot of it, the
+// cell's choice (MM_BOTH for other ops) and the word as the cell keeps it.
+template <int MAXS, int G, class F>
+__device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, uint32_t mm,
+                                              const uint4* scr, const bool (&amb)[G],
+                                              uint32_t lane, F&& on_word) {
+    constexpr int AW = (MAXS + 31) / 32;
+    uint32_t act[G][AW];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+#pragma unroll
+        for (int k = 0; k < AW; ++k) act[g][k] = 0;
+        act[g][0] = amb[g] ? 1u : 0u;                  // slot 0 is the result
+    }
+    int32_t jm = (int32_t)mm - 1;
+    uint4 row[G];                                      // next choice rows, read ahead
+#pragma unroll
+    for (int g = 0; g < G; ++g) row[g] = scr[(jm > 0 ? jm : 0) * G + g];
+    int32_t e = (int32_t)n;                            // window = words [e-64, e)
+    uint32_t bw0 = window_load(tw, e - 64, n, lane);
+    uint32_t bw1 = window_load(tw, e - 128, n, lane);
+    uint32_t bw2 = window_load(tw, e - 192, n, lane);
+    while (e > 0) {
+        const int32_t s = e - 64;
+        const int32_t lo = s > 0 ? s : 0;
+        for (int32_t i = e - 1; i >= lo; --i) {
+            const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
+            const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
+            const bool is_mm = (op == OP_MIN || op == OP_MAX);
+            const uint32_t mi = jm > 0 ? (uint32_t)jm : 0u;
+            uint32_t ch[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) ch[g] = MM_BOTH;
+            if (is_mm) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    const uint4 v = row[g];
+                    const uint64_t Fm = ((uint64_t)v.y << 32) | v.x;
+                    const uint64_t Sm = ((uint64_t)v.w << 32) | v.z;
+                    ch[g] = (uint32_t)((Fm >> lane) & 1u) | ((uint32_t)((Sm >> lane) & 1u) << 1);
+                }
+                --jm;
+#pragma unroll
+                for (int g = 0; g < G; ++g) row[g] = scr[(jm > 0 ? jm : 0) * G + g];
+            }
+            const uint32_t obit = 1u << (o & 31u);
+            bool live[G];
+            uint64_t L[G], any = 0;
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                live[g] = (act[g][o >> 5] & obit) != 0;
+                L[g] = __ballot(live[g]);
+                any |= L[g];
+            }
+            // a word no cell keeps changes nothing (common once the cells of
+            // a chunk agree on their choices)
+            if (any == 0) continue;
+            const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu);
+            const uint32_t ib = slot_field<MAXS>(w >> 24);
+            uint32_t ow[G];
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                // operands a kept word reads: none for CONST, both for
+                // ADD/SUB/MUL/DIV, the chosen ones for MIN/MAX, else A
+                const uint32_t sel = op == OP_CONST ? MM_NONE
+                                     : (op - OP_ADD <= OP_DIV - OP_ADD) ? MM_BOTH
+                                     : (is_mm ? ch[g] : MM_FIRST);
+                // lane masks instead of branches: clear the output slot
+                // first, then set the operand slots (shorten.comp:179)
+                const uint32_t keep = live[g] ? ~0u : 0u;
+                act[g][o >> 5] &= ~(obit & keep);
+                act[g][ia >> 5] |= ((sel & MM_FIRST) ? (1u << (ia & 31u)) : 0u) & keep;
+                act[g][ib >> 5] |= ((sel & MM_SECOND) ? (1u << (ib & 31u)) : 0u) & keep;
+                ow[g] = shortened_word(w, is_mm, ch[g], o, ib);
+            }
+            on_word((uint32_t)i, mi, is_mm, live, L, ch, ow);
+        }
+        bw0 = bw1;
+        bw1 = bw2;
+        bw2 = window_load(tw, e - 256, n, lane);
+        e -= 64;
+    }
+}
+
+// Shortened tapes and child records of a wavefront's ambiguous cells
+// (Bamb[g]: the ambiguous cells of group g; gx/gy/gz: this lane's cells'
+// coords in units of the next level's nodes).  The parent tape is at toff
+// (n words, mm MIN/MAX whose choice rows are in scr); logw / logL are the
+// wavefront's log (cap entries: index and row, then G ballots each).
+// child[g] gets the child id of this lane's cell of group g (kNoChild for
+// cells that are not ambiguous or when the level ran out of capacity).
+template <int MAXS, int G, class Reserve>
+__device__ __forceinline__ void shorten_children(const LevelArgs& a, uint32_t lane,
+                                                 uint32_t toff, uint32_t n, uint32_t mm,
+                                                 const uint4* scr, uint2* logw, uint2* logL,
+                                                 uint32_t cap, const uint64_t (&Bamb)[G],
+                                                 const uint32_t (&gx)[G],
+                                                 const uint32_t (&gy)[G],
+                                                 const uint32_t (&gz)[G], uint64_t child_base,
+                                                 const Reserve& reserve, uint32_t (&child)[G]) {
+    const uint32_t* tw = a.pool + toff + 1;
+    bool amb[G];
+    uint32_t namb = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        amb[g] = ((Bamb[g] >> lane) & 1u) != 0;
+        child[g] = kNoChild;
+        namb += (uint32_t)__popcll(Bamb[g]);
+    }
+    // ---- the walk: kept counts, classes, log ----
+    uint32_t kept[G];
+    uint64_t eq[G];                                    // cells deciding as this lane's so far
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        kept[g] = 0;
+        eq[g] = Bamb[g];
+    }
+    uint32_t nlog = 0;
+    if (namb != 0) {
+        backward_walk<MAXS, G>(
+            tw, n, mm, scr, amb, lane,
+            [&](uint32_t i, uint32_t mi, bool is_mm, const bool (&live)[G],
+                const uint64_t (&L)[G], const uint32_t (&ch)[G], const uint32_t (&)[G]) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    kept[g] += live[g] ? 1u : 0u;
+                    // cells of one class have the same live slots, so they
+                    // can part only where a live MIN/MAX chose differently
+                    if (is_mm) {
+                        const bool f = live[g] && (ch[g] & MM_FIRST) != 0;
+                        const bool sc = live[g] && (ch[g] & MM_SECOND) != 0;
+                        const uint64_t K1 = __ballot(f), K2 = __ballot(sc);
+                        if (live[g]) eq[g] &= (f ? K1 : ~K1) & (sc ? K2 : ~K2);
+                    }
+                }
+                if (nlog < cap && lane == 0) {
+                    logw[nlog] = make_uint2(i, mi);
+#pragma unroll
+                    for (int g = 0; g < G; ++g)
+                        logL[nlog * G + g] = make_uint2((uint32_t)L[g], (uint32_t)(L[g] >> 32));
+                }
+                ++nlog;
+            });
+    }
+    // ---- owners and one exact reservation ----
+    bool own[G];
+    uint32_t lead[G], off[G], words = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        lead[g] = amb[g] ? (uint32_t)__builtin_ctzll(eq[g] & Bamb[g]) : lane;
+        own[g] = amb[g] && lead[g] == lane && kept[g] < n;
+        const uint32_t need = own[g] ? kept[g] + 1u : 0u;
+        const uint32_t incl = wave_incl_scan(need, lane);
+        off[g] = words + incl - need;
+        words += rdlane(incl, 63);
+    }
+    const uint64_t old =
+        reserve((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
+                ((uint64_t)namb << kAllocShift) | (uint64_t)words, lane);
+    if (namb == 0) return;
+    const uint32_t cbase = alloc_nodes(old);
+    const uint64_t pbase = child_base + alloc_words(old);
+    const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
+    const bool pool_ok = pbase + words <= a.pool_cap;
+    if (!(nodes_ok && pool_ok)) {
+        if (lane == 0)
+            atomicOr(a.counters + CNT_OVERFLOW,
+                     ((nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL)) << (2 * a.level));
+        return;
+    }
+    uint32_t reg[G], lreg[G];
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        reg[g] = (uint32_t)(pbase + off[g]);
+        lreg[g] = (uint32_t)__shfl((int)reg[g], (int)lead[g]);
+    }
+    // ---- the owners' tapes ----
+    if (nlog <= cap) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            uint64_t M = __ballot(own[g]);
+            while (M != 0) {
+                const uint32_t ld = (uint32_t)__builtin_ctzll(M);
+                M &= M - 1u;
+                const uint32_t kL = rdlane(kept[g], ld);
+                uint32_t* out = a.pool + rdlane(reg[g], ld);
+                if (lane == 0) out[0] = kL;
+                // the r-th kept entry of the log (tape order reversed) is
+                // word kL - r of the tape
+                uint32_t r = 0;
+                for (uint32_t e0 = 0; e0 < nlog; e0 += 64) {
+                    const uint32_t e = e0 + lane < nlog ? e0 + lane : nlog - 1u;
+                    const uint2 ew = logw[e];
+                    const uint2 el = logL[e * G + g];
+                    const uint64_t Lm = ((uint64_t)el.y << 32) | el.x;
+                    const bool kp = e0 + lane < nlog && ((Lm >> ld) & 1u) != 0;
+                    const uint64_t b = __ballot(kp);
+                    if (kp) {
+                        uint32_t wd = tw[ew.x];
+                        const uint32_t op = wd & 0xFFu;
+                        if (op == OP_MIN || op == OP_MAX) {
+                            const uint4 v = scr[ew.y * G + g];
+                            const uint64_t Fm = ((uint64_t)v.y << 32) | v.x;
+                            const uint64_t Sm = ((uint64_t)v.w << 32) | v.z;
+                            const uint32_t c = (uint32_t)((Fm >> ld) & 1u) |
+                                               ((uint32_t)((Sm >> ld) & 1u) << 1);
+                            wd = shortened_word(wd, true, c, slot_field<MAXS>((wd >> 8) & 0xFFu),
+                                                slot_field<MAXS>(wd >> 24));
+                        }
+                        out[kL - r - lane_rank(b)] = wd;
+                    }
+                    r += (uint32_t)__popcll(b);
+                }
+            }
+        }
+    } else {
+        uint32_t r[G];
+#pragma unroll
+        for (int g = 0; g < G; ++g) r[g] = 0;
+        backward_walk<MAXS, G>(
+            tw, n, mm, scr, amb, lane,
+            [&](uint32_t, uint32_t, bool, const bool (&live)[G], const uint64_t (&)[G],
+                const uint32_t (&)[G], const uint32_t (&ow)[G]) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    if (own[g] && live[g]) a.pool[reg[g] + kept[g] - r[g]] = ow[g];
+                    r[g] += live[g] ? 1u : 0u;
+                }
+            });
+#pragma unroll
+        for (int g = 0; g < G; ++g)
+            if (own[g]) a.pool[reg[g]] = kept[g];
+    }
+    // ---- child records ----
+    uint32_t rank0 = 0;
+#pragma unroll
+    for (int g = 0; g < G; ++g) {
+        if (amb[g]) {
+            child[g] = cbase + rank0 + lane_rank(Bamb[g]);
+            uint32_t* crec = a.child_nodes + (size_t)child[g] * a.child_ns;
+            crec[0] = kept[g] < n ? lreg[g] : toff;
+            crec[1] = gx[g];
+            crec[2] = gy[g];
+            crec[3] = gz[g];
+        }
+        rank0 += (uint32_t)__popcll(Bamb[g]);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Non-last level: interval classification of every cell, min/max choice
 // recording, and tape shortening of ambiguous cells
 // (vulkan/shaders/voxelizer/main.comp:88-149, vulkan/shaders/tape/
@@ -163,18 +457,17 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 // the cell whose bit index is 64*sub + l.  Mask words are built by ballot and
 // written once (no atomics).  Each MIN/MAX records two 64-bit ballots (lanes
 // whose choice includes FIRST / SECOND) in the wave's scratch row; the
-// backward liveness pass of the shortening reads them back.  Ambiguous lanes
-// reserve their child id and an (n+1)-word tape region with one atomic per
-// wave, then write the kept instructions from the end of the region towards
-// its start, so the shortened tape [k, instrs] is contiguous without a
-// second pass.  If nothing was removed the child reuses the parent's tape
+// backward liveness pass of the shortening reads them back
+// (shorten_children: one tape per class of cells that keep the same words,
+// written by ballot and prefix-sum compaction into exactly reserved
+// regions).  If nothing was removed the child reuses the parent's tape
 // (shorten.comp:188-191).
 // ---------------------------------------------------------------------------
 template <int MAXS, class Reserve>
 __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                uint64_t child_base, uint32_t lane, uint4* scr,
+                                               uint2* logw, uint2* logL, uint32_t cap,
                                                const Reserve& reserve) {
-    constexpr int AW = (MAXS + 31) / 32;
     const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
     ctape_t hdr = as_const(rec);
@@ -284,104 +577,12 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     store_mask64(rec + 4 + sub * 2, Bfull, lane);
     store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
 
-    uint32_t child_id = kNoChild;
-    // one packed reservation: namb child ids and namb*stride pool words
-    const uint32_t namb = (uint32_t)__popcll(Bamb);
-    const uint32_t stride = n + 1;
-    const uint64_t old =
-        reserve((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
-                ((uint64_t)namb << kAllocShift) | ((uint64_t)namb * stride), lane);
-    if (Bamb != 0) {
-        const uint32_t rank = lane_rank(Bamb);
-        const uint32_t cbase = alloc_nodes(old);
-        const uint64_t pbase = child_base + alloc_words(old);
-        const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
-        const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
-        if (!(nodes_ok && pool_ok)) {
-            if (lane == 0)
-                atomicOr(a.counters + CNT_OVERFLOW,
-                         ((nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL)) << (2 * a.level));
-        } else {
-            // ---- backward pass: liveness + compaction (shorten.comp:164-223) ----
-            // Runs with the whole wavefront active (the tape windows are
-            // loaded by every lane); only ambiguous lanes keep anything.
-            const uint64_t mybase = pbase + (uint64_t)rank * stride;
-            uint32_t* out = a.pool + mybase;
-            uint32_t act[AW];
-#pragma unroll
-            for (int k = 0; k < AW; ++k) act[k] = 0;
-            act[0] = amb ? 1u : 0u;                        // slot 0 is the result
-            uint32_t kept = 0;
-            int32_t jm = (int32_t)mm - 1;
-            uint4 row = scr[jm > 0 ? jm : 0];              // next choice row, read ahead
-            int32_t e = (int32_t)n;                        // window = words [e-64, e)
-            uint32_t bw0 = window_load(tw, e - 64, n, lane);
-            uint32_t bw1 = window_load(tw, e - 128, n, lane);
-            uint32_t bw2 = window_load(tw, e - 192, n, lane);
-            while (e > 0) {
-                const int32_t s = e - 64;
-                const int32_t lo = s > 0 ? s : 0;
-                for (int32_t i = e - 1; i >= lo; --i) {
-                    const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
-                    const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
-                    const bool is_mm = (op == OP_MIN || op == OP_MAX);
-                    uint32_t ch = MM_BOTH;
-                    if (is_mm) {
-                        const uint4 v = row;
-                        --jm;
-                        row = scr[jm > 0 ? jm : 0];
-                        const uint64_t F = ((uint64_t)v.y << 32) | v.x;
-                        const uint64_t S = ((uint64_t)v.w << 32) | v.z;
-                        ch = (uint32_t)((F >> lane) & 1u) |
-                             ((uint32_t)((S >> lane) & 1u) << 1);
-                    }
-                    const uint32_t obit = 1u << (o & 31u);
-                    const bool live = (act[o >> 5] & obit) != 0;
-                    // a word no lane keeps changes nothing (common once the
-                    // cells of a chunk agree on their choices)
-                    if (__ballot(live) == 0) continue;
-                    const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu);
-                    const uint32_t ib = slot_field<MAXS>(w >> 24);
-                    // operands a kept word reads: none for CONST, both for
-                    // ADD/SUB/MUL/DIV, the chosen ones for MIN/MAX, else A
-                    const uint32_t sel = op == OP_CONST ? MM_NONE
-                                         : (op - OP_ADD <= OP_DIV - OP_ADD) ? MM_BOTH
-                                         : (is_mm ? ch : MM_FIRST);
-                    // lane masks instead of branches: clear the output slot
-                    // first, then set the operand slots (shorten.comp:179)
-                    const uint32_t keep = live ? ~0u : 0u;
-                    act[o >> 5] &= ~(obit & keep);
-                    act[ia >> 5] |= ((sel & MM_FIRST) ? (1u << (ia & 31u)) : 0u) & keep;
-                    act[ib >> 5] |= ((sel & MM_SECOND) ? (1u << (ib & 31u)) : 0u) & keep;
-                    const uint32_t copy_a = (w & ~0xFFu) | OP_COPY;
-                    const uint32_t copy_b = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
-                    const uint32_t ow = !is_mm ? w
-                                        : (ch == MM_FIRST ? copy_a
-                                                          : (ch == MM_SECOND ? copy_b : w));
-                    if (live) out[n - kept] = ow;
-                    kept += live ? 1u : 0u;
-                }
-                bw0 = bw1;
-                bw1 = bw2;
-                bw2 = window_load(tw, e - 256, n, lane);
-                e -= 64;
-            }
-            if (amb) {
-                uint32_t final_off = toff;
-                if (kept < n) {
-                    out[n - kept] = kept;
-                    final_off = (uint32_t)(mybase + n - kept);
-                }
-                child_id = cbase + rank;
-                uint32_t* crec = a.child_nodes + (size_t)child_id * a.child_ns;
-                crec[0] = final_off;
-                crec[1] = gx;
-                crec[2] = gy;
-                crec[3] = gz;
-            }
-        }
-    }
-    rec[4 + 2 * W + idx] = child_id;
+    const uint64_t Bambs[1] = {Bamb};
+    const uint32_t cgx[1] = {gx}, cgy[1] = {gy}, cgz[1] = {gz};
+    uint32_t child[1];
+    shorten_children<MAXS, 1>(a, lane, toff, n, mm, scr, logw, logL, cap, Bambs, cgx, cgy, cgz,
+                              child_base, reserve, child);
+    rec[4 + 2 * W + idx] = child[0];
     __builtin_amdgcn_s_setprio(0);
 }
 
@@ -395,8 +596,8 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
 template <int MAXS, class Reserve>
 __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                 uint64_t child_base, uint32_t lane, uint4* scr,
+                                                uint2* logw, uint2* logL, uint32_t cap,
                                                 const Reserve& reserve) {
-    constexpr int AW = (MAXS + 31) / 32;
     const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
     ctape_t hdr = as_const(rec);
@@ -521,126 +722,14 @@ __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t nod
     store_mask64(rec + 4 + W + sub * 4, Bamb0, lane);
     store_mask64(rec + 4 + W + sub * 4 + 2, Bamb1, lane);
 
-    uint32_t child0 = kNoChild, child1 = kNoChild;
-    // one packed reservation for both groups: group 0's children first
-    const uint32_t namb0 = (uint32_t)__popcll(Bamb0), namb = namb0 + (uint32_t)__popcll(Bamb1);
-    const uint32_t stride = n + 1;
-    const uint64_t old =
-        reserve((unsigned long long*)(a.counters + CNT_ALLOC + 2 * (a.level + 1)),
-                ((uint64_t)namb << kAllocShift) | ((uint64_t)namb * stride), lane);
-    if (namb != 0) {
-        const uint32_t rank0 = lane_rank(Bamb0), rank1 = namb0 + lane_rank(Bamb1);
-        const uint32_t cbase = alloc_nodes(old);
-        const uint64_t pbase = child_base + alloc_words(old);
-        const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
-        const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
-        if (!(nodes_ok && pool_ok)) {
-            if (lane == 0)
-                atomicOr(a.counters + CNT_OVERFLOW,
-                         ((nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL)) << (2 * a.level));
-        } else {
-            // ---- backward pass: liveness + compaction (shorten.comp:164-223) ----
-            const uint64_t base0 = pbase + (uint64_t)rank0 * stride;
-            const uint64_t base1 = pbase + (uint64_t)rank1 * stride;
-            uint32_t* out0 = a.pool + base0;
-            uint32_t* out1 = a.pool + base1;
-            uint32_t act0[AW], act1[AW];
-#pragma unroll
-            for (int k = 0; k < AW; ++k) act0[k] = act1[k] = 0;
-            act0[0] = amb0 ? 1u : 0u;                      // slot 0 is the result
-            act1[0] = amb1 ? 1u : 0u;
-            uint32_t kept0 = 0, kept1 = 0;
-            int32_t jm = (int32_t)mm - 1;
-            uint4 row0 = scr[2 * (jm > 0 ? jm : 0)], row1 = scr[2 * (jm > 0 ? jm : 0) + 1];
-            int32_t e = (int32_t)n;
-            uint32_t bw0 = window_load(tw, e - 64, n, lane);
-            uint32_t bw1 = window_load(tw, e - 128, n, lane);
-            uint32_t bw2 = window_load(tw, e - 192, n, lane);
-            while (e > 0) {
-                const int32_t s = e - 64;
-                const int32_t lo = s > 0 ? s : 0;
-                for (int32_t i = e - 1; i >= lo; --i) {
-                    const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
-                    const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
-                    const bool is_mm = (op == OP_MIN || op == OP_MAX);
-                    uint32_t ch0 = MM_BOTH, ch1 = MM_BOTH;
-                    if (is_mm) {
-                        const uint4 v0 = row0, v1 = row1;
-                        --jm;
-                        row0 = scr[2 * (jm > 0 ? jm : 0)];
-                        row1 = scr[2 * (jm > 0 ? jm : 0) + 1];
-                        ch0 = (uint32_t)(((((uint64_t)v0.y << 32) | v0.x) >> lane) & 1u) |
-                              ((uint32_t)(((((uint64_t)v0.w << 32) | v0.z) >> lane) & 1u) << 1);
-                        ch1 = (uint32_t)(((((uint64_t)v1.y << 32) | v1.x) >> lane) & 1u) |
-                              ((uint32_t)(((((uint64_t)v1.w << 32) | v1.z) >> lane) & 1u) << 1);
-                    }
-                    const uint32_t obit = 1u << (o & 31u);
-                    const bool live0 = (act0[o >> 5] & obit) != 0;
-                    const bool live1 = (act1[o >> 5] & obit) != 0;
-                    if (__ballot(live0 || live1) == 0) continue;
-                    const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu);
-                    const uint32_t ib = slot_field<MAXS>(w >> 24);
-                    const uint32_t base_sel = op == OP_CONST ? MM_NONE
-                                              : (op - OP_ADD <= OP_DIV - OP_ADD) ? MM_BOTH
-                                                                                 : MM_FIRST;
-                    const uint32_t sel0 = is_mm ? ch0 : base_sel;
-                    const uint32_t sel1 = is_mm ? ch1 : base_sel;
-                    const uint32_t abit = 1u << (ia & 31u), bbit = 1u << (ib & 31u);
-                    const uint32_t keep0 = live0 ? ~0u : 0u, keep1 = live1 ? ~0u : 0u;
-                    act0[o >> 5] &= ~(obit & keep0);
-                    act1[o >> 5] &= ~(obit & keep1);
-                    act0[ia >> 5] |= ((sel0 & MM_FIRST) ? abit : 0u) & keep0;
-                    act1[ia >> 5] |= ((sel1 & MM_FIRST) ? abit : 0u) & keep1;
-                    act0[ib >> 5] |= ((sel0 & MM_SECOND) ? bbit : 0u) & keep0;
-                    act1[ib >> 5] |= ((sel1 & MM_SECOND) ? bbit : 0u) & keep1;
-                    const uint32_t copy_a = (w & ~0xFFu) | OP_COPY;
-                    const uint32_t copy_b = OP_COPY | (o << 8) | (ib << 16) | (ib << 24);
-                    const uint32_t ow0 = !is_mm ? w
-                                         : (ch0 == MM_FIRST ? copy_a
-                                                            : (ch0 == MM_SECOND ? copy_b : w));
-                    const uint32_t ow1 = !is_mm ? w
-                                         : (ch1 == MM_FIRST ? copy_a
-                                                            : (ch1 == MM_SECOND ? copy_b : w));
-                    if (live0) out0[n - kept0] = ow0;
-                    if (live1) out1[n - kept1] = ow1;
-                    kept0 += live0 ? 1u : 0u;
-                    kept1 += live1 ? 1u : 0u;
-                }
-                bw0 = bw1;
-                bw1 = bw2;
-                bw2 = window_load(tw, e - 256, n, lane);
-                e -= 64;
-            }
-            if (amb0) {
-                uint32_t final_off = toff;
-                if (kept0 < n) {
-                    out0[n - kept0] = kept0;
-                    final_off = (uint32_t)(base0 + n - kept0);
-                }
-                child0 = cbase + rank0;
-                uint32_t* crec = a.child_nodes + (size_t)child0 * a.child_ns;
-                crec[0] = final_off;
-                crec[1] = gx0;
-                crec[2] = gy0;
-                crec[3] = gz0;
-            }
-            if (amb1) {
-                uint32_t final_off = toff;
-                if (kept1 < n) {
-                    out1[n - kept1] = kept1;
-                    final_off = (uint32_t)(base1 + n - kept1);
-                }
-                child1 = cbase + rank1;
-                uint32_t* crec = a.child_nodes + (size_t)child1 * a.child_ns;
-                crec[0] = final_off;
-                crec[1] = gx1;
-                crec[2] = gy1;
-                crec[3] = gz1;
-            }
-        }
-    }
-    rec[4 + 2 * W + idx0] = child0;
-    rec[4 + 2 * W + idx1] = child1;
+    // group 0's children first
+    const uint64_t Bambs[2] = {Bamb0, Bamb1};
+    const uint32_t cgx[2] = {gx0, gx1}, cgy[2] = {gy0, gy1}, cgz[2] = {gz0, gz1};
+    uint32_t child[2];
+    shorten_children<MAXS, 2>(a, lane, toff, n, mm, scr, logw, logL, cap, Bambs, cgx, cgy, cgz,
+                              child_base, reserve, child);
+    rec[4 + 2 * W + idx0] = child[0];
+    rec[4 + 2 * W + idx1] = child[1];
     __builtin_amdgcn_s_setprio(0);
 }
 
@@ -652,12 +741,12 @@ __device__ __forceinline__ uint64_t child_tape_base(const uint32_t* counters, ui
     return b;
 }
 
-// Slot files of up to 12 slots are built for 8 wavefronts per SIMD (at most
-// 64 VGPRs and a scalar budget to match); larger ones keep the compiler's
-// choice.
+// One-cell chunks with slot files of up to 12 slots are built for 7
+// wavefronts per SIMD (at most 72 VGPRs and a scalar budget to match), two-
+// cell chunks for 4; larger files keep the compiler's choice.
 template <int MAXS, int KC = 1>
 __global__ __launch_bounds__(kLevelBlock)
-__attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 8 : 1)))) void k_level_interval(LevelArgs a) {
+__attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 7 : 1)))) void k_level_interval(LevelArgs a) {
     // min/max choice ballots of this wavefront's current chunk: mm_stride rows
     // of {FIRST lanes, SECOND lanes} (two 64-bit masks) per wavefront
     extern __shared__ uint4 lds_choice[];
@@ -671,6 +760,10 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 8 : 1)))) void k_
     const uint32_t cpn = a.dim * a.dim * a.dim / (64 * KC);
     const uint32_t total = n_nodes * cpn;
     uint4* scr = lds_choice + (size_t)wib * a.mm_stride * KC;
+    // the shortening log after every wavefront's choice rows (log_entry_bytes)
+    uint2* logw = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * a.mm_stride * KC) +
+                  (size_t)wib * a.log_cap * (1 + KC);
+    uint2* logL = logw + a.log_cap;
     // rounds of kLevelBlock/64 consecutive chunks per block, one reservation
     // atomic per round; a wavefront past the end still takes part
     constexpr uint32_t WPB = kLevelBlock / 64;
@@ -682,8 +775,12 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 8 : 1)))) void k_
         const uint32_t chunk = g * WPB + wib;
         if (chunk < total) {
             const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
-            if constexpr (KC == 2) interval_chunk2<MAXS>(a, node, sub, child_base, lane, scr, br);
-            else interval_chunk<MAXS>(a, node, sub, child_base, lane, scr, br);
+            if constexpr (KC == 2)
+                interval_chunk2<MAXS>(a, node, sub, child_base, lane, scr, logw, logL, a.log_cap,
+                                      br);
+            else
+                interval_chunk<MAXS>(a, node, sub, child_base, lane, scr, logw, logL, a.log_cap,
+                                     br);
         } else {
             br(ctr, 0, lane);
         }
@@ -711,12 +808,12 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 8 : 1)))) void k_
 // flag of its output slot (the slot's later values were handled in later
 // passes), and if it was set, marks the instruction kept and sets the flags
 // of its chosen operands.  Kept instructions are then written in tape
-// order, LPC at a time, at the same place in the cell's (n+1)-word region
-// as the per-cell kernel writes them.
+// order, LPC at a time, by ballot and prefix-sum compaction.
 //
 // A block of C cells owns C consecutive mask bits (a 16- or 8-bit slice of
-// a mask word, stored as such) and reserves the child ids and tape regions
-// of its ambiguous cells with one atomic.
+// a mask word, stored as such) and reserves the child ids and the exact
+// tape words of its ambiguous cells ([k, k words] each, none where nothing
+// was removed) with one atomic once their backward passes have counted k.
 //
 // LDS: constant table {c, c}[n_const] then instruction ids u16[n_const];
 // per cell, values float2[n_slot], flags u8[n] (choice | kept << 2, rounded
@@ -739,7 +836,8 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
     constexpr uint32_t G = 64 / LPC;                     // cells per wavefront
     extern __shared__ __align__(16) unsigned char root_lds[];
     __shared__ uint32_t blk_flags[C];
-    __shared__ uint32_t blk_amb;
+    __shared__ uint32_t blk_need[C];                     // words of each cell's tape, then its offset
+    __shared__ uint32_t blk_amb, blk_words;
     __shared__ unsigned long long blk_old;
     const LevelArgs& a = r.a;
     // global-typed views of the argument pointers: plain (not flat) memory
@@ -894,20 +992,72 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
     const bool empty = R.x > 0.0f;
     const bool amb = !full && !empty;
 
-    // ---- block: mask slices and one reservation for the block's children ----
-    const uint32_t stride = n + 1;
-    if (sl == 0) blk_flags[cib] = (full ? 1u : 0u) | (amb ? 2u : 0u);
+    // ---- backward pass: instructions the result reaches through its choices ----
+    uint32_t k = 0;                                      // kept instructions of this cell
+    if (amb) {
+        if (sl == 0) mark(r.res);
+        // passes P-1 .. 0, entries loaded 3 passes ahead
+        auto backward = [&](const uint2 e) {
+            if ((e.x & 0xFu) != kRootNop) {
+                const uint32_t op = e.x & 0xFu, so = (e.x >> 4) & 0xFFFu, i = e.x >> 16;
+                const uint32_t lv = live[so];
+                live[so] = 0;
+                if (lv) {
+                    const uint32_t f = flags[i];
+                    flags[i] = (uint8_t)(f | 4u);
+                    uint32_t sel = MM_FIRST;
+                    if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
+                        sel = MM_BOTH;
+                    else if (op == OP_MIN || op == OP_MAX)
+                        sel = f & 3u;
+                    if (sel & MM_FIRST) mark(e.y & 0xFFFFu);
+                    if (sel & MM_SECOND) mark(e.y >> 16);
+                }
+            }
+        };
+        if (P) {
+            auto ldr = [&](uint32_t t) { return ld(t < P ? P - 1u - t : 0u); };
+            uint2 q0 = ldr(0), q1 = ldr(1), q2 = ldr(2), q3 = ldr(3);
+            for (uint32_t t = 0; t < P; t += 4) {
+                backward(q0);
+                q0 = ldr(t + 4);
+                backward(q1);
+                q1 = ldr(t + 5);
+                backward(q2);
+                q2 = ldr(t + 6);
+                backward(q3);
+                q3 = ldr(t + 7);
+            }
+        }
+        for (uint32_t s = 0; s < n; s += LPC)
+            k += (uint32_t)__popcll(group_bits(
+                __ballot(s + sl < n && (flags[s + sl < n ? s + sl : n - 1u] & 4u) != 0)));
+    }
+
+    // ---- block: mask slices and one exact reservation for the block's
+    // children: a child id per ambiguous cell and [k, k words] per shortened
+    // tape (none when nothing was removed: the child reuses the root tape,
+    // shorten.comp:188-191) ----
+    if (sl == 0) {
+        blk_flags[cib] = (full ? 1u : 0u) | (amb ? 2u : 0u);
+        blk_need[cib] = (amb && k < n) ? k + 1u : 0u;
+    }
     __syncthreads();
     if (threadIdx.x < 64) {
         const uint32_t f = lane < C ? blk_flags[lane] : 0u;
         const uint64_t bf = __ballot((f & 1u) != 0), ba = __ballot((f & 2u) != 0);
         const uint64_t namb = (uint64_t)__popcll(ba);
+        const uint32_t nd = lane < C ? blk_need[lane] : 0u;
+        const uint32_t incl = wave_incl_scan(nd, lane);
+        const uint64_t words = rdlane(incl, 63);
+        if (lane < C) blk_need[lane] = incl - nd;        // the cell's offset in the block's words
         if (lane == 0) {
             store_mask_slice<C>(rec + 4, blk, (uint32_t)bf);
             store_mask_slice<C>(rec + 4 + W, blk, (uint32_t)ba);
             blk_amb = (uint32_t)ba;
+            blk_words = (uint32_t)words;
             unsigned long long* ctr = (unsigned long long*)(a.counters + CNT_ALLOC + 2);
-            blk_old = namb ? atomicAdd(ctr, (namb << kAllocShift) | (namb * stride)) : 0ull;
+            blk_old = namb ? atomicAdd(ctr, (namb << kAllocShift) | words) : 0ull;
         }
     }
     __syncthreads();
@@ -921,56 +1071,17 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
         const uint32_t cbase = alloc_nodes(old);
         const uint64_t pbase = child_tape_base(a.counters, 0) + alloc_words(old);
         const bool nodes_ok = (uint64_t)cbase + namb <= a.child_cap;
-        const bool pool_ok = pbase + (uint64_t)namb * stride <= a.pool_cap;
+        const bool pool_ok = pbase + blk_words <= a.pool_cap;
         if (!(nodes_ok && pool_ok)) {
             if (sl == 0)
                 atomicOr(a.counters + CNT_OVERFLOW,
                          (nodes_ok ? 0u : OVF_NODES) | (pool_ok ? 0u : OVF_POOL));
         } else {
-            // ---- backward pass: instructions the result reaches through its choices ----
-            if (sl == 0) mark(r.res);
-            // passes P-1 .. 0, entries loaded 3 passes ahead
-            auto backward = [&](const uint2 e) {
-                if ((e.x & 0xFu) != kRootNop) {
-                    const uint32_t op = e.x & 0xFu, so = (e.x >> 4) & 0xFFFu, i = e.x >> 16;
-                    const uint32_t lv = live[so];
-                    live[so] = 0;
-                    if (lv) {
-                        const uint32_t f = flags[i];
-                        flags[i] = (uint8_t)(f | 4u);
-                        uint32_t sel = MM_FIRST;
-                        if (op == OP_ADD || op == OP_SUB || op == OP_MUL || op == OP_DIV)
-                            sel = MM_BOTH;
-                        else if (op == OP_MIN || op == OP_MAX)
-                            sel = f & 3u;
-                        if (sel & MM_FIRST) mark(e.y & 0xFFFFu);
-                        if (sel & MM_SECOND) mark(e.y >> 16);
-                    }
-                }
-            };
-            if (P) {
-                auto ldr = [&](uint32_t t) { return ld(t < P ? P - 1u - t : 0u); };
-                uint2 q0 = ldr(0), q1 = ldr(1), q2 = ldr(2), q3 = ldr(3);
-                for (uint32_t t = 0; t < P; t += 4) {
-                    backward(q0);
-                    q0 = ldr(t + 4);
-                    backward(q1);
-                    q1 = ldr(t + 5);
-                    backward(q2);
-                    q2 = ldr(t + 6);
-                    backward(q3);
-                    q3 = ldr(t + 7);
-                }
-            }
             // ---- kept instructions in tape order (shorten.comp:61-88, 188-191) ----
-            uint32_t k = 0;
-            for (uint32_t s = 0; s < n; s += LPC)
-                k += (uint32_t)__popcll(group_bits(
-                    __ballot(s + sl < n && (flags[s + sl < n ? s + sl : n - 1u] & 4u) != 0)));
-            const uint64_t mybase = pbase + (uint64_t)rank * stride;
+            const uint64_t mybase = pbase + blk_need[cib];
             uint32_t final_off = toff;
             if (k < n) {
-                gword* out = (gword*)a.pool + mybase + (n - k);
+                gword* out = (gword*)a.pool + mybase;
                 if (sl == 0) out[0] = k;
                 // the tape words of kDeep windows are loaded together
                 uint32_t j = 1;
@@ -1004,7 +1115,7 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
                     j += (uint32_t)__popcll(b);
                     }
                 }
-                final_off = (uint32_t)(mybase + n - k);
+                final_off = (uint32_t)mybase;
             }
             child_id = cbase + rank;
             if (sl == 0) {
@@ -1269,15 +1380,6 @@ namespace {
 
 typedef const __attribute__((address_space(4))) LevelArgs* cargs_t;
 
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, uint32_t lane) {
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t u = (uint32_t)__shfl_up((int)v, (unsigned)d);
-        if (lane >= (uint32_t)d) v += u;
-    }
-    return v;
-}
-
 // Chunks of shape `lane` at this launch's level, and the first pool word of
 // its child tapes.  Only counts that no wavefront of this launch modifies
 // enter the plan (the node count of this level was final when the previous
@@ -1317,7 +1419,7 @@ __device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
 
 template <int MAXS>
 __global__ __launch_bounds__(kLevelBlock)
-__attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval_batch(BatchArgs b) {
+__attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 7 : 1))) void k_level_interval_batch(BatchArgs b) {
     extern __shared__ uint4 lds_choice[];
     const uint32_t lane = lane_id();
     const uint32_t wib = uni(threadIdx.x >> 6);
@@ -1327,6 +1429,9 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval_b
     const uint32_t incl = wave_incl_scan(plan_lane<false>(b, lane, &cbase), lane);
     const uint32_t total = rdlane(incl, 63);
     uint4* scr = lds_choice + (size_t)wib * b.mm_stride;
+    uint2* logw = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * b.mm_stride) +
+                  (size_t)wib * b.log_cap * 2;
+    uint2* logL = logw + b.log_cap;
     cargs_t shapes = (cargs_t)b.shapes;
     uint32_t cur = 0xFFFFFFFFu, start = 0, cpn = 1;
     uint64_t cb = 0;
@@ -1348,7 +1453,7 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 8 : 1))) void k_level_interval_b
         if (skip) continue;
         const uint32_t local = chunk - start;
         const uint32_t node = local / cpn, sub = local - node * cpn;
-        interval_chunk<MAXS>(a, node, sub, cb, lane, scr, WaveReserve{});
+        interval_chunk<MAXS>(a, node, sub, cb, lane, scr, logw, logL, b.log_cap, WaveReserve{});
     }
 }
 
@@ -1878,15 +1983,38 @@ bool two_cells_per_lane() {
     return !(e && e[0] == '1');
 }
 
-size_t interval_lds_bytes(uint32_t mm_stride) {
-    return (size_t)(kLevelBlock / 64) * mm_stride * sizeof(uint4);
+// Shortening log of a wavefront: an index / choice-row pair and KC ballots
+// per entry.
+size_t log_entry_bytes(uint32_t kc) { return 8u * (1u + kc); }
+
+size_t interval_lds_bytes(uint32_t mm_stride, uint32_t kc, uint32_t cap) {
+    return (size_t)(kLevelBlock / 64) *
+           ((size_t)mm_stride * kc * sizeof(uint4) + (size_t)cap * log_entry_bytes(kc));
+}
+
+// Log entries per wavefront: what fits in block_budget bytes of LDS per
+// block (the block's share of a CU at the occupancy the kernel is built
+// for) after the choice rows, between 64 and 4096, and within the device's
+// LDS per block.  FREP_LOG_CAP=n sets it (read at each launch; a tiny log
+// makes every chunk take the second walk).
+uint32_t log_cap_for(uint32_t mm_stride, uint32_t kc, size_t block_budget) {
+    const size_t per_wave = block_budget / (kLevelBlock / 64);
+    const size_t rows = (size_t)mm_stride * kc * sizeof(uint4);
+    size_t cap = per_wave > rows ? (per_wave - rows) / log_entry_bytes(kc) : 0;
+    cap = std::min<size_t>(std::max<size_t>(cap, 64), 4096);
+    if (const char* e = std::getenv("FREP_LOG_CAP")) {
+        const long v = std::atol(e);
+        if (v >= 1 && v <= 4096) cap = (size_t)v;
+    }
+    while (cap > 1 && interval_lds_bytes(mm_stride, kc, (uint32_t)cap) > kMaxChoiceLds) cap /= 2;
+    return (uint32_t)cap;
 }
 
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                                  uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
                                  uint32_t nodes_hint) {
-    const size_t lds = interval_lds_bytes(a.mm_stride);
-    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
+    if (interval_lds_bytes(a.mm_stride, 1, 0) > kMaxChoiceLds) return hipErrorInvalidValue;
+    LevelArgs la = a;
     // two cells per lane (one choice row per group, so twice the LDS) for
     // files of up to 16 slots on nodes of a multiple of 128 cells, unless
     // the level is known to be small: below 16 one-cell chunks per CU, the
@@ -1894,34 +2022,37 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
     // cells ran 50 us with one cell per lane, 55.5 us with two)
     const uint32_t cells = a.dim * a.dim * a.dim;
     const bool big = nodes_hint == 0 || (uint64_t)nodes_hint * (cells / 64) >= 16ull * cu_count;
-    if (maxs <= 16 && cells % 128 == 0 && 2 * lds <= kMaxChoiceLds && big &&
-        two_cells_per_lane()) {
-        if (maxs == 8)
-            k_level_interval<8, 2><<<interval_grid<8, 2>(cu_count, max_grid, 2 * lds),
-                                     kLevelBlock, 2 * lds, st>>>(a);
-        else if (slot_count <= 12)
-            k_level_interval<12, 2><<<interval_grid<12, 2>(cu_count, max_grid, 2 * lds),
-                                      kLevelBlock, 2 * lds, st>>>(a);
-        else
-            k_level_interval<16, 2><<<interval_grid<16, 2>(cu_count, max_grid, 2 * lds),
-                                      kLevelBlock, 2 * lds, st>>>(a);
-        return hipGetLastError();
+    if (maxs <= 16 && cells % 128 == 0 && big && two_cells_per_lane()) {
+        la.log_cap = log_cap_for(a.mm_stride, 2, 32 * 1024);
+        const size_t lds = interval_lds_bytes(a.mm_stride, 2, la.log_cap);
+        if (lds <= kMaxChoiceLds) {
+            if (maxs == 8)
+                k_level_interval<8, 2><<<interval_grid<8, 2>(cu_count, max_grid, lds),
+                                         kLevelBlock, lds, st>>>(la);
+            else if (slot_count <= 12)
+                k_level_interval<12, 2><<<interval_grid<12, 2>(cu_count, max_grid, lds),
+                                          kLevelBlock, lds, st>>>(la);
+            else
+                k_level_interval<16, 2><<<interval_grid<16, 2>(cu_count, max_grid, lds),
+                                          kLevelBlock, lds, st>>>(la);
+            return hipGetLastError();
+        }
     }
     // a 12-slot file frees 8 registers per lane over the 16-slot one, which
-    // is what 8 wavefronts per SIMD need
+    // is what 8 wavefronts per SIMD need (and a 20 KiB share of LDS per block)
+    const bool file12 = maxs == 8 || (maxs == 16 && slot_count <= 12);
+    la.log_cap = log_cap_for(a.mm_stride, 1, file12 ? 20 * 1024 : 32 * 1024);
+    const size_t lds = interval_lds_bytes(a.mm_stride, 1, la.log_cap);
+    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
     if (maxs == 16 && slot_count <= 12) {
         k_level_interval<12><<<interval_grid<12>(cu_count, max_grid, lds), kLevelBlock, lds,
-                               st>>>(a);
+                               st>>>(la);
         return hipGetLastError();
     }
     FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<interval_grid<M>(cu_count, max_grid, lds),
-                                                    kLevelBlock, lds, st>>>(a)));
+                                                    kLevelBlock, lds, st>>>(la)));
     return hipGetLastError();
 }
-
-// 16 cells when the block fits in 64 KiB, else 8 when it fits the device:
-// blocks above 64 KiB would leave a CU little LDS for the kernels that run
-// beside them (other trees of a batch).
 uint32_t root_cells_per_block(uint32_t table_bytes, uint32_t cell_bytes, size_t lds_per_block) {
     if (lds_per_block <= kRootStaticLds) return 0;
     const size_t cap = lds_per_block - kRootStaticLds;
@@ -2117,18 +2248,22 @@ hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32
 hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
                                        uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
     if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
-    const size_t lds = interval_lds_bytes(b.mm_stride);
+    if (interval_lds_bytes(b.mm_stride, 1, 0) > kMaxChoiceLds) return hipErrorInvalidValue;
+    BatchArgs bb = b;
+    const bool file12 = maxs == 8 || (maxs == 16 && slot_count <= 12);
+    bb.log_cap = log_cap_for(b.mm_stride, 1, file12 ? 20 * 1024 : 32 * 1024);
+    const size_t lds = interval_lds_bytes(b.mm_stride, 1, bb.log_cap);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
     if (maxs == 16 && slot_count <= 12) {
         k_level_interval_batch<12><<<batch_grid((const void*)k_level_interval_batch<12>, lds,
                                                 cu_count, max_grid),
-                                     kLevelBlock, lds, st>>>(b);
+                                     kLevelBlock, lds, st>>>(bb);
         return hipGetLastError();
     }
     FREP_DISPATCH_MAXS(maxs, (k_level_interval_batch<M><<<
                                   batch_grid((const void*)k_level_interval_batch<M>, lds, cu_count,
                                              max_grid),
-                                  kLevelBlock, lds, st>>>(b)));
+                                  kLevelBlock, lds, st>>>(bb)));
     return hipGetLastError();
 }
 

@@ -765,6 +765,74 @@ def test_interval_one_and_two_cells_per_lane_agree(ctx, oracle, name, dims, t):
     ctx.tape_free(th)
 
 
+@pytest.mark.parametrize("name,dims,t", [("csg24", [16, 8, 8], 0.0), ("trig", [8, 8, 4], 0.5),
+                                          ("csg24", [8, 8, 8, 4], 0.0), ("spheres8", [4, 4, 4, 4], 0.0)])
+def test_shortening_log_and_second_walk_agree(ctx, oracle, name, dims, t):
+    """Shortened tapes are written from the wavefront's LDS log of the words
+    its cells keep; a chunk whose log outgrows its LDS walks the parent tape
+    a second time instead.  FREP_LOG_CAP (read at each launch) shrinks the
+    log so that every chunk, or every chunk with more than 3 kept words,
+    takes the second walk, with one and two cells per lane: the trees equal
+    the oracle's every time."""
+    import os
+    tape = TAPES[name]()
+    grid = shapes.make_grid(dims, tape_time=t)
+    th = ctx.upload(tape)
+    want = oracle.Tree(tape, grid).canonical()
+    for cap, k1 in ((None, None), ("1", None), ("1", "1"), ("3", None), ("3", "1")):
+        for key, v in (("FREP_LOG_CAP", cap), ("FREP_INTERVAL_K1", k1)):
+            if v:
+                os.environ[key] = v
+        try:
+            tr = ctx.voxelize(th, grid)
+        finally:
+            os.environ.pop("FREP_LOG_CAP", None)
+            os.environ.pop("FREP_INTERVAL_K1", None)
+        got = ctx.export(tr)
+        ctx.tree_free(tr)
+        assert np.array_equal(got, want), (cap, k1)
+    ctx.tape_free(th)
+
+
+@pytest.mark.parametrize("name,dims", [("csg24", [16, 8, 8]), ("spheres8", [8, 8, 4]),
+                                       ("trig", [8, 8, 4]), ("csg24", [8, 8, 8, 4])])
+def test_tape_pool_holds_only_written_tapes(ctx, name, dims):
+    """Each level reserves exactly the words of the tapes it writes: one
+    [k, k words] tape per class of cells of a wavefront that keep the same
+    words, none for a child that keeps its whole parent tape.  So the words
+    in use past the root tape lie between one copy of each distinct
+    shortened tape per level and one tape per shortened child, and far
+    below a region of the parent's length per child."""
+    from frep import tree as T
+    tape = TAPES[name]()
+    grid = shapes.make_grid(dims)
+    th = ctx.upload(tape)
+    tr = ctx.voxelize(th, grid)
+    levels = T.parse(ctx.export(tr))
+    used = int(ctx.tree_info(tr).tape_pool_words)
+    ctx.tree_free(tr)
+    ctx.tape_free(th)
+    root = 16 + ((tape.n + 1 + 7) & ~7)          # root tape at word 16, 8-word aligned
+    lo = hi = per_child_region = 0
+    for L in range(1, len(levels)):
+        par = levels[L - 1]
+        plen = np.zeros(len(levels[L].tapes), dtype=np.int64)
+        for j in range(len(par.tapes)):
+            for c in par.child[j]:
+                if c != 0xFFFFFFFF:
+                    plen[c] = len(par.tapes[j]) - 1
+        distinct = set()
+        for j, tp in enumerate(levels[L].tapes):
+            k = len(tp) - 1
+            per_child_region += plen[j] + 1
+            if k < plen[j]:
+                hi += k + 1
+                distinct.add(tp.tobytes())
+        lo += sum(len(d) // 4 for d in distinct)
+    assert lo <= used - root <= hi, (lo, used - root, hi)
+    assert hi == 0 or used - root < per_child_region / 2
+
+
 def test_cpp_engine_classes_match_the_c_abi(ctx, tmp_path):
     """The C++ engine classes (include/frep_engine.hpp), driven like the
     reference's Application::Draw by cpp/engine_demo, give the same tree,

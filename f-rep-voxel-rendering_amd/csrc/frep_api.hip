@@ -443,6 +443,8 @@ frep_status frep_init(int device, frep_ctx** out) {
     // progress, so a SIMD's wavefronts finish together
     const char* pb = std::getenv("FREP_POINT_BALANCE");
     c->point_balance = !(pb && pb[0] == '0');
+    const char* rb = std::getenv("FREP_ROOT_BALANCE");
+    c->root_balance = !(rb && rb[0] == '0');
     HIP_TRY(scratch_address(device, &c->noalloc_staging));
     HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     for (auto& e : c->ev) HIP_TRY(hipEventCreate(&e));
@@ -1298,7 +1300,10 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         if (ctx->root_ops_required && nl > 1 && !use_root)
             return fail(ctx, FREP_UNSUPPORTED, "tape has no level-0 schedule (FREP_ROOT_OPS=2)");
         RootArgs ra{};
-        if (use_root) ra = root_args(tp, t, tp->root[rv], j.mm_stride, tp->root[rv].cpb);
+        if (use_root) {
+            ra = root_args(tp, t, tp->root[rv], j.mm_stride, tp->root[rv].cpb);
+            ra.a.balance = ctx->root_balance ? 1u : 0u;
+        }
         // init, the level launches and the counter readback; with timing on,
         // an event before the init and after each launch
         auto enqueue = [&](bool events) -> frep_status {

@@ -213,6 +213,7 @@ struct frep_ctx {
     bool root_ops = true;              // FREP_ROOT_OPS=0: level 0 by the per-cell kernel
     bool root_ops_required = false;    // FREP_ROOT_OPS=2: a tape without a level-0 schedule is an error
     bool point_balance = true;         // FREP_POINT_BALANCE=0: no last-level issue priority by progress
+    bool root_balance = true;          // FREP_ROOT_BALANCE=0: no level-0 issue priority by progress
     size_t lds_per_block = 65536;      // device limit (hipDeviceAttributeMaxSharedMemoryPerBlock)
     uint64_t next_uid = 1;
     hipEvent_t ev[frep::kMaxLevels + 4] = {};

@@ -973,9 +973,16 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
     };
     if (P) {
         uint2 q0 = ld(0), q1 = ld(1), q2 = ld(2), q3 = ld(3);
+        // with balance on, a wavefront steps its issue priority down as it
+        // progresses (3 at the start, 2 past half the forward passes, 1 after
+        // them, 0 past half the backward ones), so the SIMD's wavefronts
+        // move together instead of oldest first
+        const uint32_t half = (P / 2u) & ~3u;
+        if (a.balance) __builtin_amdgcn_s_setprio(3);
         // each entry is reloaded once its pass has used it, into the same
         // registers (no copies that would wait for the loads in flight)
         for (uint32_t p = 0; p < P; p += 4) {
+            if (a.balance && p == half) __builtin_amdgcn_s_setprio(2);
             forward(q0, p);
             q0 = ld(p + 4);
             forward(q1, p + 1);
@@ -986,6 +993,7 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
             q3 = ld(p + 7);
         }
     }
+    if (a.balance) __builtin_amdgcn_s_setprio(1);
     const float2 R = opnd(r.res);
     // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
     const bool full = R.y < 0.0f;
@@ -1018,7 +1026,9 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
         if (P) {
             auto ldr = [&](uint32_t t) { return ld(t < P ? P - 1u - t : 0u); };
             uint2 q0 = ldr(0), q1 = ldr(1), q2 = ldr(2), q3 = ldr(3);
+            const uint32_t half = (P / 2u) & ~3u;
             for (uint32_t t = 0; t < P; t += 4) {
+                if (a.balance && t == half) __builtin_amdgcn_s_setprio(0);
                 backward(q0);
                 q0 = ldr(t + 4);
                 backward(q1);
@@ -1128,6 +1138,7 @@ __device__ __forceinline__ void root_block(const RootArgs& r, uint32_t blk) {
         }
     }
     if (sl == 0) rec[4 + 2 * W + idx] = child_id;
+    __builtin_amdgcn_s_setprio(0);
 }
 
 template <int C, int LPC>

@@ -174,15 +174,14 @@ __global__ void k_init_tree(uint32_t* counters, uint32_t* nodes0, uint32_t* pool
 // classes that decide the same at every word (the same kept words and the
 // same min/max rewrites, so the same shortened tape: cells that decided the
 // same so far have the same live slots, so a class splits only at a live
-// MIN/MAX whose choices differ), and logs in LDS, for
-// every word some cell keeps, its index, its min/max row and the ballots of
-// the cells keeping it.  The first cell of each class owns the class's
-// tape.  One reservation (per wavefront, or per block) takes the child ids
-// and exactly the words of those tapes ([k, k words] each; none where
-// nothing was removed: the child then reuses the parent's tape,
-// shorten.comp:188-191).  Each owner's tape is written from the log by
-// ballot and prefix-sum compaction, 64 log entries at a time.  A log that
-// outgrew its LDS is replaced by a second walk in which the owners write
+// MIN/MAX whose choices differ), and logs in LDS, for every word, the
+// ballots of the cells keeping it.  The first cell of each class owns the
+// class's tape.  One reservation (per wavefront, or per block) takes the
+// child ids and exactly the words of those tapes ([k, k words] each; none
+// where nothing was removed: the child then reuses the parent's tape,
+// shorten.comp:188-191).  Each owner's tape is written in tape order from
+// the log by ballot and prefix-sum compaction, 64 words at a time.  A tape
+// longer than the log is walked a second time instead, the owners writing
 // their kept words from the end of their region.
 // ---------------------------------------------------------------------------
 
@@ -200,10 +199,14 @@ __device__ __forceinline__ uint32_t shortened_word(uint32_t w, bool is_mm, uint3
 // on_word(i, m, is_mm, live, L, ch, ow) for every word some cell keeps,
 // with, per group, this lane's cell's liveness, the ballot of it, the
 // cell's choice (MM_BOTH for other ops) and the word as the cell keeps it.
-template <int MAXS, int G, class F>
+// With LOG, the ballots of word i are also stored as log entry n - 1 - i,
+// logL[(n - 1 - i) * G + g] (zero for a word no cell keeps): each window's
+// ballots are gathered in registers, one lane per word, and stored by every
+// lane at the window's end.
+template <int MAXS, int G, bool LOG, class F>
 __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, uint32_t mm,
                                               const uint4* scr, const bool (&amb)[G],
-                                              uint32_t lane, F&& on_word) {
+                                              uint32_t lane, uint2* logL, F&& on_word) {
     constexpr int AW = (MAXS + 31) / 32;
     uint32_t act[G][AW];
 #pragma unroll
@@ -220,9 +223,14 @@ __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, ui
     uint32_t bw0 = window_load(tw, e - 64, n, lane);
     uint32_t bw1 = window_load(tw, e - 128, n, lane);
     uint32_t bw2 = window_load(tw, e - 192, n, lane);
+    uint32_t acc_lo[G], acc_hi[G];                     // the window's ballots (LOG)
     while (e > 0) {
         const int32_t s = e - 64;
         const int32_t lo = s > 0 ? s : 0;
+        if constexpr (LOG) {
+#pragma unroll
+            for (int g = 0; g < G; ++g) acc_lo[g] = acc_hi[g] = 0u;
+        }
         for (int32_t i = e - 1; i >= lo; --i) {
             const uint32_t w = rdlane(bw0, (uint32_t)(i - s));
             const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
@@ -255,6 +263,32 @@ __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, ui
             // a word no cell keeps changes nothing (common once the cells of
             // a chunk agree on their choices)
             if (any == 0) continue;
+            if constexpr (LOG) {
+                // lane e - 1 - i takes the word's ballots; the lane select
+                // goes through M0 (one scalar operand per instruction besides
+                // it; the compiler sets M0 before each of its own uses, so it
+                // is not listed).  A VALU-written SGPR needs wait states only
+                // as a lane select, not as the value written.
+                const uint32_t slot = (uint32_t)(e - 1 - i);
+                if constexpr (G == 1) {
+                    asm volatile("s_mov_b32 m0, %4\n"
+                                 "v_writelane_b32 %0, %2, m0\n"
+                                 "v_writelane_b32 %1, %3, m0\n"
+                                 : "+v"(acc_lo[0]), "+v"(acc_hi[0])
+                                 : "s"((uint32_t)L[0]), "s"((uint32_t)(L[0] >> 32)), "s"(slot));
+                } else {
+                    static_assert(G == 2, "one or two cell groups");
+                    asm volatile("s_mov_b32 m0, %8\n"
+                                 "v_writelane_b32 %0, %4, m0\n"
+                                 "v_writelane_b32 %1, %5, m0\n"
+                                 "v_writelane_b32 %2, %6, m0\n"
+                                 "v_writelane_b32 %3, %7, m0\n"
+                                 : "+v"(acc_lo[0]), "+v"(acc_hi[0]), "+v"(acc_lo[1]),
+                                   "+v"(acc_hi[1])
+                                 : "s"((uint32_t)L[0]), "s"((uint32_t)(L[0] >> 32)),
+                                   "s"((uint32_t)L[1]), "s"((uint32_t)(L[1] >> 32)), "s"(slot));
+                }
+            }
             const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu);
             const uint32_t ib = slot_field<MAXS>(w >> 24);
             uint32_t ow[G];
@@ -275,6 +309,13 @@ __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, ui
             }
             on_word((uint32_t)i, mi, is_mm, live, L, ch, ow);
         }
+        if constexpr (LOG) {
+            if (lane < (uint32_t)(e - lo)) {
+                const uint32_t entry = n - (uint32_t)e + lane;
+#pragma unroll
+                for (int g = 0; g < G; ++g) logL[entry * G + g] = make_uint2(acc_lo[g], acc_hi[g]);
+            }
+        }
         bw0 = bw1;
         bw1 = bw2;
         bw2 = window_load(tw, e - 256, n, lane);
@@ -283,21 +324,21 @@ __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, ui
 }
 
 // Shortened tapes and child records of a wavefront's ambiguous cells
-// (Bamb[g]: the ambiguous cells of group g; gx/gy/gz: this lane's cells'
-// coords in units of the next level's nodes).  The parent tape is at toff
-// (n words, mm MIN/MAX whose choice rows are in scr); logw / logL are the
-// wavefront's log (cap entries: index and row, then G ballots each).
+// (Bamb[g]: the ambiguous cells of group g; this lane's cell of group g is
+// mask bit cell0 + 64 g + lane of the node at ncx, ncy, ncz).  The parent
+// tape is at toff
+// (n words, mm MIN/MAX whose choice rows are in scr); logL is the
+// wavefront's log (cap entries of G ballots, one entry per word).
 // child[g] gets the child id of this lane's cell of group g (kNoChild for
 // cells that are not ambiguous or when the level ran out of capacity).
 template <int MAXS, int G, class Reserve>
 __device__ __forceinline__ void shorten_children(const LevelArgs& a, uint32_t lane,
                                                  uint32_t toff, uint32_t n, uint32_t mm,
-                                                 const uint4* scr, uint2* logw, uint2* logL,
-                                                 uint32_t cap, const uint64_t (&Bamb)[G],
-                                                 const uint32_t (&gx)[G],
-                                                 const uint32_t (&gy)[G],
-                                                 const uint32_t (&gz)[G], uint64_t child_base,
-                                                 const Reserve& reserve, uint32_t (&child)[G]) {
+                                                 const uint4* scr, uint2* logL, uint32_t cap,
+                                                 const uint64_t (&Bamb)[G], uint32_t cell0,
+                                                 uint32_t ncx, uint32_t ncy, uint32_t ncz,
+                                                 uint64_t child_base, const Reserve& reserve,
+                                                 uint32_t (&child)[G]) {
     const uint32_t* tw = a.pool + toff + 1;
     bool amb[G];
     uint32_t namb = 0;
@@ -315,32 +356,25 @@ __device__ __forceinline__ void shorten_children(const LevelArgs& a, uint32_t la
         kept[g] = 0;
         eq[g] = Bamb[g];
     }
-    uint32_t nlog = 0;
+    const bool logged = n <= cap;                      // the log holds every word
+    auto count = [&](uint32_t, uint32_t, bool is_mm, const bool (&live)[G], const uint64_t (&)[G],
+                     const uint32_t (&ch)[G], const uint32_t (&)[G]) {
+#pragma unroll
+        for (int g = 0; g < G; ++g) {
+            kept[g] += live[g] ? 1u : 0u;
+            // cells of one class have the same live slots, so they can part
+            // only where a live MIN/MAX chose differently
+            if (is_mm) {
+                const bool f = live[g] && (ch[g] & MM_FIRST) != 0;
+                const bool sc = live[g] && (ch[g] & MM_SECOND) != 0;
+                const uint64_t K1 = __ballot(f), K2 = __ballot(sc);
+                if (live[g]) eq[g] &= (f ? K1 : ~K1) & (sc ? K2 : ~K2);
+            }
+        }
+    };
     if (namb != 0) {
-        backward_walk<MAXS, G>(
-            tw, n, mm, scr, amb, lane,
-            [&](uint32_t i, uint32_t mi, bool is_mm, const bool (&live)[G],
-                const uint64_t (&L)[G], const uint32_t (&ch)[G], const uint32_t (&)[G]) {
-#pragma unroll
-                for (int g = 0; g < G; ++g) {
-                    kept[g] += live[g] ? 1u : 0u;
-                    // cells of one class have the same live slots, so they
-                    // can part only where a live MIN/MAX chose differently
-                    if (is_mm) {
-                        const bool f = live[g] && (ch[g] & MM_FIRST) != 0;
-                        const bool sc = live[g] && (ch[g] & MM_SECOND) != 0;
-                        const uint64_t K1 = __ballot(f), K2 = __ballot(sc);
-                        if (live[g]) eq[g] &= (f ? K1 : ~K1) & (sc ? K2 : ~K2);
-                    }
-                }
-                if (nlog < cap && lane == 0) {
-                    logw[nlog] = make_uint2(i, mi);
-#pragma unroll
-                    for (int g = 0; g < G; ++g)
-                        logL[nlog * G + g] = make_uint2((uint32_t)L[g], (uint32_t)(L[g] >> 32));
-                }
-                ++nlog;
-            });
+        if (logged) backward_walk<MAXS, G, true>(tw, n, mm, scr, amb, lane, logL, count);
+        else backward_walk<MAXS, G, false>(tw, n, mm, scr, amb, lane, logL, count);
     }
     // ---- owners and one exact reservation ----
     bool own[G];
@@ -375,41 +409,50 @@ __device__ __forceinline__ void shorten_children(const LevelArgs& a, uint32_t la
         lreg[g] = (uint32_t)__shfl((int)reg[g], (int)lead[g]);
     }
     // ---- the owners' tapes ----
-    if (nlog <= cap) {
+    if (logged) {
+        // tape order, 64 words at a time: lane j holds word w0 + j and its
+        // MIN/MAX row; each owner takes the words its log bit marks, at the
+        // positions its running count (lane ld of pos[g]) and the ballot's
+        // prefix give
+        uint64_t M[G];
+        uint32_t pos[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) {
-            uint64_t M = __ballot(own[g]);
-            while (M != 0) {
-                const uint32_t ld = (uint32_t)__builtin_ctzll(M);
-                M &= M - 1u;
-                const uint32_t kL = rdlane(kept[g], ld);
-                uint32_t* out = a.pool + rdlane(reg[g], ld);
-                if (lane == 0) out[0] = kL;
-                // the r-th kept entry of the log (tape order reversed) is
-                // word kL - r of the tape
-                uint32_t r = 0;
-                for (uint32_t e0 = 0; e0 < nlog; e0 += 64) {
-                    const uint32_t e = e0 + lane < nlog ? e0 + lane : nlog - 1u;
-                    const uint2 ew = logw[e];
-                    const uint2 el = logL[e * G + g];
-                    const uint64_t Lm = ((uint64_t)el.y << 32) | el.x;
-                    const bool kp = e0 + lane < nlog && ((Lm >> ld) & 1u) != 0;
+            M[g] = __ballot(own[g]);
+            pos[g] = 1u;
+            if (own[g]) a.pool[reg[g]] = kept[g];
+        }
+        uint32_t mm0 = 0;
+        for (uint32_t w0 = 0; w0 < n; w0 += 64) {
+            const uint32_t i = w0 + lane;
+            const uint32_t ic = i < n ? i : n - 1u;
+            const uint32_t wd = tw[ic];
+            const uint32_t op = wd & 0xFFu;
+            const bool is_mm = i < n && (op == OP_MIN || op == OP_MAX);
+            const uint64_t mmb = __ballot(is_mm);
+            const uint32_t mi = mm0 + lane_rank(mmb);
+            mm0 += (uint32_t)__popcll(mmb);
+            const uint32_t o = slot_field<MAXS>((wd >> 8) & 0xFFu), ib = slot_field<MAXS>(wd >> 24);
+#pragma unroll
+            for (int g = 0; g < G; ++g) {
+                const uint2 el = logL[(n - 1u - ic) * G + g];
+                const uint64_t Lm = ((uint64_t)el.y << 32) | el.x;
+                uint4 v = make_uint4(0u, 0u, 0u, 0u);
+                if (is_mm) v = scr[mi * G + g];
+                const uint64_t Fm = ((uint64_t)v.y << 32) | v.x;
+                const uint64_t Sm = ((uint64_t)v.w << 32) | v.z;
+                for (uint64_t Mo = M[g]; Mo != 0; Mo &= Mo - 1u) {
+                    const uint32_t ld = (uint32_t)__builtin_ctzll(Mo);
+                    const bool kp = i < n && ((Lm >> ld) & 1u) != 0;
                     const uint64_t b = __ballot(kp);
+                    const uint32_t r = rdlane(pos[g], ld);
                     if (kp) {
-                        uint32_t wd = tw[ew.x];
-                        const uint32_t op = wd & 0xFFu;
-                        if (op == OP_MIN || op == OP_MAX) {
-                            const uint4 v = scr[ew.y * G + g];
-                            const uint64_t Fm = ((uint64_t)v.y << 32) | v.x;
-                            const uint64_t Sm = ((uint64_t)v.w << 32) | v.z;
-                            const uint32_t c = (uint32_t)((Fm >> ld) & 1u) |
-                                               ((uint32_t)((Sm >> ld) & 1u) << 1);
-                            wd = shortened_word(wd, true, c, slot_field<MAXS>((wd >> 8) & 0xFFu),
-                                                slot_field<MAXS>(wd >> 24));
-                        }
-                        out[kL - r - lane_rank(b)] = wd;
+                        const uint32_t c = (uint32_t)((Fm >> ld) & 1u) |
+                                           ((uint32_t)((Sm >> ld) & 1u) << 1);
+                        a.pool[rdlane(reg[g], ld) + r + lane_rank(b)] =
+                            shortened_word(wd, is_mm, c, o, ib);
                     }
-                    r += (uint32_t)__popcll(b);
+                    if (lane == ld) pos[g] = r + (uint32_t)__popcll(b);
                 }
             }
         }
@@ -417,8 +460,8 @@ __device__ __forceinline__ void shorten_children(const LevelArgs& a, uint32_t la
         uint32_t r[G];
 #pragma unroll
         for (int g = 0; g < G; ++g) r[g] = 0;
-        backward_walk<MAXS, G>(
-            tw, n, mm, scr, amb, lane,
+        backward_walk<MAXS, G, false>(
+            tw, n, mm, scr, amb, lane, logL,
             [&](uint32_t, uint32_t, bool, const bool (&live)[G], const uint64_t (&)[G],
                 const uint32_t (&)[G], const uint32_t (&ow)[G]) {
 #pragma unroll
@@ -437,11 +480,13 @@ __device__ __forceinline__ void shorten_children(const LevelArgs& a, uint32_t la
     for (int g = 0; g < G; ++g) {
         if (amb[g]) {
             child[g] = cbase + rank0 + lane_rank(Bamb[g]);
+            uint32_t cx, cy, cz;
+            cell_of(cell0 + 64u * g + lane, a.dim, a.dim_shift, &cx, &cy, &cz);
             uint32_t* crec = a.child_nodes + (size_t)child[g] * a.child_ns;
             crec[0] = kept[g] < n ? lreg[g] : toff;
-            crec[1] = gx[g];
-            crec[2] = gy[g];
-            crec[3] = gz[g];
+            crec[1] = ncx * a.dim + cx;
+            crec[2] = ncy * a.dim + cy;
+            crec[3] = ncz * a.dim + cz;
         }
         rank0 += (uint32_t)__popcll(Bamb[g]);
     }
@@ -466,7 +511,7 @@ __device__ __forceinline__ void shorten_children(const LevelArgs& a, uint32_t la
 template <int MAXS, class Reserve>
 __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                uint64_t child_base, uint32_t lane, uint4* scr,
-                                               uint2* logw, uint2* logL, uint32_t cap,
+                                               uint2* logL, uint32_t cap,
                                                const Reserve& reserve) {
     const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
@@ -578,10 +623,9 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     store_mask64(rec + 4 + W + sub * 2, Bamb, lane);
 
     const uint64_t Bambs[1] = {Bamb};
-    const uint32_t cgx[1] = {gx}, cgy[1] = {gy}, cgz[1] = {gz};
     uint32_t child[1];
-    shorten_children<MAXS, 1>(a, lane, toff, n, mm, scr, logw, logL, cap, Bambs, cgx, cgy, cgz,
-                              child_base, reserve, child);
+    shorten_children<MAXS, 1>(a, lane, toff, n, mm, scr, logL, cap, Bambs, sub * 64, ncx,
+                              ncy, ncz, child_base, reserve, child);
     rec[4 + 2 * W + idx] = child[0];
     __builtin_amdgcn_s_setprio(0);
 }
@@ -596,7 +640,7 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
 template <int MAXS, class Reserve>
 __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t node, uint32_t sub,
                                                 uint64_t child_base, uint32_t lane, uint4* scr,
-                                                uint2* logw, uint2* logL, uint32_t cap,
+                                                uint2* logL, uint32_t cap,
                                                 const Reserve& reserve) {
     const uint32_t dim = a.dim, cells = dim * dim * dim, W = cells / 32;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
@@ -724,10 +768,9 @@ __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t nod
 
     // group 0's children first
     const uint64_t Bambs[2] = {Bamb0, Bamb1};
-    const uint32_t cgx[2] = {gx0, gx1}, cgy[2] = {gy0, gy1}, cgz[2] = {gz0, gz1};
     uint32_t child[2];
-    shorten_children<MAXS, 2>(a, lane, toff, n, mm, scr, logw, logL, cap, Bambs, cgx, cgy, cgz,
-                              child_base, reserve, child);
+    shorten_children<MAXS, 2>(a, lane, toff, n, mm, scr, logL, cap, Bambs, sub * 128, ncx,
+                              ncy, ncz, child_base, reserve, child);
     rec[4 + 2 * W + idx0] = child[0];
     rec[4 + 2 * W + idx1] = child[1];
     __builtin_amdgcn_s_setprio(0);
@@ -761,9 +804,8 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 7 : 1)))) void k_
     const uint32_t total = n_nodes * cpn;
     uint4* scr = lds_choice + (size_t)wib * a.mm_stride * KC;
     // the shortening log after every wavefront's choice rows (log_entry_bytes)
-    uint2* logw = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * a.mm_stride * KC) +
-                  (size_t)wib * a.log_cap * (1 + KC);
-    uint2* logL = logw + a.log_cap;
+    uint2* logL = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * a.mm_stride * KC) +
+                  (size_t)wib * a.log_cap * KC;
     // rounds of kLevelBlock/64 consecutive chunks per block, one reservation
     // atomic per round; a wavefront past the end still takes part
     constexpr uint32_t WPB = kLevelBlock / 64;
@@ -776,10 +818,10 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 7 : 1)))) void k_
         if (chunk < total) {
             const uint32_t node = chunk / cpn, sub = chunk - node * cpn;
             if constexpr (KC == 2)
-                interval_chunk2<MAXS>(a, node, sub, child_base, lane, scr, logw, logL, a.log_cap,
+                interval_chunk2<MAXS>(a, node, sub, child_base, lane, scr, logL, a.log_cap,
                                       br);
             else
-                interval_chunk<MAXS>(a, node, sub, child_base, lane, scr, logw, logL, a.log_cap,
+                interval_chunk<MAXS>(a, node, sub, child_base, lane, scr, logL, a.log_cap,
                                      br);
         } else {
             br(ctr, 0, lane);
@@ -1440,9 +1482,8 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 7 : 1))) void k_level_interval_b
     const uint32_t incl = wave_incl_scan(plan_lane<false>(b, lane, &cbase), lane);
     const uint32_t total = rdlane(incl, 63);
     uint4* scr = lds_choice + (size_t)wib * b.mm_stride;
-    uint2* logw = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * b.mm_stride) +
-                  (size_t)wib * b.log_cap * 2;
-    uint2* logL = logw + b.log_cap;
+    uint2* logL = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * b.mm_stride) +
+                  (size_t)wib * b.log_cap;
     cargs_t shapes = (cargs_t)b.shapes;
     uint32_t cur = 0xFFFFFFFFu, start = 0, cpn = 1;
     uint64_t cb = 0;
@@ -1464,7 +1505,7 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 7 : 1))) void k_level_interval_b
         if (skip) continue;
         const uint32_t local = chunk - start;
         const uint32_t node = local / cpn, sub = local - node * cpn;
-        interval_chunk<MAXS>(a, node, sub, cb, lane, scr, logw, logL, b.log_cap, WaveReserve{});
+        interval_chunk<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap, WaveReserve{});
     }
 }
 
@@ -1994,18 +2035,35 @@ bool two_cells_per_lane() {
     return !(e && e[0] == '1');
 }
 
-// Shortening log of a wavefront: an index / choice-row pair and KC ballots
-// per entry.
-size_t log_entry_bytes(uint32_t kc) { return 8u * (1u + kc); }
+// Shortening log of a wavefront: KC ballots per entry (one entry per word of
+// the parent tape).
+size_t log_entry_bytes(uint32_t kc) { return 8u * kc; }
 
 size_t interval_lds_bytes(uint32_t mm_stride, uint32_t kc, uint32_t cap) {
     return (size_t)(kLevelBlock / 64) *
            ((size_t)mm_stride * kc * sizeof(uint4) + (size_t)cap * log_entry_bytes(kc));
 }
 
+int blocks_per_cu(const void* kern, size_t lds);
+
+// LDS a block of kern may take without lowering the blocks per CU that its
+// registers allow: the CU's 160 KiB shared among them, less the kernel's
+// static LDS and a margin.
+size_t lds_share(const void* kern) {
+    thread_local std::map<const void*, size_t> cache;
+    const auto it = cache.find(kern);
+    if (it != cache.end()) return it->second;
+    constexpr size_t kLdsPerCu = 160 * 1024;
+    hipFuncAttributes at{};
+    const size_t st = hipFuncGetAttributes(&at, kern) == hipSuccess ? at.sharedSizeBytes : 1024;
+    const size_t per = kLdsPerCu / (size_t)blocks_per_cu(kern, 0);
+    const size_t share = per > st + 256 ? per - st - 256 : 0;
+    cache[kern] = share;
+    return share;
+}
+
 // Log entries per wavefront: what fits in block_budget bytes of LDS per
-// block (the block's share of a CU at the occupancy the kernel is built
-// for) after the choice rows, between 64 and 4096, and within the device's
+// block after the choice rows, between 64 and 4096, and within the device's
 // LDS per block.  FREP_LOG_CAP=n sets it (read at each launch; a tiny log
 // makes every chunk take the second walk).
 uint32_t log_cap_for(uint32_t mm_stride, uint32_t kc, size_t block_budget) {
@@ -2021,11 +2079,21 @@ uint32_t log_cap_for(uint32_t mm_stride, uint32_t kc, size_t block_budget) {
     return (uint32_t)cap;
 }
 
+// One interval launch: the log sized to the kernel's LDS share.
+template <int M, int KC>
+hipError_t launch_interval_kc(hipStream_t st, LevelArgs la, uint32_t cu_count, uint32_t max_grid) {
+    la.log_cap = log_cap_for(la.mm_stride, KC, lds_share((const void*)k_level_interval<M, KC>));
+    const size_t lds = interval_lds_bytes(la.mm_stride, KC, la.log_cap);
+    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
+    k_level_interval<M, KC><<<interval_grid<M, KC>(cu_count, max_grid, lds), kLevelBlock, lds,
+                              st>>>(la);
+    return hipGetLastError();
+}
+
 hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t maxs,
                                  uint32_t slot_count, uint32_t cu_count, uint32_t max_grid,
                                  uint32_t nodes_hint) {
-    if (interval_lds_bytes(a.mm_stride, 1, 0) > kMaxChoiceLds) return hipErrorInvalidValue;
-    LevelArgs la = a;
+    if (interval_lds_bytes(a.mm_stride, 1, 1) > kMaxChoiceLds) return hipErrorInvalidValue;
     // two cells per lane (one choice row per group, so twice the LDS) for
     // files of up to 16 slots on nodes of a multiple of 128 cells, unless
     // the level is known to be small: below 16 one-cell chunks per CU, the
@@ -2033,35 +2101,16 @@ hipError_t launch_level_interval(hipStream_t st, const LevelArgs& a, uint32_t ma
     // cells ran 50 us with one cell per lane, 55.5 us with two)
     const uint32_t cells = a.dim * a.dim * a.dim;
     const bool big = nodes_hint == 0 || (uint64_t)nodes_hint * (cells / 64) >= 16ull * cu_count;
-    if (maxs <= 16 && cells % 128 == 0 && big && two_cells_per_lane()) {
-        la.log_cap = log_cap_for(a.mm_stride, 2, 32 * 1024);
-        const size_t lds = interval_lds_bytes(a.mm_stride, 2, la.log_cap);
-        if (lds <= kMaxChoiceLds) {
-            if (maxs == 8)
-                k_level_interval<8, 2><<<interval_grid<8, 2>(cu_count, max_grid, lds),
-                                         kLevelBlock, lds, st>>>(la);
-            else if (slot_count <= 12)
-                k_level_interval<12, 2><<<interval_grid<12, 2>(cu_count, max_grid, lds),
-                                          kLevelBlock, lds, st>>>(la);
-            else
-                k_level_interval<16, 2><<<interval_grid<16, 2>(cu_count, max_grid, lds),
-                                          kLevelBlock, lds, st>>>(la);
-            return hipGetLastError();
-        }
+    if (maxs <= 16 && cells % 128 == 0 && big && two_cells_per_lane() &&
+        interval_lds_bytes(a.mm_stride, 2, 1) <= kMaxChoiceLds) {
+        if (maxs == 8) return launch_interval_kc<8, 2>(st, a, cu_count, max_grid);
+        if (slot_count <= 12) return launch_interval_kc<12, 2>(st, a, cu_count, max_grid);
+        return launch_interval_kc<16, 2>(st, a, cu_count, max_grid);
     }
     // a 12-slot file frees 8 registers per lane over the 16-slot one, which
-    // is what 8 wavefronts per SIMD need (and a 20 KiB share of LDS per block)
-    const bool file12 = maxs == 8 || (maxs == 16 && slot_count <= 12);
-    la.log_cap = log_cap_for(a.mm_stride, 1, file12 ? 20 * 1024 : 32 * 1024);
-    const size_t lds = interval_lds_bytes(a.mm_stride, 1, la.log_cap);
-    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
-    if (maxs == 16 && slot_count <= 12) {
-        k_level_interval<12><<<interval_grid<12>(cu_count, max_grid, lds), kLevelBlock, lds,
-                               st>>>(la);
-        return hipGetLastError();
-    }
-    FREP_DISPATCH_MAXS(maxs, (k_level_interval<M><<<interval_grid<M>(cu_count, max_grid, lds),
-                                                    kLevelBlock, lds, st>>>(la)));
+    // is what 8 wavefronts per SIMD need
+    if (maxs == 16 && slot_count <= 12) return launch_interval_kc<12, 1>(st, a, cu_count, max_grid);
+    FREP_DISPATCH_MAXS(maxs, return (launch_interval_kc<M, 1>(st, a, cu_count, max_grid)));
     return hipGetLastError();
 }
 uint32_t root_cells_per_block(uint32_t table_bytes, uint32_t cell_bytes, size_t lds_per_block) {
@@ -2259,10 +2308,12 @@ hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32
 hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
                                        uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
     if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
-    if (interval_lds_bytes(b.mm_stride, 1, 0) > kMaxChoiceLds) return hipErrorInvalidValue;
+    if (interval_lds_bytes(b.mm_stride, 1, 1) > kMaxChoiceLds) return hipErrorInvalidValue;
     BatchArgs bb = b;
-    const bool file12 = maxs == 8 || (maxs == 16 && slot_count <= 12);
-    bb.log_cap = log_cap_for(b.mm_stride, 1, file12 ? 20 * 1024 : 32 * 1024);
+    const void* kern = nullptr;
+    if (maxs == 16 && slot_count <= 12) kern = (const void*)k_level_interval_batch<12>;
+    else FREP_DISPATCH_MAXS(maxs, kern = (const void*)k_level_interval_batch<M>);
+    bb.log_cap = log_cap_for(b.mm_stride, 1, lds_share(kern));
     const size_t lds = interval_lds_bytes(b.mm_stride, 1, bb.log_cap);
     if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
     if (maxs == 16 && slot_count <= 12) {

@@ -442,7 +442,14 @@ frep_status frep_init(int device, frep_ctx** out) {
     // the plain last level's wavefronts lower their issue priority as they
     // progress, so a SIMD's wavefronts finish together
     const char* pb = std::getenv("FREP_POINT_BALANCE");
-    c->point_balance = !(pb && pb[0] == '0');
+    if (pb) {
+        const int v = std::atoi(pb);
+        if (v >= 0 && v <= 64) c->point_balance = (uint32_t)v;
+    }
+    if (const char* bl = std::getenv("FREP_BATCH_LANES")) {
+        const int v = std::atoi(bl);
+        if (v >= 1 && v <= (int)frep::kBatchLanes) c->batch_lanes = (uint32_t)v;
+    }
     const char* rb = std::getenv("FREP_ROOT_BALANCE");
     c->root_balance = !(rb && rb[0] == '0');
     HIP_TRY(scratch_address(device, &c->noalloc_staging));
@@ -1091,7 +1098,7 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
         for (const uint32_t k : idx)
             if (jobs[k].t->grid.n_levels > 1 && !root_job(BatchPlan{0, true}, jobs[k]))
                 return fail(ctx, FREP_UNSUPPORTED, "tape has no level-0 schedule (FREP_ROOT_OPS=2)");
-    const BatchPlan P = plan_batch(jobs, idx, kBatchLanes, ctx->root_ops, ctx->cu_count);
+    const BatchPlan P = plan_batch(jobs, idx, ctx->batch_lanes, ctx->root_ops, ctx->cu_count);
     if (P.m == 0) return FREP_OK;
     BatchStage* S = &ctx->stage;
     if (S->stage_bytes < P.bytes || S->cnt_words < (size_t)P.m * CNT_WORDS) {
@@ -1284,7 +1291,7 @@ frep_status frep_voxelize(frep_ctx* ctx, frep_tape_t th, const frep_grid* grid, 
         for (uint32_t L = 0; L < nl; ++L) nodes_hint[L] = host_nodes(t, L);
         la[nl - 1].finish = 1;
         la[nl - 1].host_counters = t->h_counters;
-        la[nl - 1].balance = ctx->point_balance ? 1u : 0u;
+        la[nl - 1].balance = ctx->point_balance;
         // the buffers are as k_init_tree leaves them for this tape when the
         // last plain build of it into them completed (finish_tree)
         const bool skip_init = t->ready_gen == t->dev_gen && t->ready_tape == tp->uid;

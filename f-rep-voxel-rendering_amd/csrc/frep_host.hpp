@@ -212,8 +212,9 @@ struct frep_ctx {
     bool graphs = false;               // FREP_GRAPH=1: repeated plain builds replayed as a graph
     bool root_ops = true;              // FREP_ROOT_OPS=0: level 0 by the per-cell kernel
     bool root_ops_required = false;    // FREP_ROOT_OPS=2: a tape without a level-0 schedule is an error
-    bool point_balance = true;         // FREP_POINT_BALANCE=0: no last-level issue priority by progress
+    uint32_t point_balance = 4;        // FREP_POINT_BALANCE=n: last-level priority steps every 1/n of a share (0: off)
     bool root_balance = true;          // FREP_ROOT_BALANCE=0: no level-0 issue priority by progress
+    uint32_t batch_lanes = frep::kBatchLanes;  // FREP_BATCH_LANES=1..3: lanes of a synchronous batch
     size_t lds_per_block = 65536;      // device limit (hipDeviceAttributeMaxSharedMemoryPerBlock)
     uint64_t next_uid = 1;
     hipEvent_t ev[frep::kMaxLevels + 4] = {};

@@ -1346,7 +1346,7 @@ __device__ __forceinline__ void level_point_body(const LevelArgs& a) {
         // alone.  With balance on, a wavefront steps its priority down from 3
         // as it completes each quarter of its share and the others catch up.
         const uint32_t q =
-            a.balance ? uni(max(1u, (total + 4u * nwaves - 1u) / (4u * nwaves))) : 0u;
+            a.balance ? uni(max(1u, (total + a.balance * nwaves - 1u) / (a.balance * nwaves))) : 0u;
         if (q) __builtin_amdgcn_s_setprio(3);
         uint32_t done = 0;
         for (;;) {

@@ -362,6 +362,9 @@ def main():
     achieved = work[dom]["flops"] / (level_ms[dom] * 1e-3) / 1e12
     out = {
         "metric": "Gvoxel-evals/s (1024^3, 200-op tape); batched vs plain req/s; p50/p99 latency",
+        "metric_note": "the metric's name is the configuration's; the headline tape compiles to "
+                       f"{tape.n} ops, and value counts executed voxel-evals (value_basis), the "
+                       "dense-equivalent rate is effective_gvoxel_per_s",
         "value": round(value, 3),
         "unit": "Gvoxel-evals/s",
         "n_gpus": 1,

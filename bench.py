@@ -129,7 +129,7 @@ def cpu_baseline(tape, grid, budgets=(("faithful", 0, 10.0), ("tight", 0, 5.0),
             "cpu_model": cpu_model(), "host_threads": os.cpu_count(), "runs": runs}
 
 
-def bench_batched(ctx, shapes, reps=5, warm=6):
+def bench_batched(ctx, shapes, reps=5, warm=8):
     """The 64-shape mix (sphere unions of 4..400 primitives at 128^3..1024^3):
     requests/s of one frep_voxelize per shape (plain) against one
     frep_voxelize_batch over all 64 (batched), same trees, same results.

@@ -797,8 +797,11 @@ size_t align16(size_t v) { return (v + 15) & ~(size_t)15; }
 // (blocks per CU) and keep, per launch, the fastest; later calls run the
 // chosen sizes from a captured hipGraph (replayed with one launch).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kTuneBlocksPerCu[] = {8, 4, 2};   // candidates, 4 wavefronts per block
-constexpr uint32_t kTuneTrials = sizeof(kTuneBlocksPerCu) / sizeof(kTuneBlocksPerCu[0]);
+// Candidates: blocks per CU (4 wavefronts per block) x cells per lane of
+// the interval launches (the tile edge: 64 or 128 cells per wavefront).
+constexpr uint32_t kTuneBlocksPerCu[] = {8, 4, 2};
+constexpr uint32_t kTuneGrids = sizeof(kTuneBlocksPerCu) / sizeof(kTuneBlocksPerCu[0]);
+constexpr uint32_t kTuneTrials = 2 * kTuneGrids;
 constexpr size_t kTuneMaxEntries = 64;
 
 // A batched root launch uses one schedule variant for all its trees (one
@@ -891,6 +894,17 @@ BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& 
                     la.maxs = std::max(la.maxs, j.t->maxs);
                     la.slots = std::max(la.slots, j.t->slot_count);
                     la.mm = std::max(la.mm, j.mm_stride);
+                }
+                if (la.count && !la.point) {
+                    bool ok = la.maxs <= 16;
+                    for (uint32_t k = 0; k < G.count && ok; ++k) {
+                        const Job& j = jobs[idx[G.first + k]];
+                        const uint32_t nl = j.t->grid.n_levels;
+                        if (L >= nl || L + 1 == nl || (L == 0 && root_job(P, j))) continue;
+                        const uint32_t d = j.t->grid.dims[L];
+                        ok = (d * d * d) % 128u == 0;
+                    }
+                    la.kc2_ok = ok;
                 }
                 if (la.count && la.point) {
                     // the assembly-only kernel when every shape takes the
@@ -1023,7 +1037,7 @@ void free_stage(BatchStage* S) {
 // after each launch.
 frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const BatchPlan& P,
                           const std::vector<uint32_t>& grid, hipEvent_t* ev,
-                          const LaneSet* lanes) {
+                          const LaneSet* lanes, const std::vector<uint32_t>* kcs) {
     uint8_t* ds = static_cast<uint8_t*>(S->d_stage);
     HIP_TRY(hipMemcpyAsync(ds, S->h_stage, P.bytes, hipMemcpyHostToDevice, st));
     bool used[kBatchLanes] = {};
@@ -1046,6 +1060,7 @@ frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const Ba
         for (; li < P.launches.size() && P.launches[li].group == g; ++li) {
             const BatchLaunch& la = P.launches[li];
             BatchArgs b{reinterpret_cast<const LevelArgs*>(ds + la.off), la.count, la.mm};
+            b.kc = (la.kc2_ok && kcs && (*kcs)[li] == 2) ? 2u : 1u;
             if (la.root)
                 HIP_TRY(launch_root_interval_batch(
                     gs, reinterpret_cast<const RootArgs*>(ds + la.off),
@@ -1109,6 +1124,7 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
     fill_batch(P, jobs, idx, static_cast<uint8_t*>(S->h_stage));
     const size_t nl = P.launches.size();
     std::vector<uint32_t> grid(nl, ctx->level_grid);
+    std::vector<uint32_t> kcs(nl, 1u);
     TuneRecord* rec = nullptr;
     if (tune && ctx->autotune) {
         const uint64_t key = batch_key(P, static_cast<const uint8_t*>(S->h_stage));
@@ -1118,20 +1134,28 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
             it = ctx->tune.emplace(key, TuneRecord{}).first;
             it->second.best_ms.assign(nl, 1e30f);
             it->second.grid.assign(nl, ctx->level_grid);
+            it->second.kc.assign(nl, 1u);
         }
         rec = &it->second;
     }
     hipEvent_t* ev = nullptr;
     bool timed_trial = false;
+    // trials that would repeat an earlier one (no launch can take two cells
+    // per lane) are skipped
+    bool any_kc2 = false;
+    for (const BatchLaunch& la : P.launches) any_kc2 = any_kc2 || la.kc2_ok;
+    if (rec && rec->trial >= kTuneGrids && !any_kc2) rec->trial = kTuneTrials;
     if (rec && rec->trial < kTuneTrials) {
-        const uint32_t cap = ctx->cu_count * kTuneBlocksPerCu[rec->trial];
+        const uint32_t cap = ctx->cu_count * kTuneBlocksPerCu[rec->trial % kTuneGrids];
         for (auto& g : grid) g = std::min(cap, ctx->level_grid);
+        for (auto& k : kcs) k = rec->trial < kTuneGrids ? 1u : 2u;
         const frep_status s = ensure_events(ctx, nl + 1);
         if (s != FREP_OK) return s;
         ev = ctx->bev.data();
         timed_trial = true;
     } else if (rec) {
         grid = rec->grid;
+        kcs = rec->kc;
     }
     // a plan with side lanes is launched directly: a graph replays forked
     // branches one after another, losing the overlap the lanes are for
@@ -1142,7 +1166,7 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
         if (!rec->exec) {
             hipGraph_t graph = nullptr;
             HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-            const frep_status s = enqueue_batch(ctx, S, ctx->stream, P, grid, nullptr);
+            const frep_status s = enqueue_batch(ctx, S, ctx->stream, P, grid, nullptr, nullptr, &kcs);
             const hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
             if (s != FREP_OK) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1156,7 +1180,7 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
         HIP_TRY(hipGraphLaunch(rec->exec, ctx->stream));
     } else {
         const frep_status s =
-            enqueue_batch(ctx, S, ctx->stream, P, grid, ev, ev ? nullptr : &ctx->lanes);
+            enqueue_batch(ctx, S, ctx->stream, P, grid, ev, ev ? nullptr : &ctx->lanes, &kcs);
         if (s != FREP_OK) return s;
     }
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[1], ctx->stream));
@@ -1172,6 +1196,7 @@ frep_status run_batch(frep_ctx* ctx, std::vector<Job>& jobs, const std::vector<u
             if (ms < rec->best_ms[li]) {
                 rec->best_ms[li] = ms;
                 rec->grid[li] = grid[li];
+                rec->kc[li] = kcs[li];
             }
         }
         ++rec->trial;

@@ -126,6 +126,7 @@ struct TuneRecord {
     uint32_t trial = 0;                // candidates timed so far
     std::vector<float> best_ms;        // per launch
     std::vector<uint32_t> grid;        // per launch: best persistent grid so far
+    std::vector<uint32_t> kc;          // per launch: cells per lane of the best (interval launches)
     hipGraphExec_t exec = nullptr;     // graph of the tuned batch
 };
 
@@ -173,6 +174,8 @@ struct BatchLaunch {
                                        // at most this many slots (0: not all)
     size_t lds = 0;                    // dynamic LDS per block
     size_t starts_off = 0;             // byte offset of the block starts (count + 1 words)
+    bool kc2_ok = false;               // interval launch: two cells per lane possible (nodes of a
+                                       // multiple of 128 cells, slot files of at most 16)
 };
 struct BatchGroup {
     uint32_t first, count;
@@ -304,7 +307,8 @@ void free_stage(BatchStage* S);
 // of lane 0 (ev must then be null).
 frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const BatchPlan& P,
                           const std::vector<uint32_t>& grid, hipEvent_t* ev,
-                          const LaneSet* lanes = nullptr);
+                          const LaneSet* lanes = nullptr,
+                          const std::vector<uint32_t>* kcs = nullptr);
 
 // Kernel arguments of a render of tree t (outputs are device pointers).
 RenderArgs render_args(const TreeObj* t, const frep_camera* cam, uint8_t* rgba,

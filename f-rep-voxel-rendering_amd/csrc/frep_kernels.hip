@@ -1449,7 +1449,7 @@ __device__ __forceinline__ uint32_t plan_lane(const BatchArgs& b, uint32_t lane,
     nn = nn < A->node_cap ? nn : A->node_cap;
     const uint32_t cells = A->dim * A->dim * A->dim;
     if (!POINT) *cbase = child_tape_base(cnt, A->level);
-    return nn * (POINT ? cells / (64u * A->pk) : cells / 64u);
+    return nn * (POINT ? cells / (64u * A->pk) : cells / (64u * b.kc));
 }
 
 // Scalar (constant address space) copy of one shape's arguments.
@@ -1470,9 +1470,9 @@ __device__ __forceinline__ uint64_t rdlane64(uint64_t v, uint32_t l) {
 
 }  // namespace
 
-template <int MAXS>
+template <int MAXS, int KC = 1>
 __global__ __launch_bounds__(kLevelBlock)
-__attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 7 : 1))) void k_level_interval_batch(BatchArgs b) {
+__attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 7 : 1)))) void k_level_interval_batch(BatchArgs b) {
     extern __shared__ uint4 lds_choice[];
     const uint32_t lane = lane_id();
     const uint32_t wib = uni(threadIdx.x >> 6);
@@ -1481,9 +1481,9 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 7 : 1))) void k_level_interval_b
     uint64_t cbase;
     const uint32_t incl = wave_incl_scan(plan_lane<false>(b, lane, &cbase), lane);
     const uint32_t total = rdlane(incl, 63);
-    uint4* scr = lds_choice + (size_t)wib * b.mm_stride;
-    uint2* logL = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * b.mm_stride) +
-                  (size_t)wib * b.log_cap;
+    uint4* scr = lds_choice + (size_t)wib * b.mm_stride * KC;
+    uint2* logL = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * b.mm_stride * KC) +
+                  (size_t)wib * b.log_cap * KC;
     cargs_t shapes = (cargs_t)b.shapes;
     uint32_t cur = 0xFFFFFFFFu, start = 0, cpn = 1;
     uint64_t cb = 0;
@@ -1496,7 +1496,7 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 7 : 1))) void k_level_interval_b
             a = load_args(shapes + s);
             start = s ? rdlane(incl, s - 1) : 0u;
             cb = rdlane64(cbase, s);
-            cpn = a.dim * a.dim * a.dim / 64;
+            cpn = a.dim * a.dim * a.dim / (64 * KC);
             // a shape that ran out of capacity at an earlier level is rebuilt
             // by the host; one that runs out during this launch goes on, so
             // its counters report what the whole level needs
@@ -1505,7 +1505,10 @@ __attribute__((amdgpu_waves_per_eu(MAXS <= 12 ? 7 : 1))) void k_level_interval_b
         if (skip) continue;
         const uint32_t local = chunk - start;
         const uint32_t node = local / cpn, sub = local - node * cpn;
-        interval_chunk<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap, WaveReserve{});
+        if constexpr (KC == 2)
+            interval_chunk2<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap, WaveReserve{});
+        else
+            interval_chunk<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap, WaveReserve{});
     }
 }
 
@@ -2305,27 +2308,34 @@ hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32
     return hipGetLastError();
 }
 
+// One batched interval launch: the log sized to the kernel's LDS share.
+template <int M, int KC>
+hipError_t launch_interval_batch_kc(hipStream_t st, BatchArgs bb, uint32_t cu_count,
+                                    uint32_t max_grid) {
+    const void* kern = (const void*)k_level_interval_batch<M, KC>;
+    bb.kc = KC;
+    bb.log_cap = log_cap_for(bb.mm_stride, KC, lds_share(kern));
+    const size_t lds = interval_lds_bytes(bb.mm_stride, KC, bb.log_cap);
+    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
+    k_level_interval_batch<M, KC><<<batch_grid(kern, lds, cu_count, max_grid), kLevelBlock, lds,
+                                    st>>>(bb);
+    return hipGetLastError();
+}
+
 hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
                                        uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
     if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
     if (interval_lds_bytes(b.mm_stride, 1, 1) > kMaxChoiceLds) return hipErrorInvalidValue;
-    BatchArgs bb = b;
-    const void* kern = nullptr;
-    if (maxs == 16 && slot_count <= 12) kern = (const void*)k_level_interval_batch<12>;
-    else FREP_DISPATCH_MAXS(maxs, kern = (const void*)k_level_interval_batch<M>);
-    bb.log_cap = log_cap_for(b.mm_stride, 1, lds_share(kern));
-    const size_t lds = interval_lds_bytes(b.mm_stride, 1, bb.log_cap);
-    if (lds > kMaxChoiceLds) return hipErrorInvalidValue;
-    if (maxs == 16 && slot_count <= 12) {
-        k_level_interval_batch<12><<<batch_grid((const void*)k_level_interval_batch<12>, lds,
-                                                cu_count, max_grid),
-                                     kLevelBlock, lds, st>>>(bb);
-        return hipGetLastError();
+    // two cells per lane when asked for (the caller checks that every
+    // shape's nodes hold a multiple of 128 cells) and the slot file allows it
+    if (b.kc == 2 && maxs <= 16 && interval_lds_bytes(b.mm_stride, 2, 1) <= kMaxChoiceLds) {
+        if (maxs == 8) return launch_interval_batch_kc<8, 2>(st, b, cu_count, max_grid);
+        if (slot_count <= 12) return launch_interval_batch_kc<12, 2>(st, b, cu_count, max_grid);
+        return launch_interval_batch_kc<16, 2>(st, b, cu_count, max_grid);
     }
-    FREP_DISPATCH_MAXS(maxs, (k_level_interval_batch<M><<<
-                                  batch_grid((const void*)k_level_interval_batch<M>, lds, cu_count,
-                                             max_grid),
-                                  kLevelBlock, lds, st>>>(bb)));
+    if (maxs == 16 && slot_count <= 12)
+        return launch_interval_batch_kc<12, 1>(st, b, cu_count, max_grid);
+    FREP_DISPATCH_MAXS(maxs, return (launch_interval_batch_kc<M, 1>(st, b, cu_count, max_grid)));
     return hipGetLastError();
 }
 

@@ -577,9 +577,9 @@ def test_voxelize_batch_assembly_point_kernels_vs_oracle(ctx, oracle, jobs):
 
 def test_voxelize_batch_tuned_and_graph_replays(ctx, oracle):
     """The same batch run repeatedly: the first calls time the autotuner's
-    candidate grid sizes, later ones replay a captured graph.  Every call,
-    with other work on the library's stream in between, gives the oracle's
-    trees."""
+    candidates (grid sizes x one or two cells per lane at the interval
+    levels), later ones replay a captured graph.  Every call, with other
+    work on the library's stream in between, gives the oracle's trees."""
     jobs = [
         (TAPES["csg24"](), shapes.make_grid([16, 8, 8])),
         (shapes.spheres_union(20, 12), shapes.make_grid([8, 4, 4])),
@@ -590,7 +590,7 @@ def test_voxelize_batch_tuned_and_graph_replays(ctx, oracle):
     ths = [ctx.upload(t) for t, _ in jobs]
     trees = [ctx.tree_create() for _ in jobs]
     other = ctx.upload(shapes.spheres_union(8, 1))
-    for call in range(7):
+    for call in range(11):
         ctx.voxelize_batch(ths, [g for _, g in jobs], trees)
         for tr, w in zip(trees, want):
             assert np.array_equal(ctx.export(tr), w), call

@@ -114,10 +114,12 @@ def _fusion_tapes():
         cases[name + "_on_const_b"] = [const, _word(op, 4, 2, 4), _word(12, 0, 4, 4)]
         cases[name + "_both_const"] = [const, _word(op, 4, 4, 4), _word(op, 0, 0, 4)]
         cases[name + "_new_slot"] = [const, _word(op, 5, 4, 1), _word(op, 0, 5, 4)]
-    fill = [_word(12, 6, 0, 0)] * 62
-    cases["const_at_word_62"] = fill + [const, _word(5, 4, 4, 1), _word(3, 0, 4, 0)]
-    cases["const_at_word_63"] = fill + [_word(12, 6, 1, 1), const, _word(5, 4, 4, 1),
-                                        _word(3, 0, 4, 0)]
+    # 62 words that stay live through shortening (slot 6 = y, 2y, y, ...,
+    # read at the end), so a shortened copy keeps the window edge too
+    fill = [_word(12, 6, 1, 1)] + [_word(3 + (i & 1), 6, 6, 1) for i in range(61)]
+    tail = [_word(5, 4, 4, 1), _word(3, 5, 4, 6), _word(3, 0, 5, 0)]
+    cases["const_at_word_62"] = fill + [const] + tail
+    cases["const_at_word_63"] = fill + [_word(3, 6, 6, 1), const] + tail
     cases["const_last"] = [_word(12, 5, 0, 0), _word(0, 0, 0, 0)]
     return {k: tape_from_words(np.array([len(w)] + w, dtype=np.uint32),
                                np.array([0.75], dtype=np.float32), 8)
@@ -136,6 +138,20 @@ def test_fused_const_handlers_bit_exact(ctx, oracle):
         ref = np.array([oracle.eval_point(tape, *q, 0.25) for q in p], dtype=np.float32)
         assert same_bits(got, ref), name
         ctx.tape_free(th)
+
+
+def test_fused_const_trees_match(ctx, oracle):
+    """The same tapes voxelized into three-level trees whose last level (8-wide
+    nodes) runs the assembly interpreter on the shortened copies the level
+    before writes: the trees equal the oracle's."""
+    grid = shapes.make_grid([4, 4, 8])
+    for name, tape in _fusion_tapes().items():
+        th = ctx.upload(tape)
+        tr = ctx.voxelize(th, grid)
+        got = ctx.export(tr)
+        ctx.tree_free(tr)
+        ctx.tape_free(th)
+        assert np.array_equal(got, oracle.Tree(tape, grid).canonical()), name
 
 
 @pytest.mark.parametrize("name", sorted(TAPES))
@@ -766,7 +782,8 @@ def test_interval_one_and_two_cells_per_lane_agree(ctx, oracle, name, dims, t):
 
 
 @pytest.mark.parametrize("name,dims,t", [("csg24", [16, 8, 8], 0.0), ("trig", [8, 8, 4], 0.5),
-                                          ("csg24", [8, 8, 8, 4], 0.0), ("spheres8", [4, 4, 4, 4], 0.0)])
+                                          ("csg24", [8, 8, 8, 4], 0.0), ("spheres8", [4, 4, 4, 4], 0.0),
+                                          ("spheres8", [8, 4, 8], 0.0)])
 def test_shortening_log_and_second_walk_agree(ctx, oracle, name, dims, t):
     """Shortened tapes are written from the wavefront's LDS log of the words
     its cells keep; a chunk whose log outgrows its LDS walks the parent tape

@@ -223,8 +223,14 @@ __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, ui
     uint32_t bw0 = window_load(tw, e - 64, n, lane);
     uint32_t bw1 = window_load(tw, e - 128, n, lane);
     uint32_t bw2 = window_load(tw, e - 192, n, lane);
+    // nothing in flight when the loop starts, so that the compiler's waits
+    // for the loads issued in the loop fall at the window's end
+    __builtin_amdgcn_s_waitcnt(0);
     uint32_t acc_lo[G], acc_hi[G];                     // the window's ballots (LOG)
     while (e > 0) {
+        // the window after next, loaded now and taken at this window's end
+        // (a load taken right after it is issued would wait for it there)
+        const uint32_t bw3 = window_load(tw, e - 256, n, lane);
         const int32_t s = e - 64;
         const int32_t lo = s > 0 ? s : 0;
         if constexpr (LOG) {
@@ -318,7 +324,7 @@ __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, ui
         }
         bw0 = bw1;
         bw1 = bw2;
-        bw2 = window_load(tw, e - 256, n, lane);
+        bw2 = bw3;
         e -= 64;
     }
 }
@@ -546,7 +552,15 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
     uint32_t fw2 = window_load(tw, 128, n, lane);
     float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
+    // nothing in flight when the loop starts, so that the compiler's waits
+    // for the loads issued in the loop fall at the window's end
+    __builtin_amdgcn_s_waitcnt(0);
     for (uint32_t s = 0; s < n; s += 64) {
+        // the next windows' requests go out now and are taken at this
+        // window's end (a load taken right after it is issued would wait
+        // for it there)
+        const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
+        const float fc2 = window_consts(fw2, a.consts);
         const uint32_t m = n - s < 64 ? n - s : 64;
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
@@ -608,8 +622,8 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
             shi[o] = r.hi;
         }
         fw0 = fw1; fc0 = fc1;
-        fw1 = fw2; fc1 = window_consts(fw2, a.consts);
-        fw2 = window_load(tw, (int32_t)s + 192, n, lane);
+        fw1 = fw2; fc1 = fc2;
+        fw2 = fw3;
     }
     ilo = slo[0];
     ihi = shi[0];
@@ -674,7 +688,15 @@ __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t nod
     uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
     uint32_t fw2 = window_load(tw, 128, n, lane);
     float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
+    // nothing in flight when the loop starts, so that the compiler's waits
+    // for the loads issued in the loop fall at the window's end
+    __builtin_amdgcn_s_waitcnt(0);
     for (uint32_t s = 0; s < n; s += 64) {
+        // the next windows' requests go out now and are taken at this
+        // window's end (a load taken right after it is issued would wait
+        // for it there)
+        const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
+        const float fc2 = window_consts(fw2, a.consts);
         const uint32_t m = n - s < 64 ? n - s : 64;
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
@@ -753,8 +775,8 @@ __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t nod
             shi1[o] = r1.hi;
         }
         fw0 = fw1; fc0 = fc1;
-        fw1 = fw2; fc1 = window_consts(fw2, a.consts);
-        fw2 = window_load(tw, (int32_t)s + 192, n, lane);
+        fw1 = fw2; fc1 = fc2;
+        fw2 = fw3;
     }
     // main.comp:88-108: lo > 0 empty, hi < 0 full, else ambiguous
     const bool full0 = shi0[0] < 0.0f, empty0 = slo0[0] > 0.0f, amb0 = !full0 && !empty0;
@@ -1725,7 +1747,15 @@ __device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t
     uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
     uint32_t fw2 = window_load(tw, 128, n, lane);
     float fc0 = window_consts(fw0, consts), fc1 = window_consts(fw1, consts);
+    // nothing in flight when the loop starts, so that the compiler's waits
+    // for the loads issued in the loop fall at the window's end
+    __builtin_amdgcn_s_waitcnt(0);
     for (uint32_t s = 0; s < n; s += 64) {
+        // the next windows' requests go out now and are taken at this
+        // window's end (a load taken right after it is issued would wait
+        // for it there)
+        const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
+        const float fc2 = window_consts(fw2, consts);
         const uint32_t m = n - s < 64 ? n - s : 64;
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
@@ -1749,8 +1779,8 @@ __device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t
             gw[o] = r.w;
         }
         fw0 = fw1; fc0 = fc1;
-        fw1 = fw2; fc1 = window_consts(fw2, consts);
-        fw2 = window_load(tw, (int32_t)s + 192, n, lane);
+        fw1 = fw2; fc1 = fc2;
+        fw2 = fw3;
     }
     return V4{gx[0], gy[0], gz[0], gw[0]};
 }

@@ -533,6 +533,9 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     const float lz = a.low[2] + (float)gz * a.cs;
     const uint32_t n = as_const(a.pool + toff)[0];
     const uint32_t* tw = a.pool + toff + 1;             // instruction words
+    // window 0 goes out with the length instead of after it: a tape starts
+    // below the pool's end and 64 words of padding follow the pool
+    const uint32_t raw0 = tw[lane];
     // The level ends with its longest tapes: their wavefronts take issue
     // priority over the shorter ones on the same SIMD (restored at the end).
     if (n >= 96) __builtin_amdgcn_s_setprio(3);
@@ -549,16 +552,18 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
     slo[3] = a.tape_time; shi[3] = a.tape_time;
     // three windows in flight: the current one (with its constants), the
     // next (constants requested), the one after (words requested)
-    uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
-    uint32_t fw2 = window_load(tw, 128, n, lane);
-    float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
-    // nothing in flight when the loop starts, so that the compiler's waits
-    // for the loads issued in the loop fall at the window's end
+    uint32_t fw0 = lane < n ? raw0 : 0xFFFFFFFFu;
+    float fc0 = window_consts(fw0, a.consts);               // windows 1 and 2 beside it
+    uint32_t fw1 = window_load(tw, 64, n, lane), fw2 = window_load(tw, 128, n, lane);
+    // nothing in flight when the loop starts but window 1's constants, so
+    // that the compiler's waits for the loads issued in the loop fall at the
+    // window's end
     __builtin_amdgcn_s_waitcnt(0);
+    float fc1 = window_consts(fw1, a.consts);
     for (uint32_t s = 0; s < n; s += 64) {
-        // the next windows' requests go out now and are taken at this
-        // window's end (a load taken right after it is issued would wait
-        // for it there)
+        // the constants of the window after next and the words of the one
+        // after that go out now and are taken at this window's end (a load
+        // taken right after it is issued would wait for it there)
         const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
         const float fc2 = window_consts(fw2, a.consts);
         const uint32_t m = n - s < 64 ? n - s : 64;
@@ -668,6 +673,9 @@ __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t nod
     const uint32_t gx1 = ncx * dim + cx, gy1 = ncy * dim + cy, gz1 = ncz * dim + cz;
     const uint32_t n = as_const(a.pool + toff)[0];
     const uint32_t* tw = a.pool + toff + 1;
+    // window 0 goes out with the length instead of after it: a tape starts
+    // below the pool's end and 64 words of padding follow the pool
+    const uint32_t raw0 = tw[lane];
     if (n >= 96) __builtin_amdgcn_s_setprio(3);
     else if (n >= 64) __builtin_amdgcn_s_setprio(2);
     else if (n >= 32) __builtin_amdgcn_s_setprio(1);
@@ -685,16 +693,18 @@ __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t nod
         slo0[2] = lz0; shi0[2] = lz0 + a.cs; slo1[2] = lz1; shi1[2] = lz1 + a.cs;
         slo0[3] = shi0[3] = slo1[3] = shi1[3] = a.tape_time;
     }
-    uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
-    uint32_t fw2 = window_load(tw, 128, n, lane);
-    float fc0 = window_consts(fw0, a.consts), fc1 = window_consts(fw1, a.consts);
-    // nothing in flight when the loop starts, so that the compiler's waits
-    // for the loads issued in the loop fall at the window's end
+    uint32_t fw0 = lane < n ? raw0 : 0xFFFFFFFFu;
+    float fc0 = window_consts(fw0, a.consts);               // windows 1 and 2 beside it
+    uint32_t fw1 = window_load(tw, 64, n, lane), fw2 = window_load(tw, 128, n, lane);
+    // nothing in flight when the loop starts but window 1's constants, so
+    // that the compiler's waits for the loads issued in the loop fall at the
+    // window's end
     __builtin_amdgcn_s_waitcnt(0);
+    float fc1 = window_consts(fw1, a.consts);
     for (uint32_t s = 0; s < n; s += 64) {
-        // the next windows' requests go out now and are taken at this
-        // window's end (a load taken right after it is issued would wait
-        // for it there)
+        // the constants of the window after next and the words of the one
+        // after that go out now and are taken at this window's end (a load
+        // taken right after it is issued would wait for it there)
         const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
         const float fc2 = window_consts(fw2, a.consts);
         const uint32_t m = n - s < 64 ? n - s : 64;
@@ -1736,6 +1746,9 @@ template <int MAXS>
 __device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t n_cap,
                                             const float* consts, float x, float y, float z,
                                             float t, uint32_t lane) {
+    // window 0 goes out with the length instead of after it (clamped to
+    // the readable words)
+    const uint32_t raw0 = tape_words[1u + lane < n_cap ? 1u + lane : (n_cap ? n_cap - 1u : 0u)];
     uint32_t n = as_const(tape_words)[0];
     if ((uint64_t)n + 1 > n_cap) n = n_cap ? (uint32_t)(n_cap - 1) : 0u;
     const uint32_t* tw = tape_words + 1;
@@ -1744,18 +1757,20 @@ __device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t
     gx[1] = 0.0f; gy[1] = 1.0f; gz[1] = 0.0f; gw[1] = y;
     gx[2] = 0.0f; gy[2] = 0.0f; gz[2] = 1.0f; gw[2] = z;
     gx[3] = 0.0f; gy[3] = 0.0f; gz[3] = 0.0f; gw[3] = t;
-    uint32_t fw0 = window_load(tw, 0, n, lane), fw1 = window_load(tw, 64, n, lane);
-    uint32_t fw2 = window_load(tw, 128, n, lane);
-    float fc0 = window_consts(fw0, consts), fc1 = window_consts(fw1, consts);
+    // the current window with its constants and the next window's words
+    // (the short tapes of a last level take one window)
+    uint32_t fw0 = lane < n ? raw0 : 0xFFFFFFFFu;
+    float fc0 = window_consts(fw0, consts);               // window 1 beside it
+    uint32_t fw1 = window_load(tw, 64, n, lane);
     // nothing in flight when the loop starts, so that the compiler's waits
     // for the loads issued in the loop fall at the window's end
     __builtin_amdgcn_s_waitcnt(0);
     for (uint32_t s = 0; s < n; s += 64) {
-        // the next windows' requests go out now and are taken at this
-        // window's end (a load taken right after it is issued would wait
-        // for it there)
-        const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
-        const float fc2 = window_consts(fw2, consts);
+        // the next window's constants and the words of the one after go out
+        // now and are taken at this window's end (a load taken right after
+        // it is issued would wait for it there)
+        const float fc1 = window_consts(fw1, consts);
+        const uint32_t fw2 = window_load(tw, (int32_t)s + 128, n, lane);
         const uint32_t m = n - s < 64 ? n - s : 64;
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
@@ -1779,8 +1794,7 @@ __device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t
             gw[o] = r.w;
         }
         fw0 = fw1; fc0 = fc1;
-        fw1 = fw2; fc1 = fc2;
-        fw2 = fw3;
+        fw1 = fw2;
     }
     return V4{gx[0], gy[0], gz[0], gw[0]};
 }

@@ -1899,7 +1899,7 @@ frep_status render_into(frep_ctx* ctx, TreeObj* t, const frep_camera* cam, char*
                                      hit_ids ? (unsigned long long*)(base + S.off_hit) : nullptr,
                                      normals ? (float*)(base + S.off_nrm) : nullptr);
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 2], ctx->stream));
-    HIP_TRY(launch_render(ctx->stream, a, t->maxs));
+    HIP_TRY(launch_render(ctx->stream, a, t->maxs, t->slot_count));
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 3], ctx->stream));
     if (rgba) HIP_TRY(hipMemcpyAsync(rgba, base, S.rgba_b, hipMemcpyDeviceToHost, ctx->stream));
     if (hit_ids)
@@ -2003,7 +2003,7 @@ frep_status frep_render_layer(frep_ctx* ctx, frep_tree_t h, const frep_camera* c
     uint8_t* dst = (uint8_t*)tg->d + (size_t)layer * tg->width * tg->height * 4;
     const RenderArgs a = render_args(t, cam, dst, nullptr, nullptr);
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 2], ctx->stream));
-    HIP_TRY(launch_render(ctx->stream, a, t->maxs));
+    HIP_TRY(launch_render(ctx->stream, a, t->maxs, t->slot_count));
     if (ctx->timing) HIP_TRY(hipEventRecord(ctx->ev[kMaxLevels + 3], ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if (ctx->timing)

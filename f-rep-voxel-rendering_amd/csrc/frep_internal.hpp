@@ -221,7 +221,7 @@ uint32_t point_k_used(uint32_t maxs, uint32_t dim);
 hipError_t launch_tree_stats(hipStream_t st, const uint32_t* nodes, uint32_t ns, uint32_t n_nodes,
                              const uint32_t* pool, uint64_t pool_words, bool interval,
                              unsigned long long* out);
-hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs);
+hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs, uint32_t slots);
 // Mean of n_layers rgba8 layers of `pixels` pixels each into out (rgba8).
 hipError_t launch_resolve(hipStream_t st, const void* layers, uint32_t pixels, uint32_t n_layers,
                           void* out);

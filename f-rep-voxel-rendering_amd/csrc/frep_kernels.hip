@@ -1774,8 +1774,8 @@ __device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t
         const uint32_t m = n - s < 64 ? n - s : 64;
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
-            const uint32_t op = w & 0xFFu, o = ((w >> 8) & 0xFFu) & (MAXS - 1);
-            const uint32_t ia = ((w >> 16) & 0xFFu) & (MAXS - 1), ib = (w >> 24) & (MAXS - 1);
+            const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);
+            const uint32_t ia = slot_in<MAXS>((w >> 16) & 0xFFu), ib = slot_in<MAXS>(w >> 24);
             const V4 A{gx[ia], gy[ia], gz[ia], gw[ia]};
             FREP_FENCE();
             V4 B = A;
@@ -2408,8 +2408,15 @@ hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t
     return hipGetLastError();
 }
 
-hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs) {
+hipError_t launch_render(hipStream_t st, const RenderArgs& a, uint32_t maxs, uint32_t slots) {
     const uint32_t tiles = ((a.width + 7) / 8) * ((a.height + 7) / 8);
+    // tapes of at most 10 slots on a 10-slot gradient file: 4 wavefronts per
+    // SIMD where the 16-slot file allows 3, more rays in flight to hide the
+    // walk's dependent loads
+    if (maxs == 16 && slots <= 10) {
+        k_render<10><<<tiles, 64, 0, st>>>(a);
+        return hipGetLastError();
+    }
     FREP_DISPATCH_MAXS(maxs, (k_render<M><<<tiles, 64, 0, st>>>(a)));
     return hipGetLastError();
 }

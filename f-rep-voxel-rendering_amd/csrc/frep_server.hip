@@ -254,7 +254,7 @@ frep_status enqueue_renders(frep_server* srv, Slot& S, frep_scene_t only) {
             t, &r.cam, o.b_rgba ? (uint8_t*)(d + o.o_rgba) : nullptr,
             o.b_hit ? (unsigned long long*)(d + o.o_hit) : nullptr,
             o.b_nrm ? (float*)(d + o.o_nrm) : nullptr);
-        HIP_TRY(launch_render(rs, a, t->maxs));
+        HIP_TRY(launch_render(rs, a, t->maxs, t->slot_count));
         if (o.b_rgba)
             HIP_TRY(hipMemcpyAsync(h + o.o_rgba, d + o.o_rgba, o.b_rgba, hipMemcpyDeviceToHost, rs));
         if (o.b_hit)

@@ -1684,14 +1684,18 @@ __device__ bool dda(const RenderArgs& a, const float o[3], const float d[3], Hit
     float t_curr = fmax_glm(t_enter, 0.0f) + kEps;
     float t_max = t_curr;
     uint32_t steps = 0;
+    // the current level's parameters stay in registers: a step's only
+    // memory round trip is the cell's words (with the next level's
+    // parameters beside them), not the per-lane level table first
+    const RenderLevel L0 = a.lv[0];
     while (t_max < t_leave - kEps) {
         uint32_t lvl = 0, node = 0;
+        RenderLevel L = L0;
         float np[3] = {a.low[0], a.low[1], a.low[2]};
         t_curr = t_max + kEps;
         t_max = t_leave;
         while (t_curr < t_max - kEps) {
             if (++steps > kMaxSteps) return false;
-            const RenderLevel& L = a.lv[lvl];
             const float cs = L.cs;
             const uint32_t dim = L.dim;
             const float s = t_curr + kEps;
@@ -1712,23 +1716,35 @@ __device__ bool dda(const RenderArgs& a, const float o[3], const float d[3], Hit
             const uint32_t W = dim * dim * dim / 32;
             const uint32_t bit = 1u << (idx & 31u);
             const bool last = (lvl + 1 == a.n_levels);
-            if (rec[4 + (idx >> 5)] & bit) {
+            // the cell's leaf word, interior word and child entry are loaded
+            // together (one memory round trip per step instead of up to three
+            // dependent ones); a last-level record has only the leaf mask
+            const uint32_t leaf_w = rec[4 + (idx >> 5)];
+            uint32_t int_w = 0, child = 0;
+            RenderLevel Ln = L;
+            if (!last) {
+                int_w = rec[4 + W + (idx >> 5)];
+                child = rec[4 + 2 * W + idx];
+                Ln = a.lv[lvl + 1];
+            }
+            if (leaf_w & bit) {
                 hit->t = t_curr;
                 hit->node = node;
                 hit->level = lvl;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) hit->gc[c] = rec[1 + c] * dim + (uint32_t)cell[c];
                 return true;
-            } else if (!last && (rec[4 + W + (idx >> 5)] & bit)) {
-                node = rec[4 + 2 * W + idx];
+            } else if (!last && (int_w & bit)) {
+                node = child;
                 // a child id outside the next level's buffer (a tree whose
                 // build ran out of capacity) ends the ray instead of reading
                 // past the buffer
-                if (node >= a.lv[lvl + 1].cap) return false;
+                if (node >= Ln.cap) return false;
 #pragma unroll
                 for (int c = 0; c < 3; ++c) np[c] = np[c] + (float)cell[c] * cs;
                 t_max = fmin_glm(tn[0], fmin_glm(tn[1], tn[2]));
                 ++lvl;
+                L = Ln;
             } else {
                 t_curr = fmin_glm(tn[0], fmin_glm(tn[1], tn[2]));
             }

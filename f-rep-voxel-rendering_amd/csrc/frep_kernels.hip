@@ -566,7 +566,7 @@ __device__ __forceinline__ void interval_chunk(const LevelArgs& a, uint32_t node
         // taken right after it is issued would wait for it there)
         const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
         const float fc2 = window_consts(fw2, a.consts);
-        const uint32_t m = n - s < 64 ? n - s : 64;
+        const uint32_t m = uni(n - s < 64 ? n - s : 64);   // a scalar loop bound
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
             // operand fields are slot indices except for CONST (pool
@@ -707,7 +707,7 @@ __device__ __forceinline__ void interval_chunk2(const LevelArgs& a, uint32_t nod
         // taken right after it is issued would wait for it there)
         const uint32_t fw3 = window_load(tw, (int32_t)s + 192, n, lane);
         const float fc2 = window_consts(fw2, a.consts);
-        const uint32_t m = n - s < 64 ? n - s : 64;
+        const uint32_t m = uni(n - s < 64 ? n - s : 64);   // a scalar loop bound
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
             const uint32_t op = w & 0xFFu, o = slot_field<MAXS>((w >> 8) & 0xFFu);
@@ -1787,7 +1787,7 @@ __device__ __forceinline__ V4 eval_gradient(const uint32_t* tape_words, uint64_t
         // it is issued would wait for it there)
         const float fc1 = window_consts(fw1, consts);
         const uint32_t fw2 = window_load(tw, (int32_t)s + 128, n, lane);
-        const uint32_t m = n - s < 64 ? n - s : 64;
+        const uint32_t m = uni(n - s < 64 ? n - s : 64);   // a scalar loop bound
         for (uint32_t j = 0; j < m; ++j) {
             const uint32_t w = rdlane(fw0, j);
             const uint32_t op = w & 0xFFu, o = slot_in<MAXS>((w >> 8) & 0xFFu);

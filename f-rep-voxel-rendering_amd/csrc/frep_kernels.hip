@@ -298,20 +298,39 @@ __device__ __forceinline__ void backward_walk(const uint32_t* tw, uint32_t n, ui
             const uint32_t ia = slot_field<MAXS>((w >> 16) & 0xFFu);
             const uint32_t ib = slot_field<MAXS>(w >> 24);
             uint32_t ow[G];
+            if (!is_mm) {
+                // operands a kept word reads, the same for every cell: none
+                // for CONST, both for ADD/SUB/MUL/DIV, else A
+                const uint32_t rd = op == OP_CONST ? MM_NONE
+                                    : (op - OP_ADD <= OP_DIV - OP_ADD) ? MM_BOTH : MM_FIRST;
+                const uint32_t abit = (rd & MM_FIRST) ? 1u << (ia & 31u) : 0u;
+                const uint32_t bbit = (rd & MM_SECOND) ? 1u << (ib & 31u) : 0u;
 #pragma unroll
-            for (int g = 0; g < G; ++g) {
-                // operands a kept word reads: none for CONST, both for
-                // ADD/SUB/MUL/DIV, the chosen ones for MIN/MAX, else A
-                const uint32_t sel = op == OP_CONST ? MM_NONE
-                                     : (op - OP_ADD <= OP_DIV - OP_ADD) ? MM_BOTH
-                                     : (is_mm ? ch[g] : MM_FIRST);
-                // lane masks instead of branches: clear the output slot
-                // first, then set the operand slots (shorten.comp:179)
-                const uint32_t keep = live[g] ? ~0u : 0u;
-                act[g][o >> 5] &= ~(obit & keep);
-                act[g][ia >> 5] |= ((sel & MM_FIRST) ? (1u << (ia & 31u)) : 0u) & keep;
-                act[g][ib >> 5] |= ((sel & MM_SECOND) ? (1u << (ib & 31u)) : 0u) & keep;
-                ow[g] = shortened_word(w, is_mm, ch[g], o, ib);
+                for (int g = 0; g < G; ++g) {
+                    // a kept word clears its output slot, then sets its
+                    // operand slots (shorten.comp:179)
+                    if constexpr (AW == 1) {
+                        act[g][0] = live[g] ? ((act[g][0] & ~obit) | abit | bbit) : act[g][0];
+                    } else {
+                        const uint32_t keep = live[g] ? ~0u : 0u;
+                        act[g][o >> 5] &= ~(obit & keep);
+                        act[g][ia >> 5] |= abit & keep;
+                        act[g][ib >> 5] |= bbit & keep;
+                    }
+                    ow[g] = w;
+                }
+            } else {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    // MIN/MAX: the operands this cell's choice reads; lane
+                    // masks instead of branches, output slot cleared first
+                    const uint32_t sel = ch[g];
+                    const uint32_t keep = live[g] ? ~0u : 0u;
+                    act[g][o >> 5] &= ~(obit & keep);
+                    act[g][ia >> 5] |= ((sel & MM_FIRST) ? (1u << (ia & 31u)) : 0u) & keep;
+                    act[g][ib >> 5] |= ((sel & MM_SECOND) ? (1u << (ib & 31u)) : 0u) & keep;
+                    ow[g] = shortened_word(w, is_mm, ch[g], o, ib);
+                }
             }
             on_word((uint32_t)i, mi, is_mm, live, L, ch, ow);
         }

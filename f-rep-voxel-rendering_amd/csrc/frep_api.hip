@@ -813,6 +813,14 @@ bool root_job(const BatchPlan& P, const Job& j) {
     return true;
 }
 
+// A shape whose last level (level L) runs 8 voxels per lane: a slot file of
+// at most 16 and nodes of a multiple of 8 wavefront chunks (its point
+// launch then takes 8 voxels per lane for every shape in it).
+bool point8_shape(const Job& j, uint32_t L) {
+    const uint32_t d = j.t->grid.dims[L];
+    return j.t->maxs <= 16 && (d * d * d / 64) % 8 == 0;
+}
+
 BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& idx,
                      uint32_t lanes, bool root_ops, uint32_t cu_count) {
     BatchPlan P;
@@ -883,13 +891,19 @@ BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& 
                     P.launches.push_back(la);
                 }
             }
-            for (int pt = 0; pt < 2; ++pt) {
-                BatchLaunch la{g, L, pt == 1, 0, 8, 1, P.bytes};
+            // interval launch, then the point launches: the shapes whose last
+            // level runs 8 voxels per lane, then the others (one or four per
+            // lane) on a kernel without the 8-voxel paths, whose few registers
+            // let many more wavefronts hide the interpreter's latency
+            for (int pt = 0; pt < 3; ++pt) {
+                BatchLaunch la{g, L, pt >= 1, 0, 8, 1, P.bytes};
+                la.pkset = (uint32_t)pt;
                 for (uint32_t k = 0; k < G.count; ++k) {
                     const Job& j = jobs[idx[G.first + k]];
                     const uint32_t nl = j.t->grid.n_levels;
-                    if (L >= nl || (L + 1 == nl) != (pt == 1)) continue;
+                    if (L >= nl || (L + 1 == nl) != (pt >= 1)) continue;
                     if (L == 0 && root_job(P, j)) continue;
+                    if (pt >= 1 && point8_shape(j, L) != (pt == 1)) continue;
                     ++la.count;
                     la.maxs = std::max(la.maxs, j.t->maxs);
                     la.slots = std::max(la.slots, j.t->slot_count);
@@ -915,6 +929,7 @@ BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& 
                         const Job& j = jobs[idx[G.first + k]];
                         const uint32_t nl = j.t->grid.n_levels;
                         if (L >= nl || L + 1 != nl) continue;
+                        if (point8_shape(j, L) != (la.pkset == 1)) continue;
                         all_asm = j.t->asm_ok && point_k_used(la.maxs, j.t->grid.dims[L]) == 8;
                         slots = std::max(slots, j.t->slot_count);
                     }
@@ -967,6 +982,7 @@ void fill_batch(const BatchPlan& P, const std::vector<Job>& jobs, const std::vec
             const uint32_t nl = j.t->grid.n_levels;
             if (la.level >= nl || (la.level + 1 == nl) != la.point) continue;
             if (la.level == 0 && root_job(P, j)) continue;
+            if (la.point && point8_shape(j, la.level) != (la.pkset == 1)) continue;
             LevelArgs a = level_args(j.t, la.level, la.mm);
             if (la.point) {
                 a.pk = point_k_used(la.maxs, a.dim);
@@ -1068,7 +1084,7 @@ frep_status enqueue_batch(frep_ctx* ctx, BatchStage* S, hipStream_t st, const Ba
                     la.cpb, kRootLanes[la.variant], la.lds));
             else if (la.point)
                 HIP_TRY(launch_level_point_batch(gs, b, la.maxs, ctx->cu_count, grid[li],
-                                                 la.asm_slots));
+                                                 la.asm_slots, la.pkset == 2));
             else
                 HIP_TRY(launch_level_interval_batch(gs, b, la.maxs, la.slots, ctx->cu_count,
                                                     grid[li]));

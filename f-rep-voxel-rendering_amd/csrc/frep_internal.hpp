@@ -212,7 +212,8 @@ hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint3
 // asm_slots > 0: every shape takes the assembly interpreter (8 voxels per
 // lane) and has at most asm_slots (<= 16) slots.
 hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
-                                    uint32_t cu_count, uint32_t max_grid, uint32_t asm_slots);
+                                    uint32_t cu_count, uint32_t max_grid, uint32_t asm_slots,
+                                    bool small);
 hipError_t launch_gather_counters(hipStream_t st, const InitArgs* d_args, uint32_t n,
                                   uint32_t* out);
 uint32_t point_k_used(uint32_t maxs, uint32_t dim);

@@ -1594,9 +1594,11 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point_batch_asm(BatchArgs
     }
 }
 
-template <int MAXS>
+// SMALL: the launch holds only shapes evaluated with fewer than 8 voxels per
+// lane; the 8-voxel paths are left out, and with them most registers.
+template <int MAXS, bool SMALL = false>
 __global__ __launch_bounds__(kLevelBlock) void k_level_point_batch(BatchArgs b) {
-    constexpr int KM = MAXS <= 16 ? 8 : (MAXS <= 32 ? 4 : 1);
+    constexpr int KM = SMALL ? (MAXS <= 32 ? 4 : 1) : (MAXS <= 16 ? 8 : (MAXS <= 32 ? 4 : 1));
     const uint32_t lane = lane_id();
     const uint32_t gwave = blockIdx.x * (kLevelBlock / 64) + uni(threadIdx.x >> 6);
     const uint32_t nwaves = gridDim.x * (kLevelBlock / 64);
@@ -1619,7 +1621,7 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point_batch(BatchArgs b) 
         if (skip) continue;
         const uint32_t local = chunk - start;
         const uint32_t node = local / cpw, sub = local - node * cpw;
-        if constexpr (MAXS <= 16) {
+        if constexpr (MAXS <= 16 && !SMALL) {
             if (a.use_asm) {
                 point_chunk<MAXS, 8, true>(a, node, sub, lane);
                 continue;
@@ -2419,8 +2421,16 @@ hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint3
 }
 
 hipError_t launch_level_point_batch(hipStream_t st, const BatchArgs& b, uint32_t maxs,
-                                    uint32_t cu_count, uint32_t max_grid, uint32_t asm_slots) {
+                                    uint32_t cu_count, uint32_t max_grid, uint32_t asm_slots,
+                                    bool small) {
     if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
+    if (small) {
+        FREP_DISPATCH_MAXS(maxs, (k_level_point_batch<M, true><<<
+                                      batch_grid((const void*)k_level_point_batch<M, true>, 0,
+                                                 cu_count, max_grid),
+                                      kLevelBlock, 0, st>>>(b)));
+        return hipGetLastError();
+    }
     if (asm_slots && asm_slots <= 16) {
         if (asm_slots <= 8)
             k_level_point_batch_asm<8><<<batch_grid((const void*)k_level_point_batch_asm<8>, 0,

@@ -986,7 +986,11 @@ void fill_batch(const BatchPlan& P, const std::vector<Job>& jobs, const std::vec
             LevelArgs a = level_args(j.t, la.level, la.mm);
             if (la.point) {
                 a.pk = point_k_used(la.maxs, a.dim);
-                a.use_asm = (j.t->asm_ok && a.pk == 8 && la.maxs <= 16) ? 1u : 0u;
+                // 8 voxels per lane in the kernel with the 8-voxel paths, one
+                // voxel per lane in the other (la.pkset 2)
+                a.use_asm = (j.t->asm_ok && la.maxs <= 16 && a.pk == (la.pkset == 2 ? 1u : 8u))
+                                ? 1u
+                                : 0u;
             }
             out[c++] = a;
         }

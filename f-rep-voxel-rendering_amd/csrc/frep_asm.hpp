@@ -193,8 +193,8 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  s_or_b64 %[p0], %[p0], vcc\n"                                                          \
     /* fused lanes: region 13 + (next op - 3), slot v[RB + 7] */                              \
     "  v_add_u32_e32 v[RB + 6], 13, v[RB + 6]\n"                                              \
-    "  v_lshlrev_b32_e32 v[RB + 6], 10, v[RB + 6]\n"                                          \
-    "  v_lshl_or_b32 v[RB + 6], v[RB + 7], 8, v[RB + 6]\n"                                    \
+    "  v_lshlrev_b32_e32 v[RB + 6], ROPSH, v[RB + 6]\n"                                       \
+    "  v_lshl_or_b32 v[RB + 6], v[RB + 7], RVSH, v[RB + 6]\n"                                 \
     "  v_cndmask_b32_e64 v[RA + 7], v[RA + 7], v[RB + 6], %[p0]\n"
 
 // Decode window W0 (words %[s] + lane), all lanes at once: RA+5 the fields,
@@ -211,9 +211,9 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  v_bfe_u32 v[RA + 2], v[WB + 0], 16, LG\n"                                              \
     "  v_bfe_u32 v[RA + 3], v[WB + 0], 24, LG\n"                                              \
     "  v_bfe_u32 v[RA + 4], v[WB + 0], 8, LG\n"                                               \
-    "  v_lshlrev_b32_e32 v[RA + 5], 3, v[RA + 2]\n"                                           \
-    "  v_lshl_or_b32 v[RA + 5], v[RA + 3], 11, v[RA + 5]\n"                                   \
-    "  v_lshl_or_b32 v[RA + 5], v[RA + 4], 19, v[RA + 5]\n"                                   \
+    "  v_lshlrev_b32_e32 v[RA + 5], KSH, v[RA + 2]\n"                                         \
+    "  v_lshl_or_b32 v[RA + 5], v[RA + 3], 8 + KSH, v[RA + 5]\n"                              \
+    "  v_lshl_or_b32 v[RA + 5], v[RA + 4], 16 + KSH, v[RA + 5]\n"                             \
     "  v_cmp_eq_u32_e32 vcc, v[RA + 4], v[RA + 3]\n"                                          \
     "  s_nop 1\n"                                                                             \
     "  v_cndmask_b32_e64 v[RA + 6], 0, 2, vcc\n"                                              \
@@ -225,8 +225,8 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  s_mov_b32 %[tmp], 0x1100fc0\n"                                                         \
     "  v_lshrrev_b32_e64 v[RA + 2], v[RA + 2], %[tmp]\n"                                      \
     "  v_and_b32_e32 v[RA + 6], v[RA + 6], v[RA + 2]\n"                                       \
-    "  v_lshlrev_b32_e32 v[RA + 7], 10, v[RA + 7]\n"                                          \
-    "  v_lshl_or_b32 v[RA + 7], v[RA + 6], 8, v[RA + 7]\n"                                    \
+    "  v_lshlrev_b32_e32 v[RA + 7], ROPSH, v[RA + 7]\n"                                       \
+    "  v_lshl_or_b32 v[RA + 7], v[RA + 6], RVSH, v[RA + 7]\n"                                 \
     FREP_ASM_FUSE                                                                             \
     "  v_add_u32_e32 v[RA + 2], %[s], v[RA + 1]\n"                                            \
     "  v_cmp_gt_u32_e32 vcc, %[n], v[RA + 2]\n"                                               \
@@ -258,7 +258,7 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  s_setpc_b64 vcc\n"
 // Handler slot of op OP, variant V (1 KiB per op, 256 bytes per variant;
 // .org fails the build if a handler outgrows its slot).
-#define FREP_ASM_SLOT(OP, V) "  .org .Lfa_base%= + " #OP " * 1024 + " #V " * 256\n"
+#define FREP_ASM_SLOT(OP, V) "  .org .Lfa_base%= + (" #OP " << ROPSH) + (" #V " << RVSH)\n"
 #define FREP_ASM_NOP FREP_ASM_NEXT FREP_ASM_GO
 // In-place binary op: v[S] = v[S] OP v[R].
 #define FREP_ASM_INPLACE(OPC, S, R)                                                           \
@@ -374,6 +374,11 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     ".set FS, 16\n"                                                                           \
     ".set MS, %[maxs]\n"                                                                      \
     ".set LG, %[lg]\n"                                                                        \
+    /* 8 voxels per lane: a slot is 8 registers, fields scaled by 8, 1 KiB                    \
+       of handlers per op and 256 bytes per variant */                                        \
+    ".set KSH, 3\n"                                                                           \
+    ".set ROPSH, 10\n"                                                                        \
+    ".set RVSH, 8\n"                                                                          \
     ".set RA, FS + 8*MS\n"                                                                    \
     ".set RB, RA + 8\n"                                                                       \
     ".set WB, RB + 8\n"                                                                       \
@@ -517,6 +522,212 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     ".Lfa_end%=:\n"                                                                           \
     "s_waitcnt vmcnt(0)\n"
 
+// One voxel per lane (FREP_ASM_POINT_BODY1, for last levels whose nodes
+// are one wavefront chunk of 64 voxels): the same window decode, CONST
+// fusion and threaded dispatch with a slot of one register (slot s is
+// register FS + s, fields unscaled) and handlers of one instruction where
+// the 8-voxel ones take eight, in slots of 512 bytes per op and 128 per
+// variant.
+#define FREP_ASM1_READ(R, S)                                                                  \
+    "  s_set_gpr_idx_on " S ", gpr_idx(SRC0)\n"                                               \
+    "  v_mov_b32 v[" R "], v[FS]\n"                                                           \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM1_READ_A(R) FREP_ASM1_READ(R, "%[sel]")
+#define FREP_ASM1_READ_B(R) FREP_ASM_IDX_B FREP_ASM1_READ(R, "%[tmp]")
+#define FREP_ASM1_WRITE(R)                                                                    \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
+    "  v_mov_b32 v[FS], v[" R "]\n"                                                           \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM1_INPLACE(OPC, S, R)                                                          \
+    "  s_set_gpr_idx_on " S ", gpr_idx(SRC0,DST)\n"                                           \
+    "  " OPC " v[FS], v[FS], v[" R "]\n"                                                      \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM1_BINARY(OP, OPC_GEN, OPC_A, OPC_B)                                           \
+    FREP_ASM_SLOT(OP, 0)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM1_READ_A("RA")                                                                    \
+    FREP_ASM_IDX_B                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(SRC0)\n"                                              \
+    "  " OPC_GEN " v[RB], v[FS], v[RA]\n"                                                     \
+    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM1_WRITE("RB")                                                                     \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(OP, 1)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM1_READ_B("RB")                                                                    \
+    FREP_ASM1_INPLACE(OPC_A, "%[sel]", "RB")                                                  \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(OP, 2)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM1_READ_A("RA")                                                                    \
+    FREP_ASM_IDX_B                                                                            \
+    FREP_ASM1_INPLACE(OPC_B, "%[tmp]", "RA")                                                  \
+    FREP_ASM_GO
+#define FREP_ASM1_MINMAX(LT_X, LT_Y)                                                          \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM1_READ_A("RA")                                                                    \
+    FREP_ASM1_READ_B("RB")                                                                    \
+    "  v_cmp_lt_f32_e64 %[p0], v[" LT_X "], v[" LT_Y "]\n"                                    \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e64 v[RB], v[RA], v[RB], %[p0]\n"                                        \
+    FREP_ASM1_WRITE("RB")                                                                     \
+    FREP_ASM_GO
+#define FREP_ASM1_CONST_WRITE                                                                 \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_nop 1\n"                                                                             \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
+    "  v_mov_b32 v[FS], vcc_lo\n"                                                             \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM1_SCALAR_OP(OPC, S)                                                           \
+    "  s_set_gpr_idx_on " S ", gpr_idx(SRC1,DST)\n"                                           \
+    "  " OPC " v[FS], vcc_lo, v[FS]\n"                                                        \
+    "  s_set_gpr_idx_off\n"
+#define FREP_ASM1_FUSED(R, OPC_VA, OPC_VB, OPC_CA, OPC_CB)                                    \
+    FREP_ASM_SLOT(R, 0)                                                                       \
+    FREP_ASM_FUSED_HEAD FREP_ASM1_CONST_WRITE                                                 \
+    FREP_ASM1_SCALAR_OP(OPC_VA, "%[nx]")                                                      \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(R, 1)                                                                       \
+    FREP_ASM_FUSED_HEAD FREP_ASM1_CONST_WRITE                                                 \
+    "  s_lshr_b32 %[tmp], %[nx], 8\n"                                                         \
+    FREP_ASM1_SCALAR_OP(OPC_VB, "%[tmp]")                                                     \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(R, 2)                                                                       \
+    FREP_ASM_FUSED_HEAD                                                                       \
+    "  s_lshr_b32 %[tmp], %[nx], 8\n"                                                         \
+    FREP_ASM1_READ("RB", "%[tmp]")                                                            \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
+    "  " OPC_CA " v[FS], vcc_lo, v[RB]\n"                                                     \
+    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(R, 3)                                                                       \
+    FREP_ASM_FUSED_HEAD                                                                       \
+    FREP_ASM1_READ("RB", "%[nx]")                                                             \
+    FREP_ASM_IDX_O                                                                            \
+    "  s_set_gpr_idx_on %[tmp], gpr_idx(DST)\n"                                               \
+    "  " OPC_CB " v[FS], vcc_lo, v[RB]\n"                                                     \
+    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM_GO
+#define FREP_ASM_POINT_BODY1                                                                  \
+    ".set FS, 16\n"                                                                           \
+    ".set MS, %[maxs]\n"                                                                      \
+    ".set LG, %[lg]\n"                                                                        \
+    ".set KSH, 0\n"                                                                           \
+    ".set ROPSH, 9\n"                                                                         \
+    ".set RVSH, 7\n"                                                                          \
+    ".set RA, FS + MS\n"                                                                      \
+    ".set RB, RA + 8\n"                                                                       \
+    ".set WB, RB + 8\n"                                                                       \
+    ".set NS, WB + 5\n"                                                                       \
+    ".set NT, WB + 6\n"                                                                       \
+    "s_cmp_eq_u32 %[ut], 0\n"                                                                 \
+    "s_cbranch_scc1 .Lfa_nt%=\n"                                                              \
+    "v_mov_b32 v[FS + 3], %[t]\n"                                                             \
+    ".Lfa_nt%=:\n"                                                                            \
+    "s_mov_b32 %[cls], 0x260\n"                                                               \
+    "s_mov_b32 %[tiny], 0xf800000\n"                                                          \
+    "s_mov_b32 %[sgn], 0x80000000\n"                                                          \
+    "s_getpc_b64 vcc\n"                                                                       \
+    ".Lfa_pc%=:\n"                                                                            \
+    "s_add_u32 %[blo], vcc_lo, .Lfa_base%= - .Lfa_pc%=\n"                                     \
+    "s_addc_u32 %[bhi], vcc_hi, 0\n"                                                          \
+    "s_cmp_eq_u32 %[n], 0\n"                                                                  \
+    "s_cbranch_scc1 .Lfa_end%=\n"                                                             \
+    "s_sub_u32 %[nm1], %[n], 1\n"                                                             \
+    "s_mov_b32 %[s], 0\n"                                                                     \
+    "s_cmp_le_u32 %[n], 64\n"                                                                 \
+    "s_cbranch_scc1 .Lfa_dec%=\n"                                                             \
+    "s_mov_b32 %[sidx], 64\n"                                                                 \
+    FREP_ASM_LOADW("WB + 1")                                                                  \
+    "s_mov_b32 %[sidx], 128\n"                                                                \
+    FREP_ASM_LOADW("WB + 2")                                                                  \
+    "s_waitcnt vmcnt(0)\n"                                                                    \
+    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
+    "s_waitcnt vmcnt(0)\n"                                                                    \
+    ".Lfa_dec%=:\n"                                                                           \
+    FREP_ASM_DECODE                                                                           \
+    ".Lfa_base%=:\n"                                                                          \
+    /* op 0: CONST */                                                                         \
+    FREP_ASM_NEXT                                                                             \
+    "  v_readlane_b32 vcc_lo, v[WB + 3], %[j]\n"                                              \
+    FREP_ASM1_CONST_WRITE                                                                     \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(1, 0) FREP_ASM_NOP                                                          \
+    FREP_ASM_SLOT(2, 0) FREP_ASM_NOP                                                          \
+    FREP_ASM1_BINARY(3, "v_add_f32_e32", "v_add_f32_e32", "v_add_f32_e32")                    \
+    FREP_ASM1_BINARY(4, "v_subrev_f32_e32", "v_sub_f32_e32", "v_subrev_f32_e32")              \
+    FREP_ASM1_BINARY(5, "v_mul_f32_e32", "v_mul_f32_e32", "v_mul_f32_e32")                    \
+    FREP_ASM_SLOT(6, 0) FREP_ASM_NOP                                                          \
+    FREP_ASM_SLOT(7, 0) FREP_ASM1_MINMAX("RB", "RA")                                          \
+    FREP_ASM_SLOT(8, 0) FREP_ASM1_MINMAX("RA", "RB")                                          \
+    FREP_ASM_SLOT(9, 0) FREP_ASM_NOP                                                          \
+    FREP_ASM_SLOT(10, 0)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    "  s_set_gpr_idx_on %[sel], gpr_idx(SRC0)\n"                                              \
+    "  v_xor_b32_e64 v[RB], v[FS], %[sgn]\n"                                                  \
+    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM1_WRITE("RB")                                                                     \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(10, 1)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    "  s_set_gpr_idx_on %[sel], gpr_idx(SRC0,DST)\n"                                          \
+    "  v_xor_b32_e64 v[FS], v[FS], %[sgn]\n"                                                  \
+    "  s_set_gpr_idx_off\n"                                                                   \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(11, 0)                                                                      \
+    "  s_branch .Lfa_sqrt%=\n"                                                                \
+    FREP_ASM_SLOT(12, 0)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM1_READ_A("RA")                                                                    \
+    FREP_ASM1_WRITE("RA")                                                                     \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(12, 1) FREP_ASM_NOP                                                         \
+    FREP_ASM1_FUSED(13, "v_add_f32_e32", "v_add_f32_e32", "v_add_f32_e32", "v_add_f32_e32")    \
+    FREP_ASM1_FUSED(14, "v_subrev_f32_e32", "v_sub_f32_e32", "v_sub_f32_e32", "v_subrev_f32_e32") \
+    FREP_ASM1_FUSED(15, "v_mul_f32_e32", "v_mul_f32_e32", "v_mul_f32_e32", "v_mul_f32_e32")    \
+    FREP_ASM_SLOT(16, 0)                                                                      \
+    ".Lfa_adv%=:\n"                                                                           \
+    "  s_add_u32 %[s], %[s], 64\n"                                                            \
+    "  s_waitcnt vmcnt(0)\n"                                                                  \
+    "  v_mov_b32 v[WB + 0], v[WB + 1]\n"                                                      \
+    "  v_mov_b32 v[WB + 3], v[WB + 4]\n"                                                      \
+    "  v_mov_b32 v[WB + 1], v[WB + 2]\n"                                                      \
+    FREP_ASM_GATHER("WB + 4", "WB + 1")                                                       \
+    "  s_add_u32 %[sidx], %[s], 128\n"                                                        \
+    FREP_ASM_LOADW("WB + 2")                                                                  \
+    FREP_ASM_DECODE                                                                           \
+    /* correctly rounded sqrt (the sequence of the 8-voxel body, one voxel) */                \
+    ".Lfa_sqrt%=:\n"                                                                          \
+    FREP_ASM_NEXT                                                                             \
+    FREP_ASM1_READ_A("RA")                                                                    \
+    "  v_mul_f32_e32 v[RB + 0], 0x4f800000, v[RA]\n"                                          \
+    "  v_cmp_gt_f32_e32 vcc, %[tiny], v[RA]\n"                                                \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e32 v[RB + 0], v[RA], v[RB + 0], vcc\n"                                  \
+    "  v_sqrt_f32_e32 v[RB + 1], v[RB + 0]\n"                                                 \
+    "  s_nop 0\n"                                                                             \
+    "  v_add_u32_e32 v[RB + 2], -1, v[RB + 1]\n"                                              \
+    "  v_fma_f32 v[RB + 4], -v[RB + 2], v[RB + 1], v[RB + 0]\n"                               \
+    "  v_add_u32_e32 v[RB + 3], 1, v[RB + 1]\n"                                               \
+    "  v_cmp_ge_f32_e64 %[p0], 0, v[RB + 4]\n"                                                \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e64 v[RB + 2], v[RB + 1], v[RB + 2], %[p0]\n"                            \
+    "  v_fma_f32 v[RB + 1], -v[RB + 3], v[RB + 1], v[RB + 0]\n"                               \
+    "  v_cmp_lt_f32_e64 %[p0], 0, v[RB + 1]\n"                                                \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e64 v[RB + 2], v[RB + 2], v[RB + 3], %[p0]\n"                            \
+    "  v_mul_f32_e32 v[RB + 3], 0x37800000, v[RB + 2]\n"                                      \
+    "  v_cndmask_b32_e32 v[RB + 2], v[RB + 2], v[RB + 3], vcc\n"                              \
+    "  v_cmp_class_f32_e64 vcc, v[RB + 0], %[cls]\n"                                          \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e32 v[RA], v[RB + 2], v[RB + 0], vcc\n"                                  \
+    FREP_ASM1_WRITE("RA")                                                                     \
+    FREP_ASM_GO                                                                               \
+    ".Lfa_end%=:\n"                                                                           \
+    "s_waitcnt vmcnt(0)\n"
+
 // Scalar registers of the point interpreter's assembly block.
 #define FREP_ASM_POINT_DECLS                                                                  \
     uint32_t sel, ntg, nx, j, tmp, blo, bhi, s, sidx, nm1, cls, tiny, sgn;                    \
@@ -608,6 +819,41 @@ __device__ __forceinline__ v8f run_point_tape_asm8(const uint32_t* tape, const f
                        [lg] "n"(3)
                      : FREP_ASM_CLOBBER_40_63, FREP_ASM_CLOBBER_64_95, FREP_ASM_CLOBBER_W8,
                        "vcc", "memory");
+    }
+    return x;
+}
+
+// One voxel per lane (FREP_ASM_POINT_BODY1): x, y, z are the lane's point,
+// the rest as run_point_tape_asm8.  MAXS must be 8 or 16.
+template <int MAXS>
+__device__ __forceinline__ float run_point_tape_asm1(const uint32_t* tape, const float* consts,
+                                                     float t, float x, float y, float z,
+                                                     uint32_t w0, float c0, uint32_t use_t = 1) {
+    static_assert(MAXS == 8 || MAXS == 16, "one-voxel assembly path covers 8 or 16 slots");
+    typedef const __attribute__((address_space(4))) uint32_t* cword_t;
+    const uint32_t n = ((cword_t)tape)[0];
+    const uint64_t twp = (uint64_t)(tape + 1), cp = (uint64_t)consts;
+    FREP_ASM_POINT_DECLS;
+    if constexpr (MAXS == 16) {
+        asm volatile(FREP_ASM_POINT_BODY1
+                     : "+{v16}"(x), "+{v17}"(y), "+{v18}"(z), "+{v48}"(w0), "+{v51}"(c0),
+                       FREP_ASM_POINT_SCALARS
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
+                       [lg] "n"(4)
+                     : "v19", "v20", "v21", "v22", "v23", "v24", "v25", "v26", "v27", "v28",
+                       "v29", "v30", "v31", "v32", "v33", "v34", "v35", "v36", "v37", "v38",
+                       "v39", "v40", "v41", "v42", "v43", "v44", "v45", "v46", "v47",
+                       "v49", "v50", "v52", "v53", "v54", "vcc", "memory");
+    } else {
+        asm volatile(FREP_ASM_POINT_BODY1
+                     : "+{v16}"(x), "+{v17}"(y), "+{v18}"(z), "+{v40}"(w0), "+{v43}"(c0),
+                       FREP_ASM_POINT_SCALARS
+                     : [tw] "s"(twp), [cst] "s"(cp), [n] "s"(n), [t] "v"(t), [ut] "s"(use_t), [maxs] "n"(MAXS),
+                       [lg] "n"(3)
+                     : "v19", "v20", "v21", "v22", "v23", "v24", "v25", "v26", "v27", "v28",
+                       "v29", "v30", "v31", "v32", "v33", "v34", "v35", "v36", "v37", "v38",
+                       "v39",
+                       "v41", "v42", "v44", "v45", "v46", "vcc", "memory");
     }
     return x;
 }

@@ -1328,7 +1328,8 @@ __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t nod
 template <int MAXS, int K, bool ASM>
 __device__ __forceinline__ void point_chunk(const LevelArgs& a, uint32_t node, uint32_t sub,
                                             uint32_t lane) {
-    static_assert(!ASM || (K == 8 && (MAXS == 8 || MAXS == 10 || MAXS == 16)),
+    static_assert(!ASM || (K == 8 && (MAXS == 8 || MAXS == 10 || MAXS == 16)) ||
+                      (K == 1 && (MAXS == 8 || MAXS == 16)),
                   "assembly path shape");
     const uint32_t dim = a.dim;
     uint32_t* rec = a.nodes + (size_t)node * a.ns;
@@ -1347,11 +1348,18 @@ __device__ __forceinline__ void point_chunk(const LevelArgs& a, uint32_t node, u
         const uint64_t B = __ballot(d < 0.0f);
         store_mask64(rec + 4 + (sub * K + (uint32_t)k) * 2, B, lane);
     };
-    if constexpr (ASM) {
+    if constexpr (ASM && K == 8) {
         uint32_t w0;
         float c0;
         point_window0(a.pool + toff, a.consts, lane, &w0, &c0);
         point_chunk_asm<MAXS>(a, node, sub, lane, toff, ncx, ncy, ncz, w0, c0);
+    } else if constexpr (ASM) {
+        uint32_t w0;
+        float c0;
+        point_window0(a.pool + toff, a.consts, lane, &w0, &c0);
+        const float3 p = init(0);
+        out(0, run_point_tape_asm1<MAXS>(a.pool + toff, a.consts, a.tape_time, p.x, p.y, p.z, w0,
+                                         c0, a.reads_t));
     } else {
         run_point_tape<MAXS, K>(as_const(a.pool + toff), as_const(a.consts), a.tape_time, init,
                                 out);
@@ -1368,7 +1376,7 @@ __device__ __forceinline__ void level_point_body(const LevelArgs& a) {
     if (n_nodes > a.node_cap) n_nodes = a.node_cap;
     const uint32_t cpw = a.dim * a.dim * a.dim / (64 * K);
     const uint32_t total = n_nodes * cpw;
-    if constexpr (ASM) {
+    if constexpr (ASM && K == 8) {
         // Software pipeline over this wavefront's chunks c, c1, c2, c3: while
         // the interpreter runs chunk c it gathers c1's constants (from c1's
         // window, loaded during the previous chunk), loads c2's tape window
@@ -1624,6 +1632,12 @@ __global__ __launch_bounds__(kLevelBlock) void k_level_point_batch(BatchArgs b) 
         if constexpr (MAXS <= 16 && !SMALL) {
             if (a.use_asm) {
                 point_chunk<MAXS, 8, true>(a, node, sub, lane);
+                continue;
+            }
+        }
+        if constexpr (MAXS <= 16 && SMALL) {
+            if (a.use_asm) {
+                point_chunk<MAXS, 1, true>(a, node, sub, lane);
                 continue;
             }
         }
@@ -2276,6 +2290,18 @@ hipError_t launch_level_point(hipStream_t st, const LevelArgs& a, uint32_t maxs,
     }
     if (asm_ok && k == 8 && maxs == 16) {
         k_level_point<16, 8, true><<<point_grid<16, 8, true>(cu_count, max_grid), kLevelBlock, 0,
+                                     st>>>(a);
+        return hipGetLastError();
+    }
+    // one voxel per lane (nodes of one wavefront chunk): the same
+    // interpreter with one-register slots
+    if (asm_ok && k == 1 && maxs == 8) {
+        k_level_point<8, 1, true><<<point_grid<8, 1, true>(cu_count, max_grid), kLevelBlock, 0,
+                                    st>>>(a);
+        return hipGetLastError();
+    }
+    if (asm_ok && k == 1 && maxs == 16) {
+        k_level_point<16, 1, true><<<point_grid<16, 1, true>(cu_count, max_grid), kLevelBlock, 0,
                                      st>>>(a);
         return hipGetLastError();
     }

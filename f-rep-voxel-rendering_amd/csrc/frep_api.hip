@@ -909,16 +909,17 @@ BatchPlan plan_batch(const std::vector<Job>& jobs, const std::vector<uint32_t>& 
                     la.slots = std::max(la.slots, j.t->slot_count);
                     la.mm = std::max(la.mm, j.mm_stride);
                 }
-                if (la.count && !la.point) {
-                    bool ok = la.maxs <= 16;
-                    for (uint32_t k = 0; k < G.count && ok; ++k) {
+                if (la.count && !la.point && la.maxs <= 16) {
+                    // two cells per lane for the shapes whose nodes allow it
+                    bool any = false;
+                    for (uint32_t k = 0; k < G.count && !any; ++k) {
                         const Job& j = jobs[idx[G.first + k]];
                         const uint32_t nl = j.t->grid.n_levels;
                         if (L >= nl || L + 1 == nl || (L == 0 && root_job(P, j))) continue;
                         const uint32_t d = j.t->grid.dims[L];
-                        ok = (d * d * d) % 128u == 0;
+                        any = (d * d * d) % 128u == 0;
                     }
-                    la.kc2_ok = ok;
+                    la.kc2_ok = any;
                 }
                 if (la.count && la.point) {
                     // the assembly-only kernel when every shape takes the

@@ -174,8 +174,9 @@ struct BatchLaunch {
                                        // at most this many slots (0: not all)
     size_t lds = 0;                    // dynamic LDS per block
     size_t starts_off = 0;             // byte offset of the block starts (count + 1 words)
-    bool kc2_ok = false;               // interval launch: two cells per lane possible (nodes of a
-                                       // multiple of 128 cells, slot files of at most 16)
+    bool kc2_ok = false;               // interval launch: two cells per lane possible (for the
+                                       // shapes with nodes of a multiple of 128 cells; slot
+                                       // files of at most 16)
     uint32_t pkset = 0;                // point launch: the shapes whose last level runs 8 voxels
                                        // per lane (1) or the others (2), each in its own launch
 };

@@ -1498,6 +1498,12 @@ typedef const __attribute__((address_space(4))) LevelArgs* cargs_t;
 // launch ended), so every wavefront maps chunk indices to shapes the same
 // way; overflow flags, which this launch may set, are checked per shape
 // while walking the chunks.
+// Cells per lane of a shape whose nodes hold `cells` cells in an interval
+// launch of kc cells per lane: two only for nodes of a multiple of 128 cells.
+__device__ __forceinline__ uint32_t interval_kc(uint32_t kc, uint32_t cells) {
+    return kc == 2 && cells % 128u == 0 ? 2u : 1u;
+}
+
 template <bool POINT>
 __device__ __forceinline__ uint32_t plan_lane(const BatchArgs& b, uint32_t lane, uint64_t* cbase) {
     *cbase = 0;
@@ -1508,7 +1514,7 @@ __device__ __forceinline__ uint32_t plan_lane(const BatchArgs& b, uint32_t lane,
     nn = nn < A->node_cap ? nn : A->node_cap;
     const uint32_t cells = A->dim * A->dim * A->dim;
     if (!POINT) *cbase = child_tape_base(cnt, A->level);
-    return nn * (POINT ? cells / (64u * A->pk) : cells / (64u * b.kc));
+    return nn * (POINT ? cells / (64u * A->pk) : cells / (64u * interval_kc(b.kc, cells)));
 }
 
 // Scalar (constant address space) copy of one shape's arguments.
@@ -1544,7 +1550,7 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 7 : 1)))) void k_
     uint2* logL = (uint2*)(lds_choice + (size_t)(kLevelBlock / 64) * b.mm_stride * KC) +
                   (size_t)wib * b.log_cap * KC;
     cargs_t shapes = (cargs_t)b.shapes;
-    uint32_t cur = 0xFFFFFFFFu, start = 0, cpn = 1;
+    uint32_t cur = 0xFFFFFFFFu, start = 0, cpn = 1, kcs = 1;
     uint64_t cb = 0;
     bool skip = false;
     LevelArgs a{};
@@ -1555,7 +1561,8 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 7 : 1)))) void k_
             a = load_args(shapes + s);
             start = s ? rdlane(incl, s - 1) : 0u;
             cb = rdlane64(cbase, s);
-            cpn = a.dim * a.dim * a.dim / (64 * KC);
+            kcs = interval_kc(KC, a.dim * a.dim * a.dim);
+            cpn = a.dim * a.dim * a.dim / (64 * kcs);
             // a shape that ran out of capacity at an earlier level is rebuilt
             // by the host; one that runs out during this launch goes on, so
             // its counters report what the whole level needs
@@ -1564,10 +1571,16 @@ __attribute__((amdgpu_waves_per_eu(KC == 2 ? 4 : (MAXS <= 12 ? 7 : 1)))) void k_
         if (skip) continue;
         const uint32_t local = chunk - start;
         const uint32_t node = local / cpn, sub = local - node * cpn;
-        if constexpr (KC == 2)
-            interval_chunk2<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap, WaveReserve{});
-        else
-            interval_chunk<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap, WaveReserve{});
+        // a shape of 64-cell nodes in a two-cell launch walks one cell per
+        // lane, in the first half of its wavefront's LDS rows and log
+        if constexpr (KC == 2) {
+            if (kcs == 2) {
+                interval_chunk2<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap,
+                                      WaveReserve{});
+                continue;
+            }
+        }
+        interval_chunk<MAXS>(a, node, sub, cb, lane, scr, logL, b.log_cap, WaveReserve{});
     }
 }
 
@@ -2433,8 +2446,8 @@ hipError_t launch_level_interval_batch(hipStream_t st, const BatchArgs& b, uint3
                                        uint32_t slot_count, uint32_t cu_count, uint32_t max_grid) {
     if (b.n_shapes == 0 || b.n_shapes > kBatchMax) return hipErrorInvalidValue;
     if (interval_lds_bytes(b.mm_stride, 1, 1) > kMaxChoiceLds) return hipErrorInvalidValue;
-    // two cells per lane when asked for (the caller checks that every
-    // shape's nodes hold a multiple of 128 cells) and the slot file allows it
+    // two cells per lane when asked for and the slot file allows it (shapes
+    // whose nodes hold a multiple of 128 cells; the others take one)
     if (b.kc == 2 && maxs <= 16 && interval_lds_bytes(b.mm_stride, 2, 1) <= kMaxChoiceLds) {
         if (maxs == 8) return launch_interval_batch_kc<8, 2>(st, b, cu_count, max_grid);
         if (slot_count <= 12) return launch_interval_batch_kc<12, 2>(st, b, cu_count, max_grid);

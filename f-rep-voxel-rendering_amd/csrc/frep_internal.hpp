@@ -138,8 +138,9 @@ struct BatchArgs {
     uint32_t n_shapes;
     uint32_t mm_stride;       // LDS choice rows per wavefront (max over shapes)
     uint32_t log_cap;         // shortening-log entries per wavefront (set at launch)
-    uint32_t kc;              // interval launch: cells per lane, 1 or 2 (every shape's nodes
-                              // a multiple of 128 cells and slot files of at most 16 for 2)
+    uint32_t kc;              // interval launch: cells per lane, 1 or 2 (2: slot files of at
+                              // most 16; shapes whose nodes are not a multiple of 128 cells
+                              // still take one)
 };
 constexpr uint32_t kBatchMax = 64;
 

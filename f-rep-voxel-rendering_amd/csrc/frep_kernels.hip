@@ -1294,9 +1294,17 @@ __device__ __forceinline__ void point_chunk_asm(const LevelArgs& a, uint32_t nod
         // lane % 8), so y and z are the same for the lane's 8 voxels
         const float y = a.low[1] + (float)(ncy * 8u + (lane >> 3)) * a.cs;
         const float z = a.low[2] + (float)(ncz * 8u + (lane & 7u)) * a.cs;
+        // x of voxel k is the same in every lane: lanes 0..7 of each row of
+        // 16 compute the eight values, and a DPP row_newbcast:k move (lane k
+        // of its row to the whole row) fills register k
+        const int xl = __builtin_bit_cast(int, a.low[0] + (float)(ncx * 8u + (lane & 7u)) * a.cs);
+#define FREP_XBCAST(k) \
+    xs[k] = __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(xl, 0x150 + k, 0xF, 0xF, false));
+        FREP_XBCAST(0) FREP_XBCAST(1) FREP_XBCAST(2) FREP_XBCAST(3)
+        FREP_XBCAST(4) FREP_XBCAST(5) FREP_XBCAST(6) FREP_XBCAST(7)
+#undef FREP_XBCAST
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            xs[k] = a.low[0] + (float)(ncx * 8u + (uint32_t)k) * a.cs;
             ys[k] = y;
             zs[k] = z;
         }

@@ -140,11 +140,13 @@ def test_fused_const_handlers_bit_exact(ctx, oracle):
         ctx.tape_free(th)
 
 
-def test_fused_const_trees_match(ctx, oracle):
-    """The same tapes voxelized into three-level trees whose last level (8-wide
-    nodes) runs the assembly interpreter on the shortened copies the level
-    before writes: the trees equal the oracle's."""
-    grid = shapes.make_grid([4, 4, 8])
+@pytest.mark.parametrize("dims", [[4, 4, 8], [4, 8, 4]])
+def test_fused_const_trees_match(ctx, oracle, dims):
+    """The same tapes voxelized into three-level trees whose last level runs
+    the assembly interpreter on the shortened copies the level before writes
+    (8-wide nodes: 8 voxels per lane; 4-wide nodes: the one-voxel form): the
+    trees equal the oracle's."""
+    grid = shapes.make_grid(dims)
     for name, tape in _fusion_tapes().items():
         th = ctx.upload(tape)
         tr = ctx.voxelize(th, grid)

@@ -11,7 +11,9 @@
 // Register file: slot s of point k lives in v[16 + 8*s + k] (slot-major, the
 // 8 points of a slot adjacent), so one s_set_gpr_idx window (M0 = 8*s)
 // covers an operand for all 8 points.  Points arrive in slots 0..2 (x, y, z,
-// pinned operands) and the density leaves in slot 0.
+// pinned operands) and the density leaves in slot 0.  A second body
+// (FREP_ASM_POINT_BODY1, run_point_tape_asm1) evaluates one point per lane
+// the same way, slot s in v[16 + s], for nodes of one wavefront chunk.
 //
 // Tape words: staged in VGPR windows of 64 words (lane l of W0 holds word
 // s + l); the constant of every CONST word of a window is gathered into C0
@@ -102,8 +104,9 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 // the rate of v_mov_b32, and the index window offsets 64-bit operands like
 // 32-bit ones); the slot file, RA and RB start at even registers.
 //
-// A word's decoded fields sit in one SGPR: 8*a in bits 0..7, 8*b in 8..15,
-// 8*o in 16..23.  s_set_gpr_idx_on takes its index from bits 0..7 of the
+// A word's decoded fields sit in one SGPR: a, b, o scaled by the slot size
+// (8 registers, or 1 in the one-voxel body; shift KSH) in bits 0..7, 8..15
+// and 16..23.  s_set_gpr_idx_on takes its index from bits 0..7 of the
 // register, so operand A indexes from it directly and B / O after a shift.
 #define FREP_ASM_IDX_B "  s_lshr_b32 %[tmp], %[sel], 8\n"
 #define FREP_ASM_IDX_O "  s_lshr_b32 %[tmp], %[sel], 16\n"
@@ -198,7 +201,7 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  v_cndmask_b32_e64 v[RA + 7], v[RA + 7], v[RB + 6], %[p0]\n"
 
 // Decode window W0 (words %[s] + lane), all lanes at once: RA+5 the fields,
-// RA+7 the handler offset -- (op & 15) * 1024 + variant * 256, variant 1
+// RA+7 the handler offset -- (op & 15) << ROPSH plus variant << RVSH, variant 1
 // when the output slot is operand A's, else 2 when it is B's, else 0, kept
 // only for ops with in-place forms (the mask of allowed variants is 2 bits
 // per op in the word 0x1100FC0: ADD, SUB, MUL both, NEG and COPY the first)
@@ -256,7 +259,7 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  s_addc_u32 vcc_hi, %[bhi], 0\n"                                                        \
     "  s_add_u32 %[j], %[j], 1\n"                                                             \
     "  s_setpc_b64 vcc\n"
-// Handler slot of op OP, variant V (1 KiB per op, 256 bytes per variant;
+// Handler slot of op OP, variant V (1 << ROPSH bytes per op, 1 << RVSH per variant;
 // .org fails the build if a handler outgrows its slot).
 #define FREP_ASM_SLOT(OP, V) "  .org .Lfa_base%= + (" #OP " << ROPSH) + (" #V " << RVSH)\n"
 #define FREP_ASM_NOP FREP_ASM_NEXT FREP_ASM_GO

@@ -202,9 +202,10 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
 
 // Decode window W0 (words %[s] + lane), all lanes at once: RA+5 the fields,
 // RA+7 the handler offset -- (op & 15) << ROPSH plus variant << RVSH, variant 1
-// when the output slot is operand A's, else 2 when it is B's, else 0, kept
-// only for ops with in-place forms (the mask of allowed variants is 2 bits
-// per op in the word 0x1100FC0: ADD, SUB, MUL both, NEG and COPY the first)
+// when the output slot is operand A's, else 2 when it is B's, else 0, and 3
+// when A, B and the output are one slot; kept only for ops with in-place
+// forms (the mask of allowed variants is 2 bits per op in the word
+// 0x1100FC0: ADD, SUB, MUL all, NEG and COPY the first, which 3 becomes)
 // -- or the end for words past the tape.  NS / NT get them shifted down a
 // lane, NT's last lane the next window's decode (ADV) or the end; the first
 // word's fields and offset go to %[sel] and %[ntg], %[j] = 0.
@@ -223,6 +224,11 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  v_cmp_eq_u32_e32 vcc, v[RA + 4], v[RA + 2]\n"                                          \
     "  s_nop 1\n"                                                                             \
     "  v_cndmask_b32_e64 v[RA + 6], v[RA + 6], 1, vcc\n"                                      \
+    "  v_cmp_eq_u32_e32 vcc, v[RA + 2], v[RA + 3]\n"                                          \
+    "  s_nop 1\n"                                                                             \
+    "  v_cndmask_b32_e64 v[RA + 3], 0, 2, vcc\n"                                              \
+    "  v_lshlrev_b32_e32 v[RA + 4], 1, v[RA + 6]\n"                                           \
+    "  v_and_or_b32 v[RA + 6], v[RA + 3], v[RA + 4], v[RA + 6]\n"                             \
     "  v_and_b32_e32 v[RA + 7], 15, v[WB + 0]\n"                                              \
     "  v_lshlrev_b32_e32 v[RA + 2], 1, v[RA + 7]\n"                                           \
     "  s_mov_b32 %[tmp], 0x1100fc0\n"                                                         \
@@ -271,7 +277,8 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     "  .endr\n"                                                                               \
     "  s_set_gpr_idx_off\n"
 // Binary op with variants: 0 RB = v[b] OP RA (RA = a), moved to the output
-// slot; 1 in place on a (B copied to RB); 2 in place on b (A copied to RA).
+// slot; 1 in place on a (B copied to RB); 2 in place on b (A copied to RA);
+// 3 a, b and the output one slot (x OP x in place, no copy).
 // OPC_GEN / OPC_A / OPC_B are the opcodes of the three forms.
 #define FREP_ASM_BINARY(OP, OPC_GEN, OPC_A, OPC_B)                                            \
     FREP_ASM_SLOT(OP, 0)                                                                      \
@@ -295,6 +302,14 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     FREP_ASM_READ_A("RA")                                                                     \
     FREP_ASM_IDX_B                                                                            \
     FREP_ASM_INPLACE(OPC_B, "%[tmp]", "RA")                                                   \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(OP, 3)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    "  s_set_gpr_idx_on %[sel], gpr_idx(SRC0,SRC1,DST)\n"                                     \
+    "  .irp k, 0,1,2,3,4,5,6,7\n"                                                             \
+    "    " OPC_A " v[FS + \\k], v[FS + \\k], v[FS + \\k]\n"                                 \
+    "  .endr\n"                                                                               \
+    "  s_set_gpr_idx_off\n"                                                                   \
     FREP_ASM_GO
 // min(A,B) = B < A ? B : A (LT_X < LT_Y selects RB); four compares into four
 // SGPR pairs, then selects, the result in RB moved to the output slot.
@@ -566,6 +581,12 @@ __host__ __device__ constexpr bool asm_op_ok(uint32_t op) {
     FREP_ASM1_READ_A("RA")                                                                    \
     FREP_ASM_IDX_B                                                                            \
     FREP_ASM1_INPLACE(OPC_B, "%[tmp]", "RA")                                                  \
+    FREP_ASM_GO                                                                               \
+    FREP_ASM_SLOT(OP, 3)                                                                      \
+    FREP_ASM_NEXT                                                                             \
+    "  s_set_gpr_idx_on %[sel], gpr_idx(SRC0,SRC1,DST)\n"                                     \
+    "  " OPC_A " v[FS], v[FS], v[FS]\n"                                                       \
+    "  s_set_gpr_idx_off\n"                                                                   \
     FREP_ASM_GO
 #define FREP_ASM1_MINMAX(LT_X, LT_Y)                                                          \
     FREP_ASM_NEXT                                                                             \

@@ -140,6 +140,31 @@ def test_fused_const_handlers_bit_exact(ctx, oracle):
         ctx.tape_free(th)
 
 
+def test_asm_sqrt_special_inputs_bit_exact(ctx, oracle):
+    """SQRT on the assembly interpreter over normal inputs, and over inputs
+    mixed with tiny, denormal, zero, negative, infinite and NaN ones (the
+    scaling below 2^-96 and the class fix-ups): equal to the oracle (IEEE
+    sqrt) bit for bit."""
+    from frep.tape import tape_from_words
+    words = [_word(11, 0, 0, 0)]                   # slot 0 = sqrt(x)
+    tape = tape_from_words(np.array([1] + words, dtype=np.uint32),
+                           np.array([0.0], dtype=np.float32), 8)
+    rng = np.random.default_rng(9)
+    normal = rng.uniform(2.0 ** -90, 4.0, 1024).astype(np.float32)
+    special = np.array([0.0, -0.0, 1e-30, 2.0 ** -97, 2.0 ** -96, 1e-40, 1e-45, -1.0,
+                        np.inf, -np.inf, np.nan, 3.0e38], dtype=np.float32)
+    mixed = normal.copy()
+    mixed[::85][:len(special)] = special
+    th = ctx.upload(tape)
+    for xs in (normal, mixed):
+        p = np.zeros((len(xs), 3), dtype=np.float32)
+        p[:, 0] = xs
+        got = ctx.eval_points(th, p, 0.0)
+        ref = np.array([oracle.eval_point(tape, *q, 0.0) for q in p], dtype=np.float32)
+        assert same_bits(got, ref)
+    ctx.tape_free(th)
+
+
 @pytest.mark.parametrize("dims", [[4, 4, 8], [4, 8, 4]])
 def test_fused_const_trees_match(ctx, oracle, dims):
     """The same tapes voxelized into three-level trees whose last level runs
